@@ -1,0 +1,195 @@
+/*
+ * pf_capi.h -- C-ABI of the MI355X (gfx950) coupled phase-field time step.
+ *
+ * The reference (ileaof/phase-field-project-3D) runs its hot path as five
+ * argument-less Fortran subroutines that talk through the module `variaveis`
+ * (source/variaveis.f90:1-50) and keep the seven fields resident on the device
+ * with an OpenMP data environment (source/principal.f90:60-64).  This header is
+ * what replaces them: every entry point below names the reference subroutine or
+ * loop it stands in for.  A Fortran caller binds it with ISO_C_BINDING (see the
+ * module pf_capi in phase-field-project-3d_amd/fortran/pf_capi.f90); plain
+ * pointers and sizes only, no C++ or torch types.
+ *
+ * Arrays are Fortran column-major (1:nx,1:ny,1:nz): the C offset of cell
+ * (i,j,k) is (i-1) + (j-1)*nx + (k-1)*nx*ny.  Host arrays are always FP64;
+ * the FP32-mixed form converts at upload/download.
+ *
+ * Conventions
+ *   - Every call returns PF_OK (0) or a negative PF_E_* code; pf_last_error()
+ *     gives a message for the calling thread's last failure.
+ *   - One host thread per context; a context is not thread-safe.
+ *   - All device work of a context goes to one HIP stream, in call order.
+ *     Calls return without waiting, except the ones documented as synchronous
+ *     (pf_diagnosticos, pf_download*, pf_probe, pf_finalize, pf_synchronize).
+ */
+#ifndef PF_CAPI_H
+#define PF_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PF_OK 0
+#define PF_E_ARG (-1)   /* bad argument (null pointer, grid too small, ...)  */
+#define PF_E_HIP (-2)   /* a HIP runtime call failed                          */
+#define PF_E_NOMEM (-3) /* device allocation failed / over the HBM budget     */
+#define PF_E_STATE (-4) /* call not valid in the context's current state      */
+#define PF_E_NODEV (-5) /* no usable gfx950 device                            */
+
+/* ---- precision forms --------------------------------------------------- */
+#define PF_FP64 0       /* FP64 storage and arithmetic (the drop-in form)     */
+#define PF_FP32_MIXED 1 /* FP32 fields + stencils; FP64 reductions/scalars    */
+
+/* ---- noise forms (source/phase_body.inc:181-190) ------------------------ */
+#define PF_RUIDO_CELL_POSITIVE 0 /* ran*AAA*16 phi^2 (1-phi)^2              */
+#define PF_RUIDO_CELL_ZEROMEAN 1 /* (2 ran - 1)*AAA*16 phi^2 (1-phi)^2      */
+#define PF_RUIDO_SMOOTH 2        /* ruido_suave (trilinear coarse lattice)   */
+
+/*
+ * Kernel-visible scalars: the subset of module `variaveis` that the field
+ * kernels read (source/variaveis.f90:3-40).  POD, passed by pointer.
+ * Ke is REAL(4) in the reference (variaveis.f90:28) and is kept as float.
+ */
+typedef struct pf_params {
+  int32_t nx, ny, nz;   /* grid, each >= 5                                   */
+  int32_t it;           /* current step, 1-based (principal.f90:93)          */
+  int32_t n_varreduras; /* phase sweeps per step (entrada.f90:75-80)         */
+  int32_t h_quintico;   /* 1 quintic h(phi), 0 cubic (phase_body.inc:159)    */
+  int32_t ruido_tipo;   /* PF_RUIDO_*                                        */
+  int32_t precision;    /* PF_FP64 or PF_FP32_MIXED                          */
+  uint64_t seed;        /* noise seed; 0 = the reference's stream bit-exact  */
+  double dx, dy, dz;    /* mesh spacing [m]                                  */
+  double dt;            /* DELTA_T = dx^2/(7.5 D_liq) (entrada.f90:173)      */
+  double D;             /* diffusivity in temperatura; the reference never
+                           assigns it, so it is 0 there (variaveis.f90:3)     */
+  double delta_H, cp_s; /* latent heat, solid heat capacity                  */
+  double TP;            /* bath temperature seen by the kernels              */
+  double Tm, Me, Vm, R; /* melting point, liquidus slope, molar vol., gas c. */
+  double E0, W0, M0;    /* gradient coeff., barrier height, mobility         */
+  double delta_eps;     /* DELTA_EPSLON, anisotropy strength                 */
+  double AAA;           /* noise amplitude after the dt rescale              */
+  double D_liq, D_sol;  /* solute diffusivities                              */
+  double Cs_eq, Cl_eq;  /* informational: recomputed from TP by the phase call */
+  float Ke;             /* partition coefficient, REAL(4)                    */
+  int32_t reserved;     /* keep 8-byte size alignment; set to 0              */
+} pf_params;
+
+/*
+ * Host loop state that principal.f90 keeps between kernel calls
+ * (principal.f90:93-259): what pf_run_steps needs to advance TP, M0 and the
+ * REAL(4) time accumulator on its own.
+ */
+typedef struct pf_loop {
+  double T0;         /* initial temperature (entrada.dat line 5)             */
+  double taxa_resfr; /* cooling rate [K/s]; 0 keeps TP fixed                 */
+  float Sigma;       /* interface energy, REAL(4)                            */
+  float Beta;        /* kinetic coefficient, REAL(4)                         */
+  float ajuste;      /* mobility/E0 fit factor, REAL(4)                      */
+  float time;        /* simulated time, REAL(4) accumulator (principal.f90:96) */
+} pf_loop;
+
+typedef struct pf_ctx pf_ctx; /* opaque: device fields, stream, scratch      */
+
+/* ---- library ------------------------------------------------------------ */
+const char *pf_version(void);
+const char *pf_last_error(void);
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int pf_device_count(int32_t *n);
+
+/* ---- host scalar physics (bit-exact with the reference's host code) ------- */
+/* source/variaveis.f90:73-92 (+ a seed term that is 0 for seed=0). */
+double pf_pseudo_random(int32_t i, int32_t j, int32_t k, int32_t it, int32_t salt,
+                        uint64_t seed);
+/* source/variaveis.f90:98-111 */
+double pf_ruido_suave(int32_t i, int32_t j, int32_t k, int32_t it, uint64_t seed);
+/* source/W0_E0.f90:1-35: the REAL(4)-stepped search; returns the number of
+ * loop increments taken (0-based index of the accepted try) or -1 when the
+ * loop ran out without meeting the tolerance.  E0/W0 hold the last values. */
+int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E0);
+/* source/auxiliar.f90:1-49: thin-interface mobility M0(TP). */
+double pf_auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm,
+                      double E0, double W0, float Sigma, float Beta, float Ke,
+                      double Me, double D_liq, float ajuste);
+/* source/equacao_campo_fase.f90:21-22 */
+void pf_equilibrium(double TP, double Tm, float Ke, double Me, double *Cs_eq,
+                    double *Cl_eq);
+/* source/entrada.f90:173: DELTA_T = dx^2/(7.5 D_liq) (the deck's dt is unused). */
+double pf_time_step(double dx, double D_liq);
+/* source/entrada.f90:179-185: AAA rescaled to the 2-D code's dt, linearly for
+ * ruido_tipo 0 and by sqrt otherwise. */
+double pf_noise_scale(double AAA, double dt, double dx, double D_liq,
+                      int32_t ruido_tipo);
+/* source/entrada.f90:226-262: initial fields on the host.  tipo 0 planar
+ * front j<=4, 1 sphere at (nx/2,ny/2,nz/2), 2 sphere at (3,3,3), 3 planar
+ * base k<=4; new modes: 5 sphere at the bottom centre (nx/2,ny/2,1), 6
+ * n_seeds spheres centred on k=1 at SplitMix64(seed_rng) positions in
+ * [5,n-4].  Solid (dist^2 <= R^2): phi=1, c=Ke*C0; liquid: phi=0, c=C0;
+ * phi_new=0, c_new=C0, T=T0, DPP=0. */
+int pf_initial_fields(int32_t nx, int32_t ny, int32_t nz, int32_t tipo,
+                      double R_semente, double T0, double C0, float Ke,
+                      int32_t n_seeds, uint64_t seed_rng, double *phi_old,
+                      double *phi_new, double *c_old, double *c_new,
+                      double *T_old, double *T_new, double *dpp);
+
+/* ---- context: source/principal.f90:60-64 (target enter data) ------------ */
+/* Allocates the seven fields on the current device and uploads them.  Any
+ * host pointer may be NULL, which leaves that device field zero-filled. */
+int pf_init(const pf_params *p, const double *phi_old, const double *phi_new,
+            const double *c_old, const double *c_new, const double *T_old,
+            const double *T_new, const double *dpp, pf_ctx **out);
+/* source/principal.f90:268-272 (target exit data). Synchronous. */
+int pf_finalize(pf_ctx *ctx);
+
+/* ---- the drop-in kernel calls (one per reference subroutine) ------------- */
+/* source/temperatura.f90:1-80 + temp_body.inc. */
+int pf_temperatura(pf_ctx *ctx, const pf_params *p);
+/* source/equacao_campo_fase.f90:1-179 + phase_body.inc: one phase sweep
+ * (interior update, clamp, DPP, the 12-plane boundary decoration).  If probe
+ * is non-NULL the call is synchronous and fills probe[0..2] with the values
+ * the reference prints (equacao_campo_fase.f90:174): phi_new at
+ * (nx/2,10,max(nz/2,1)), TP, phi_new at (nx/2,min(100,ny-1),max(nz/2,1)). */
+int pf_equacao_campo_fase(pf_ctx *ctx, const pf_params *p, double probe[3]);
+/* source/principal.f90:127-150: phase_field_old <- phase_field_new between
+ * sweeps (a buffer swap, no copy). */
+int pf_phase_new_to_old(pf_ctx *ctx);
+/* source/concentracao_soluto.f90:1-130 + conc_body.inc. */
+int pf_concentracao_soluto(pf_ctx *ctx, const pf_params *p);
+/* source/principal.f90:200-226: old <- new for phi, T and c (buffer swaps). */
+int pf_end_step(pf_ctx *ctx);
+/* source/diagnosticos.f90:1-58 + diag_body.inc over phase_field_new and
+ * conc_soluto_new: out = {L2(phi), max phi, min phi, sum c, free energy}.
+ * Deterministic fixed-order FP64 reduction.  Synchronous. */
+int pf_diagnosticos(pf_ctx *ctx, const pf_params *p, double out[5]);
+/* source/principal.f90:182,248 (target update from).  Any pointer may be
+ * NULL.  Synchronous. */
+int pf_download(pf_ctx *ctx, double *phi_new, double *c_new, double *T_new);
+/* All seven fields in the reference's naming (for checkpoints and tests). */
+int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old,
+                    double *c_new, double *T_old, double *T_new, double *dpp);
+
+/* ---- fused fast path ------------------------------------------------------ */
+/* Runs nsteps iterations of the reference's time loop body
+ * (principal.f90:96-226 without the output event): temperatura, the TP
+ * update, n_varreduras phase sweeps, concentracao_soluto, auxiliar and the
+ * end-of-step swap.  p->it, p->TP, p->M0, p->Cs_eq, p->Cl_eq and L->time are
+ * advanced in place to the values the reference holds after the last step.
+ * The sweeps are fused (phase sweep n_varreduras + solute + the next step's
+ * temperature in one pass); the state left behind is the reference's.
+ * If probes is non-NULL it receives 3 doubles per phase sweep (the values
+ * pf_equacao_campo_fase reports), n_varreduras*nsteps*3 in total, and the
+ * call is synchronous. */
+int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps,
+                 double *probes);
+/* Wait for all queued work of the context. */
+int pf_synchronize(pf_ctx *ctx);
+/* Bytes of device memory the context holds. */
+int pf_device_bytes(const pf_ctx *ctx, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PF_CAPI_H */

@@ -1,0 +1,236 @@
+"""ctypes binding of the C-ABI (include/pf_capi.h) exported by libpfamd.so.
+
+This is the Python mirror of the Fortran ISO_C_BINDING module
+(fortran/pf_capi.f90): the same entry points, the same POD structs.  The
+library path is the in-tree build (lib/libpfamd.so); loading fails loudly if
+it is missing -- there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libpfamd.so")
+
+PF_OK, PF_E_ARG, PF_E_HIP, PF_E_NOMEM, PF_E_STATE, PF_E_NODEV = 0, -1, -2, -3, -4, -5
+PF_FP64, PF_FP32_MIXED = 0, 1
+
+
+class PfParams(C.Structure):
+    """struct pf_params (include/pf_capi.h)."""
+
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("it", C.c_int32),
+        ("n_varreduras", C.c_int32), ("h_quintico", C.c_int32), ("ruido_tipo", C.c_int32),
+        ("precision", C.c_int32), ("seed", C.c_uint64),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("D", C.c_double), ("delta_H", C.c_double), ("cp_s", C.c_double), ("TP", C.c_double),
+        ("Tm", C.c_double), ("Me", C.c_double), ("Vm", C.c_double), ("R", C.c_double),
+        ("E0", C.c_double), ("W0", C.c_double), ("M0", C.c_double), ("delta_eps", C.c_double),
+        ("AAA", C.c_double), ("D_liq", C.c_double), ("D_sol", C.c_double),
+        ("Cs_eq", C.c_double), ("Cl_eq", C.c_double), ("Ke", C.c_float), ("reserved", C.c_int32),
+    ]
+
+    def copy(self) -> "PfParams":
+        q = PfParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(PfParams))
+        return q
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PfLoop(C.Structure):
+    """struct pf_loop (include/pf_capi.h)."""
+
+    _fields_ = [("T0", C.c_double), ("taxa_resfr", C.c_double), ("Sigma", C.c_float),
+                ("Beta", C.c_float), ("ajuste", C.c_float), ("time", C.c_float)]
+
+    def copy(self) -> "PfLoop":
+        q = PfLoop()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(PfLoop))
+        return q
+
+
+class PfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pf error {code}: {msg}")
+        self.code = code
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+# Every symbol declared in include/pf_capi.h (tests check the library exports them).
+EXPORTS = (
+    "pf_version", "pf_last_error", "pf_device_count", "pf_pseudo_random", "pf_ruido_suave",
+    "pf_w0_e0", "pf_auxiliar_m0", "pf_equilibrium", "pf_time_step", "pf_noise_scale",
+    "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
+    "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
+    "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
+)
+
+
+def load(path: str = LIB_PATH):
+    """Load libpfamd.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    i32, u64, f64, f32, vp = C.c_int32, C.c_uint64, C.c_double, C.c_float, C.c_void_p
+    P, L = C.POINTER(PfParams), C.POINTER(PfLoop)
+    sig = {
+        "pf_version": ([], C.c_char_p),
+        "pf_last_error": ([], C.c_char_p),
+        "pf_device_count": ([C.POINTER(i32)], C.c_int),
+        "pf_pseudo_random": ([i32, i32, i32, i32, i32, u64], f64),
+        "pf_ruido_suave": ([i32, i32, i32, i32, u64], f64),
+        "pf_w0_e0": ([f64, f32, f32, C.POINTER(f64), C.POINTER(f64)], i32),
+        "pf_auxiliar_m0": ([f64, f64, f64, f64, f64, f64, f64, f32, f32, f32, f64, f64, f32], f64),
+        "pf_equilibrium": ([f64, f64, f32, f64, C.POINTER(f64), C.POINTER(f64)], None),
+        "pf_time_step": ([f64, f64], f64),
+        "pf_noise_scale": ([f64, f64, f64, f64, i32], f64),
+        "pf_initial_fields": ([i32, i32, i32, i32, f64, f64, f64, f32, i32, u64] + [_dp] * 7, C.c_int),
+        "pf_init": ([P] + [vp] * 7 + [C.POINTER(vp)], C.c_int),
+        "pf_finalize": ([vp], C.c_int),
+        "pf_temperatura": ([vp, P], C.c_int),
+        "pf_equacao_campo_fase": ([vp, P, vp], C.c_int),
+        "pf_phase_new_to_old": ([vp], C.c_int),
+        "pf_concentracao_soluto": ([vp, P], C.c_int),
+        "pf_end_step": ([vp], C.c_int),
+        "pf_diagnosticos": ([vp, P, vp], C.c_int),
+        "pf_download": ([vp, vp, vp, vp], C.c_int),
+        "pf_download_all": ([vp] + [vp] * 7, C.c_int),
+        "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
+        "pf_synchronize": ([vp], C.c_int),
+        "pf_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def check(rc: int, lib=None):
+    if rc != PF_OK:
+        lib = lib or load()
+        raise PfError(rc, (lib.pf_last_error() or b"").decode())
+    return rc
+
+
+def device_count(lib=None) -> int:
+    lib = lib or load()
+    n = C.c_int32(0)
+    check(lib.pf_device_count(C.byref(n)), lib)
+    return n.value
+
+
+def w0_e0(lam, ajuste, sigma, lib=None):
+    lib = lib or load()
+    W0, E0 = C.c_double(), C.c_double()
+    idx = lib.pf_w0_e0(lam, ajuste, sigma, C.byref(W0), C.byref(E0))
+    return W0.value, E0.value, idx
+
+
+def equilibrium(TP, Tm, Ke, Me, lib=None):
+    lib = lib or load()
+    cs, cl = C.c_double(), C.c_double()
+    lib.pf_equilibrium(TP, Tm, Ke, Me, C.byref(cs), C.byref(cl))
+    return cs.value, cl.value
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """Owns one pf_ctx: the seven device-resident fields of one instance."""
+
+    def __init__(self, p: PfParams, fields, lib=None):
+        self.lib = lib or load()
+        self.n = p.nx * p.ny * p.nz
+        arrs = [None if f is None else np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+                for f in fields]
+        for a in arrs:
+            if a is not None and a.size != self.n:
+                raise ValueError("field size does not match the grid")
+        h = C.c_void_p()
+        check(self.lib.pf_init(C.byref(p), *[_ptr(a) for a in arrs], C.byref(h)), self.lib)
+        self.h = h
+
+    # -- drop-in calls -----------------------------------------------------
+    def temperatura(self, p):
+        check(self.lib.pf_temperatura(self.h, C.byref(p)), self.lib)
+
+    def equacao_campo_fase(self, p, want_probe=False):
+        pr = (C.c_double * 3)()
+        check(self.lib.pf_equacao_campo_fase(self.h, C.byref(p),
+                                             C.cast(pr, C.c_void_p) if want_probe else None), self.lib)
+        return list(pr) if want_probe else None
+
+    def phase_new_to_old(self):
+        check(self.lib.pf_phase_new_to_old(self.h), self.lib)
+
+    def concentracao_soluto(self, p):
+        check(self.lib.pf_concentracao_soluto(self.h, C.byref(p)), self.lib)
+
+    def end_step(self):
+        check(self.lib.pf_end_step(self.h), self.lib)
+
+    def diagnosticos(self, p):
+        out = np.zeros(5)
+        check(self.lib.pf_diagnosticos(self.h, C.byref(p), _ptr(out)), self.lib)
+        return out
+
+    def download(self):
+        phi, c, T = np.empty(self.n), np.empty(self.n), np.empty(self.n)
+        check(self.lib.pf_download(self.h, _ptr(phi), _ptr(c), _ptr(T)), self.lib)
+        return phi, c, T
+
+    def download_all(self):
+        arrs = [np.empty(self.n) for _ in range(7)]
+        check(self.lib.pf_download_all(self.h, *[_ptr(a) for a in arrs]), self.lib)
+        return arrs
+
+    # -- fused path ----------------------------------------------------------
+    def run_steps(self, p, L, nsteps, probes=False):
+        pr = np.zeros(3 * max(1, p.n_varreduras) * nsteps) if probes else None
+        check(self.lib.pf_run_steps(self.h, C.byref(p), C.byref(L), int(nsteps), _ptr(pr)), self.lib)
+        return pr
+
+    def synchronize(self):
+        check(self.lib.pf_synchronize(self.h), self.lib)
+
+    def device_bytes(self):
+        b = C.c_size_t(0)
+        check(self.lib.pf_device_bytes(self.h, C.byref(b)), self.lib)
+        return b.value
+
+    def close(self):
+        if self.h:
+            check(self.lib.pf_finalize(self.h), self.lib)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.pf_finalize(self.h)
+                self.h = None
+        except Exception:
+            pass
