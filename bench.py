@@ -1,0 +1,223 @@
+"""Benchmark of the coupled phase-field time step on one MI355X.
+
+Workload (the columnar-growth configuration): 256 x 256 x 528 cells, Al-Cu
+(run6 deck parameters: c0 = 0.0196 = 4.5 wt% Cu, T0 = 922 K, cooling 19.84
+K/s), 16 seeds of radius 4 on the z = 1 plane at SplitMix64(20261203)
+positions, zero-mean cell noise with effective amplitude 0.01 (deck
+0.0122474) from the counter-based stream with seed 1, two phase sweeps per
+step, cubic h(phi), diagnostics every 1000 steps (inside the timed region).
+
+One "step" is one reference time step (two phase sweeps, solute,
+temperature, mobility update) over the whole grid with the fields resident
+in HBM.  Prints one JSON line: MLUPS of the FP64 drop-in form (value), the
+FP32-mixed form beside it, the roofline of the dominant kernel from HIP
+events, and the reference's own CPU path timed on this host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "phase-field-project-3d_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+NX, NY, NZ = 256, 256, 528
+N_SEEDS, SEED_RNG, NOISE_SEED = 16, 20261203, 1
+DIAG_EVERY = 1000
+HBM_PEAK_GBS = 8000.0
+# Compulsory HBM bytes per cell of each fused kernel (FP64; half for FP32):
+#   sweep A reads phi, c and writes phi                       -> 3 fields
+#   sweep B reads phi, c and writes phi, c, T (T is not read:
+#   the temperature update's Laplacian has D = 0)              -> 5 fields
+FIELDS_PER_CELL = {"sweep_A": 3, "sweep_B": 5}
+
+
+def workload_deck():
+    from pfamd.deck import Deck
+
+    return Deck(Nx=NX, Ny=NY, Nz=NZ, tipo_inicial=6, R_semente=4.0, h_quintico=0, ruido_tipo=1,
+                AAA=0.0122474, taxa_resfr=19.84, n_varreduras=2, stoptime=0.0,
+                WriteFileD_timeCounter=DIAG_EVERY)
+
+
+def _sync(sim):
+    sim.ctx.synchronize()
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        pass
+
+
+def run_form(precision, steps, warmup, timing=True):
+    """Warm-up, then exactly `steps` timed steps; returns timings."""
+    from pfamd.run import Simulation
+
+    deck = workload_deck()
+    t0 = time.time()
+    sim = Simulation(deck, precision=precision, seed=NOISE_SEED, n_seeds=N_SEEDS,
+                     seed_rng=SEED_RNG)
+    t_init = time.time() - t0
+    if warmup:
+        sim.run(warmup)
+    _sync(sim)
+    sim.ctx.timing(timing)
+    done = 0
+    t0 = time.perf_counter()
+    while done < steps:
+        chunk = min(DIAG_EVERY - (sim.p.it % DIAG_EVERY), steps - done)
+        sim.run(chunk)
+        done += chunk
+        if sim.p.it % DIAG_EVERY == 0:
+            sim.diagnostics()  # output-event reduction (synchronous)
+    _sync(sim)
+    wall = time.perf_counter() - t0
+    kt = sim.ctx.timing_read()
+    sim.ctx.timing(False)
+    diag = sim.diagnostics()
+    nbytes = sim.ctx.device_bytes()
+    sim.close()
+    return {"wall_s": wall, "kernels": kt, "diag": diag.tolist(), "init_s": t_init,
+            "device_bytes": nbytes}
+
+
+def roofline(res, precision_bytes):
+    ncell = NX * NY * NZ
+    best = None
+    for nm, nf in FIELDS_PER_CELL.items():
+        ms, cnt = res["kernels"].get(nm, (0.0, 0))
+        if cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        alg = nf * precision_bytes * ncell
+        entry = {"kernel": nm, "avg_ms": avg_s * 1e3, "alg_bytes": alg,
+                 "achieved_gbs": alg / avg_s / 1e9, "share": ms}
+        if best is None or ms > best["share"]:
+            best = entry
+    return best
+
+
+def cpu_baseline():
+    """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
+    kernels compiled from source with flang -O2 -fopenmp), timed on this
+    host's cores on 5 steps of the same grid after 1 warm-up step.  Falls
+    back to the C oracle (a faithful restatement) when the reference build
+    is absent."""
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(16, threads))
+    nwarm, nsteps = 1, 5
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    if os.path.exists(exe):
+        wd = tempfile.mkdtemp(prefix="pfcpu_")
+        try:
+            d = workload_deck()
+            d.tipo_inicial = 3  # the reference has no seeded mode; cost is data-independent
+            d.write(os.path.join(wd, "entrada.dat"))
+            with open(os.path.join(wd, "harness.nml"), "w") as fh:
+                fh.write(f"&harness\n mode='time', nsteps={nsteps}, nwarm={nwarm}\n/\n")
+            subprocess.run([exe], cwd=wd, env=env, stdout=subprocess.DEVNULL, check=True,
+                           timeout=600)
+            t = float(open(os.path.join(wd, "time.txt")).read().split()[0])
+        finally:
+            shutil.rmtree(wd, ignore_errors=True)
+        kind = "reference"
+    else:
+        import numpy as np
+
+        import oracle as O
+        from pfamd.run import setup_from_deck
+
+        d = workload_deck()
+        s = setup_from_deck(d)
+        O.set_threads(threads)
+        F = O.initial_fields(NX, NY, NZ, 3, 0.0, 922.0, d.C0, d.Ke)
+        O.run_steps(s.p, s.L, nwarm, F)
+        t0 = time.perf_counter()
+        O.run_steps(s.p, s.L, nsteps, F)
+        t = time.perf_counter() - t0
+        kind = "port"
+        _ = np
+    mlups = NX * NY * NZ * nsteps / t / 1e6
+    return {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
+            "sample": f"{nsteps} steps of the {NX}x{NY}x{NZ} FP64 step after {nwarm} warm-up, "
+                      f"OMP_NUM_THREADS={threads}, {t:.2f} s"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (instances of this "
+              "solver shard across GPUs with no communication, see DESIGN.md)", file=sys.stderr)
+        return 2
+    from pfamd import capi
+
+    if capi.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        return 1
+    print(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps", flush=True)
+    r64 = run_form(capi.PF_FP64, a.steps, a.warmup)
+    ncell = NX * NY * NZ
+    mlups = ncell * a.steps / r64["wall_s"] / 1e6
+    print(f"[bench] FP64 {mlups:.1f} MLUPS, kernels {r64['kernels']}", flush=True)
+    rl = roofline(r64, 8)
+    out = {
+        "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": r64["wall_s"] / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "columnar growth 256x256x528, 16 seeds r=4, noise 0.01, "
+                               "2 sweeps/step, diagnostics every 1000 steps",
+                   "grid": [NX, NY, NZ], "cells": ncell, "form": "FP64 drop-in (fused sweeps)"},
+        "roofline": {"bound": "hbm", "achieved": round(rl["achieved_gbs"], 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
+                     "alg_bytes_per_launch": rl["alg_bytes"]},
+        "step_bytes_per_cell_alg": 64,
+        "kernels_ms_total": {k: round(v[0], 3) for k, v in r64["kernels"].items()},
+        "kernels_launches": {k: v[1] for k, v in r64["kernels"].items()},
+        "whole_step_alg_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
+        "diag_final": r64["diag"],
+    }
+    if not a.no_fp32:
+        print("[bench] FP32-mixed", flush=True)
+        r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup)
+        m32 = ncell * a.steps / r32["wall_s"] / 1e6
+        rl32 = roofline(r32, 4)
+        out["fp32_mixed"] = {"value": round(m32, 2), "unit": "MLUPS",
+                             "ms_per_step": r32["wall_s"] / a.steps * 1e3,
+                             "speedup_vs_fp64": round(m32 / mlups, 3),
+                             "roofline_kernel": rl32["kernel"],
+                             "roofline_achieved_gbs": round(rl32["achieved_gbs"], 1),
+                             "diag_final": r32["diag"]}
+    if not a.no_cpu_baseline:
+        print("[bench] CPU baseline (reference OpenMP path)", flush=True)
+        cb = cpu_baseline()
+        out["cpu_baseline"] = cb
+        out["gpu_vs_cpu"] = round(mlups / cb["value"], 1)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -184,8 +184,27 @@ int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old
  * call is synchronous. */
 int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps,
                  double *probes);
+/* on = 0 makes pf_run_steps issue the drop-in kernels one reference call at
+ * a time instead of the fused sweeps (default 1). */
+int pf_set_fused(pf_ctx *ctx, int32_t on);
 /* Wait for all queued work of the context. */
 int pf_synchronize(pf_ctx *ctx);
+
+/* ---- measurement ----------------------------------------------------------
+ * Per-kernel durations from HIP events recorded around each launch on the
+ * context's stream (no counterpart in the reference, whose only timer is the
+ * host wall clock at output events, principal.f90:308-408). */
+#define PF_KT_SWEEP_A 0     /* fused phase sweep (all but the last sweep)   */
+#define PF_KT_SWEEP_B 1     /* fused last sweep + solute + next temperature */
+#define PF_KT_BOUNDARY 2    /* boundary decorations                         */
+#define PF_KT_TEMPERATURA 3 /* standalone temperatura                       */
+#define PF_KT_DROPIN 4      /* drop-in phase / solute kernels               */
+#define PF_KT_COUNT 5
+/* on != 0 starts recording (and clears earlier records); 0 stops. */
+int pf_timing_enable(pf_ctx *ctx, int32_t on);
+/* Synchronous: sums of durations [ms] and launch counts per kind since the
+ * last enable; clears the records. */
+int pf_timing_read(pf_ctx *ctx, double ms[PF_KT_COUNT], int64_t count[PF_KT_COUNT]);
 /* Bytes of device memory the context holds. */
 int pf_device_bytes(const pf_ctx *ctx, size_t *bytes);
 

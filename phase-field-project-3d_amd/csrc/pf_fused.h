@@ -1,0 +1,74 @@
+// pf_fused.h -- the fused time step (internal to libpfamd).
+//
+// One reference step with n_varreduras = nv becomes
+//   [temperatura]            only on the first step of a pf_run_steps call
+//   (nv-1) x sweep A         phase sweep: phi_old, c_old -> phi_new
+//   1 x sweep B              last phase sweep + solute + the next step's
+//                            temperature (or, on the last step of the call,
+//                            DPP so that the reference state is complete)
+//   boundary decorations     small kernels over the domain shell
+// Buffer roles are swapped in place of the reference's copies.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "pf_common.h"
+
+namespace pf {
+
+struct FusedState {
+  int nx = 0, ny = 0, nz = 0, prec = PF_FP64;
+  int seg_a = 0, seg_b = 0;  // z-planes per work-group segment (autotuned)
+  bool enabled = false;
+};
+
+// HIP-event pairs recorded around launches on one stream, by kernel kind.
+struct KTimer {
+  bool on = false;
+  std::vector<hipEvent_t> pool;  // 2 events per record
+  std::vector<int> kind;         // kind of record r (events 2r, 2r+1)
+  void begin(int k, hipStream_t s) {
+    if (!on) return;
+    const size_t r = kind.size();
+    while (pool.size() < 2 * (r + 1)) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      pool.push_back(e);
+    }
+    kind.push_back(k);
+    hipEventRecord(pool[2 * r], s);
+  }
+  void end(hipStream_t s) {
+    if (!on || kind.empty()) return;
+    hipEventRecord(pool[2 * (kind.size() - 1) + 1], s);
+  }
+  void release() {
+    for (hipEvent_t e : pool) hipEventDestroy(e);
+    pool.clear();
+    kind.clear();
+  }
+};
+
+struct FusedStep {
+  bool first = false;  // run temperatura (T of this step) before the sweeps
+  bool last = false;   // materialise DPP instead of the next step's T
+  int nv = 1;          // phase sweeps per step
+};
+
+int fused_init(FusedState &s, int nx, int ny, int nz, int prec);
+void fused_free(FusedState &s);
+size_t fused_bytes(const FusedState &s);
+bool fused_supported(const FusedState &s, const pf_params &p);
+
+// Advances one step.  The *_old / *_alias arguments are the context's
+// old/new bookkeeping of each field pair (see pf_runtime.hip); they are
+// updated to the reference's state after the end-of-step copy.  kt are the
+// scalars temperatura sees (TP of the previous step), k those of the sweeps.
+int fused_step(FusedState &s, const KParams &kt, const KParams &k, const FusedStep &fs,
+               void *phi[2], int *phi_old, bool *phi_alias, void *c[2], int *c_old,
+               bool *c_alias, void *T[2], int *T_old, bool *T_alias, void *dpp, double *probe,
+               hipStream_t stream, KTimer &timer);
+
+}  // namespace pf

@@ -1,0 +1,532 @@
+// pf_fused.hip -- fused 2.5D z-march sweeps for gfx950.
+//
+// A work-group owns a TX x TY column tile (x fastest, 64 = one wavefront per
+// row) and a segment of z-planes, and marches up the segment:
+//   - phi planes (tile + 2-cell halo) rotate through 5 LDS slots, loaded once
+//     from HBM with x-contiguous (coalesced) reads;
+//   - the anisotropy quantities of a phase cell's six neighbours
+//     (phase_body.inc:55-129) are the centred-gradient quantities of those
+//     neighbour nodes, so each node's eps^2, exy, eyz, ezx is computed once
+//     (tile + 1-cell ring), shared in-plane through LDS and along z in
+//     registers -- instead of six times per cell;
+//   - sweep B also evaluates the solute (conc_body.inc) with the per-node
+//     1/gamma and mu = ln(CL/(1-CL)) computed once per node, and the
+//     temperature of the next step (temp_body.inc) from the fresh DPP, which
+//     therefore never goes to HBM except on the last step of a call.
+// Compulsory HBM traffic per cell: sweep A reads phi, c and writes phi;
+// sweep B reads phi, c and writes phi, c, T.
+//
+// Arithmetic follows the reference's formulas with algebraic reuse
+// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g')), so
+// results agree with the reference to rounding, not bit for bit; the drop-in
+// kernels keep the bit-faithful form.
+#include <hip/hip_runtime.h>
+
+#include "pf_common.h"
+#include "pf_fused.h"
+#include "pf_kernels.h"
+
+namespace pf {
+namespace {
+
+constexpr int TX = 64, TY = 8, NT = TX * TY;
+constexpr int PX = TX + 4, PY = TY + 4, PPL = PX * PY;  // phi plane, halo 2
+constexpr int QX = TX + 2, QY = TY + 2, QPL = QX * QY;  // node plane, halo 1
+constexpr int NRING = 2 * QX + 2 * TY;                   // ring nodes per plane
+constexpr int NSLOT = 5;
+
+template <typename R>
+__device__ __forceinline__ R dlog(R x) { return log(x); }
+template <>
+__device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+
+// Scalars of one sweep, in the arithmetic type.
+template <typename R>
+struct SC {
+  R hx, hy, hz;        // 0.5/dx ...
+  R idx2, idy2, idz2;  // 1/dx^2 ...
+  R cxy, cyz, czx;     // 0.25/(dx dy) ...
+  R E0a, E0b, K;       // eps = E0a + E0b*S4/Q^2, K = 16 E0 delta
+  R small;
+  R Ke, W0, RTV;       // RTV = R*TP/Vm
+  R omCs, omCl;        // 1-Cs_eq, 1-Cl_eq
+  R M0dt, AAA;
+  R Dliq, Dsol, dt;
+  R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
+};
+
+template <typename R>
+SC<R> make_sc(const KParams &p) {
+  SC<R> s;
+  s.hx = (R)(0.5 / p.dx); s.hy = (R)(0.5 / p.dy); s.hz = (R)(0.5 / p.dz);
+  s.idx2 = (R)(1.0 / (p.dx * p.dx)); s.idy2 = (R)(1.0 / (p.dy * p.dy));
+  s.idz2 = (R)(1.0 / (p.dz * p.dz));
+  s.cxy = (R)(0.25 / (p.dx * p.dy)); s.cyz = (R)(0.25 / (p.dy * p.dz));
+  s.czx = (R)(0.25 / (p.dz * p.dx));
+  s.E0a = (R)(p.E0 * (1.0 - 3.0 * p.delta_eps));
+  s.E0b = (R)(p.E0 * 4.0 * p.delta_eps);
+  s.K = (R)(16.0 * p.E0 * p.delta_eps);
+  s.small = (R)1.0e-23;
+  s.Ke = (R)p.Ke; s.W0 = (R)p.W0; s.RTV = (R)(p.R * p.TP / p.Vm);
+  s.omCs = (R)(1.0 - p.Cs_eq); s.omCl = (R)(1.0 - p.Cl_eq);
+  s.M0dt = (R)(p.M0 * p.dt); s.AAA = (R)p.AAA;
+  s.Dliq = (R)p.D_liq; s.Dsol = (R)p.D_sol; s.dt = (R)p.dt;
+  s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
+  return s;
+}
+
+template <typename R>
+struct Node {
+  R e2, xy, yz, zx;
+};
+
+// Node quantities from the centred gradient at a node (phase_body.inc:55-129).
+template <typename R>
+__device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
+  const R gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
+  const R Q = gx2 + gy2 + gz2 + s.small;
+  const R iQ2 = (R)1 / (Q * Q);
+  const R S4 = gx2 * gx2 + gy2 * gy2 + gz2 * gz2;
+  const R eps = s.E0a + s.E0b * S4 * iQ2;
+  const R f = s.K * eps * iQ2;
+  Node<R> v;
+  v.e2 = eps * eps;
+  v.xy = f * gx * gy * (gx2 - gy2);
+  v.yz = f * gy * gz * (gy2 - gz2);
+  v.zx = f * gz * gx * (gz2 - gx2);
+  return v;
+}
+
+template <typename R>
+__device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
+  if (quint) {
+    h = p * p * p * ((R)10 - (R)15 * p + (R)6 * p * p);
+    hl = (R)30 * p * p * ((R)1 - p) * ((R)1 - p);
+  } else {
+    h = p * p * ((R)3 - (R)2 * p);
+    hl = (R)6 * p * ((R)1 - p);
+  }
+}
+
+// Solute node quantities: 1/gamma (gamma without the centre's D_C) and
+// mu = ln(CL/(1-CL)) (conc_body.inc:61-118, 139-144).
+template <typename R>
+__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &mu,
+                                            R &CL) {
+  R h, hl;
+  h_of(p, quint, h, hl);
+  const R den = s.Ke * h + ((R)1 - h);
+  CL = cc / den;
+  const R CS = s.Ke * CL;
+  const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
+  ig = (R)1 / g;
+  mu = dlog<R>(cc / (den - cc));
+}
+
+struct Geo {
+  int nx, ny, nz;
+  int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
+  int ntiles, per_xcd;
+};
+
+// XCD-aware tile order: consecutive work-group ids go round-robin over the
+// 8 XCDs, so give each XCD a contiguous run of tiles (neighbours share halo
+// lines in that XCD's L2).
+__device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * g.per_xcd + (b >> 3);
+  if (t >= g.ntiles) return false;
+  bx = t % g.nbx;
+  by = (t / g.nbx) % g.nby;
+  sg = t / (g.nbx * g.nby);
+  return true;
+}
+
+// SWEEP_B == false: sweep A (phase only).  SWEEP_B == true: phase + solute +
+// (WRITE_T ? next-step T : DPP).
+template <typename R, bool SWEEP_B>
+__global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po,
+                                              const R *__restrict__ co, R *__restrict__ pn,
+                                              R *__restrict__ cn, const R *__restrict__ Tcur,
+                                              R *__restrict__ Tnext, R *__restrict__ dpp,
+                                              int write_T) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R *sphi = reinterpret_cast<R *>(smem_raw);         // NSLOT * PPL
+  R *snode = sphi + NSLOT * PPL;                     // 4 * QPL
+  R *sgm = snode + 4 * QPL;                          // 2 * QPL (sweep B)
+
+  int bx, by, sg;
+  if (!tile_of(g, bx, by, sg)) return;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i0 = 2 + bx * TX, j0 = 2 + by * TY;  // first cell of the tile
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = (i <= nx - 1) && (j <= ny - 1);
+  const bool ph_cell = own && i >= 3 && i <= nx - 2 && j >= 3 && j <= ny - 2;
+  const int ks = g.k_lo + sg * g.seg;
+  const int ke = min(ks + g.seg - 1, g.k_hi);
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  const int quint = kp.h_quintico;
+
+  // ring node handled by this thread (tid < NRING): its node-frame coords
+  int rlx = -1, rly = -1;
+  if (tid < NRING) {
+    if (tid < QX) { rlx = tid; rly = 0; }
+    else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
+    else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
+  }
+  const int olx = tx + 1, oly = ty + 1;  // own node-frame coords
+
+  // phi plane loader: each thread moves elements e = tid, tid+NT of the
+  // (PY x PX) halo-2 window; global cells outside the grid are clamped.
+  R pre[2];
+  auto gload = [&](int q) {
+    const int qc = q < 1 ? 1 : (q > nz ? nz : q);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + u * NT;
+      if (e < PPL) {
+        int gi = i0 - 2 + (e % PX), gj = j0 - 2 + (e / PX);
+        gi = gi < 1 ? 1 : (gi > nx ? nx : gi);
+        gj = gj < 1 ? 1 : (gj > ny ? ny : gj);
+        pre[u] = po[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(qc - 1) * sz];
+      }
+    }
+  };
+  auto lstore = [&](int q) {
+    R *dst = sphi + (((q % NSLOT) + NSLOT) % NSLOT) * PPL;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = tid + u * NT;
+      if (e < PPL) dst[e] = pre[u];
+    }
+  };
+  auto slot = [&](int q) -> const R * { return sphi + (((q % NSLOT) + NSLOT) % NSLOT) * PPL; };
+
+  // node at node-frame (lx,ly) of plane q-1 from phi planes q-2, q-1, q
+  auto node_at = [&](int q, int lx, int ly) -> Node<R> {
+    const R *Pm = slot(q - 2), *P0 = slot(q - 1), *Pp = slot(q);
+    const int c = (ly + 1) * PX + (lx + 1);
+    const R gx = (P0[c + 1] - P0[c - 1]) * s.hx;
+    const R gy = (P0[c + PX] - P0[c - PX]) * s.hy;
+    const R gz = (Pp[c] - Pm[c]) * s.hz;
+    return node_from(gx, gy, gz, s);
+  };
+
+  // own-column registers: nodes of planes k-1 (M), k (Z), k+1 (P) and the
+  // solute quantities of planes k-1, k, k+1
+  Node<R> nM{}, nZ{}, nP{};
+  R igM = 0, muM = 0, igZ = 0, muZ = 0, clZ = 0, igP = 0, muP = 0, clP = 0;
+  R cZ = 0, cP_ = 0;  // c at planes k and k+1 (own column)
+  uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
+
+  const int q_first = ks - 2, q_last = ke + 2;
+  gload(q_first);
+  for (int q = q_first; q <= q_last; ++q) {
+    lstore(q);
+    if (q < q_last) gload(q + 1);  // prefetch; lands while this plane computes
+    __syncthreads();
+    const int kn = q - 1;  // plane whose node / solute quantities are built now
+    Node<R> nr{};
+    R rig = 0, rmu = 0;
+    R cnew_col = 0;
+    if (q >= ks) {
+      // nodes are only used for planes [2, nz-1]
+      const bool nvalid = kn >= 2 && kn <= nz - 1;
+      if (nvalid) {
+        nP = node_at(q, olx, oly);
+        if (tid < NRING) nr = node_at(q, rlx, rly);
+      }
+      if constexpr (SWEEP_B) {
+        if (kn >= 1 && kn <= nz) {
+          const R *P0 = slot(q - 1);
+          {
+            const int gi = min(max(i, 1), nx), gj = min(max(j, 1), ny);
+            cnew_col = co[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(kn - 1) * sz];
+            solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP, clP);
+          }
+          if (tid < NRING) {
+            const int gi = min(max(i0 - 1 + rlx, 1), nx), gj = min(max(j0 - 1 + rly, 1), ny);
+            const R cr = co[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(kn - 1) * sz];
+            R dummy;
+            solute_node(P0[(rly + 1) * PX + rlx + 1], cr, quint, s, rig, rmu, dummy);
+          }
+        }
+      }
+    }
+    const int k = q - 2;  // cell plane updated now
+    if (q >= ks + 2 && own) {
+      const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
+      const int c = (ty + 2) * PX + (tx + 2);
+      const R PP = P0[c];
+      const size_t gidx = (size_t)(i - 1) + (size_t)(j - 1) * sy + (size_t)(k - 1) * sz;
+      R dppv = (R)0, pnew = PP;
+      R cPP;
+      if constexpr (SWEEP_B) cPP = cZ; else cPP = co[gidx];
+      if (ph_cell && k >= 3 && k <= nz - 2) {
+        const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
+        const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
+                PSW = P0[c - PX - 1];
+        const R PT = Pp[c], PB = Pm[c];
+        const R PNT = Pp[c + PX], PST = Pp[c - PX], PET = Pp[c + 1], PWT = Pp[c - 1];
+        const R PNB = Pm[c + PX], PSB = Pm[c - PX], PEB = Pm[c + 1], PWB = Pm[c - 1];
+        const int nc = oly * QX + olx;
+        const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
+        const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
+        const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
+        const R yzN = yz[nc + QX], yzS = yz[nc - QX];
+        const R zxE = zx[nc + 1], zxW = zx[nc - 1];
+        const R yzT = nP.yz, yzB = nM.yz, zxT = nP.zx, zxB = nM.zx;
+        R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
+               (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
+               (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
+        const R Cxy = PNE * (xyE - xyN) + PSE * (xyS - xyE) + PNW * (xyN - xyW) +
+                      PSW * (xyW - xyS) + PN * (xyE - xyW) + PS * (xyW - xyE) +
+                      PE * (xyS - xyN) + PW * (xyN - xyS);
+        const R Cyz = PNT * (yzN - yzT) + PNB * (yzB - yzN) + PST * (yzT - yzS) +
+                      PSB * (yzS - yzB) + PT * (yzN - yzS) + PB * (yzS - yzN) +
+                      PN * (yzB - yzT) + PS * (yzT - yzB);
+        const R Czx = PET * (zxT - zxE) + PWT * (zxW - zxT) + PEB * (zxE - zxB) +
+                      PWB * (zxB - zxW) + PE * (zxT - zxB) + PW * (zxB - zxT) +
+                      PT * (zxW - zxE) + PB * (zxE - zxW);
+        AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
+        const R om = (R)1 - PP;
+        const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
+        R h, hl;
+        h_of(PP, quint, h, hl);
+        R CLP;
+        if constexpr (SWEEP_B) CLP = clZ;
+        else CLP = cPP / (s.Ke * h + ((R)1 - h));
+        const R CSP = s.Ke * CLP;
+        const R GG = hl * s.RTV * dlog<R>(fabs((s.omCs * ((R)1 - CLP)) / (s.omCl * ((R)1 - CSP)))) -
+                     s.W0 * gl;
+        const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
+                              ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                              : (float)hash_unit(hbase);
+        const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
+                                                              : ((R)2 * (R)ran - (R)1) * s.AAA;
+        const R ruidos = amp * (R)16 * (PP * PP) * (om * om);
+        const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
+        dppv = pstar - PP;
+        const R cl = pstar > (R)1 ? (R)1 : pstar;
+        pnew = cl < (R)0 ? (R)0 : cl;
+        pn[gidx] = pnew;
+      }
+      if constexpr (SWEEP_B) {
+        // solute at (i,j,k), k in [2,nz-1] (conc_body.inc:92-144)
+        const int nc = oly * QX + olx;
+        const R *ig = sgm, *mu = sgm + QPL;
+        const R DC = PP <= (R)0.95 ? s.Dliq : s.Dsol;
+        const R igc = igZ, muc = muZ;
+        const R fx = (mu[nc + 1] - muc) / (ig[nc + 1] + igc) + (mu[nc - 1] - muc) / (ig[nc - 1] + igc);
+        const R fy = (mu[nc + QX] - muc) / (ig[nc + QX] + igc) +
+                     (mu[nc - QX] - muc) / (ig[nc - QX] + igc);
+        const R fz = (muP - muc) / (igP + igc) + (muM - muc) / (igM + igc);
+        cn[gidx] = cPP + s.dt * ((R)2 * DC) * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        if (write_T) {
+          // temp_body.inc:31-41 of the next step: TP + D dt lap(T) + fonte
+          R hh, hl2;
+          h_of(pnew, quint, hh, hl2);
+          R lap = (R)0;
+          if (s.Ddt != (R)0) {
+            const R TP2 = (R)2 * s.TP;
+            lap = (Tcur[gidx + 1] - TP2 + Tcur[gidx - 1]) * s.idx2 +
+                  (Tcur[gidx + sy] - TP2 + Tcur[gidx - sy]) * s.idy2 +
+                  (Tcur[gidx + sz] - TP2 + Tcur[gidx - sz]) * s.idz2;
+          }
+          Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
+        } else if (ph_cell && k >= 3 && k <= nz - 2) {
+          dpp[gidx] = dppv;
+        }
+      } else {
+        (void)Tcur; (void)Tnext; (void)dpp; (void)write_T;
+      }
+      hbase += kHashStepK;
+    }
+    __syncthreads();
+    // publish plane kn's node / solute quantities and rotate the column
+    if (q >= ks) {
+      const bool nvalid = kn >= 2 && kn <= nz - 1;
+      if (nvalid) {
+        const int oc = oly * QX + olx;
+        snode[oc] = nP.e2; snode[QPL + oc] = nP.xy; snode[2 * QPL + oc] = nP.yz;
+        snode[3 * QPL + oc] = nP.zx;
+        if (tid < NRING) {
+          const int rc = rly * QX + rlx;
+          snode[rc] = nr.e2; snode[QPL + rc] = nr.xy; snode[2 * QPL + rc] = nr.yz;
+          snode[3 * QPL + rc] = nr.zx;
+        }
+      }
+      if constexpr (SWEEP_B) {
+        const int oc = oly * QX + olx;
+        sgm[oc] = igP; sgm[QPL + oc] = muP;
+        if (tid < NRING) {
+          const int rc = rly * QX + rlx;
+          sgm[rc] = rig; sgm[QPL + rc] = rmu;
+        }
+      }
+    }
+    nM = nZ; nZ = nP;
+    if constexpr (SWEEP_B) {
+      igM = igZ; muM = muZ; igZ = igP; muZ = muP; clZ = clP;
+      cZ = cnew_col;
+    }
+    (void)cP_;
+  }
+}
+
+template <typename R>
+size_t march_lds(bool sweep_b) {
+  return sizeof(R) * (size_t)(NSLOT * PPL + 4 * QPL + (sweep_b ? 2 * QPL : 0));
+}
+
+Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg) {
+  Geo g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nbx = (nx - 2 + TX - 1) / TX;
+  g.nby = (ny - 2 + TY - 1) / TY;
+  g.k_lo = k_lo; g.k_hi = k_hi;
+  const int ncell = k_hi - k_lo + 1;
+  g.seg = seg < 1 ? 1 : (seg > ncell ? ncell : seg);
+  g.nseg = (ncell + g.seg - 1) / g.seg;
+  g.ntiles = g.nbx * g.nby * g.nseg;
+  g.per_xcd = (g.ntiles + 7) / 8;
+  return g;
+}
+
+// Segment length: enough work-groups to fill 256 CUs a few times over while
+// keeping the z warm-up (4 extra planes per segment) small.
+int pick_seg(int nx, int ny, int ncell) {
+  const int tiles = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY);
+  const int target = 2048;
+  int nseg = (target + tiles - 1) / tiles;
+  if (nseg < 1) nseg = 1;
+  int seg = (ncell + nseg - 1) / nseg;
+  if (seg < 24) seg = 24;
+  return seg;
+}
+
+template <typename R>
+hipError_t launch_march(bool sweep_b, const Geo &g, const KParams &kp, const R *po, const R *co,
+                        R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
+                        hipStream_t st) {
+  const SC<R> s = make_sc<R>(kp);
+  const dim3 blk(TX, TY, 1), grd(8 * g.per_xcd, 1, 1);
+  const size_t lds = march_lds<R>(sweep_b);
+  if (sweep_b)
+    hipLaunchKernelGGL((k_march<R, true>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
+                       Tnext, dpp, write_T);
+  else
+    hipLaunchKernelGGL((k_march<R, false>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
+                       Tnext, dpp, write_T);
+  return hipGetLastError();
+}
+
+template <typename R>
+int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedStep &fs,
+              void *phi[2], int *phi_old, bool *phi_alias, void *c[2], int *c_old, bool *c_alias,
+              void *T[2], int *T_old, bool *T_alias, void *dpp, double *probe, hipStream_t st,
+              KTimer &tm) {
+  const int nx = S.nx, ny = S.ny, nz = S.nz;
+  hipError_t e;
+  auto NEWW = [](int old, bool &alias) { alias = false; return 1 - old; };
+  // temperatura for this step (later steps get it from the previous sweep B)
+  if (fs.first) {
+    const int tn = NEWW(*T_old, *T_alias);
+    tm.begin(PF_KT_TEMPERATURA, st);
+    e = launch_temperatura<R>(kt, (const R *)T[*T_old], (const R *)phi[*phi_old], (const R *)dpp,
+                              (R *)T[tn], st);
+    tm.end(st);
+    if (e != hipSuccess) return PF_E_HIP;
+  }
+  const Geo ga = make_geo(nx, ny, nz, 3, nz - 2, S.seg_a);
+  const Geo gb = make_geo(nx, ny, nz, 2, nz - 1, S.seg_b);
+  size_t pa, pb;
+  probe_cells(nx, ny, nz, pa, pb);
+  for (int sw = 1; sw <= fs.nv; ++sw) {
+    if (sw > 1) {  // principal.f90:127-150
+      if (!*phi_alias) *phi_old = 1 - *phi_old;
+      *phi_alias = true;
+    }
+    const int pnw = NEWW(*phi_old, *phi_alias);
+    const R *po = (const R *)phi[*phi_old];
+    R *pn = (R *)phi[pnw];
+    if (sw < fs.nv) {
+      tm.begin(PF_KT_SWEEP_A, st);
+      e = launch_march<R>(false, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr, nullptr,
+                          nullptr, 0, st);
+      tm.end(st);
+      if (e != hipSuccess) return PF_E_HIP;
+      tm.begin(PF_KT_BOUNDARY, st);
+      e = launch_phase_boundary<R>(k, po, pn, st);
+      tm.end(st);
+      if (e != hipSuccess) return PF_E_HIP;
+    } else {
+      const int cnw = NEWW(*c_old, *c_alias);
+      // T of the next step goes into the buffer of this step's TOLD (dead now)
+      const int t_cur = *T_alias ? *T_old : 1 - *T_old;
+      const int t_nxt = 1 - t_cur;
+      tm.begin(PF_KT_SWEEP_B, st);
+      e = launch_march<R>(true, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
+                          (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1, st);
+      tm.end(st);
+      if (e != hipSuccess) return PF_E_HIP;
+      tm.begin(PF_KT_BOUNDARY, st);
+      e = launch_phase_boundary<R>(k, po, pn, st);
+      if (e == hipSuccess) e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st);
+      tm.end(st);
+      if (e != hipSuccess) return PF_E_HIP;
+    }
+    if (probe) {
+      e = launch_probe<R>((const R *)pn, pa, pb, probe + 2 * (sw - 1), st);
+      if (e != hipSuccess) return PF_E_HIP;
+    }
+  }
+  // end of step (principal.f90:200-226)
+  if (!*phi_alias) *phi_old = 1 - *phi_old;
+  *phi_alias = true;
+  if (!*c_alias) *c_old = 1 - *c_old;
+  *c_alias = true;
+  {
+    const int t_cur = *T_alias ? *T_old : 1 - *T_old;  // holds T of this step
+    *T_old = t_cur;
+    // not last: the other buffer holds the next step's T (its temperatura)
+    *T_alias = fs.last;
+  }
+  return PF_OK;
+}
+
+}  // namespace
+
+int fused_init(FusedState &s, int nx, int ny, int nz, int prec) {
+  s.nx = nx; s.ny = ny; s.nz = nz; s.prec = prec;
+  s.seg_a = pick_seg(nx, ny, nz - 4);
+  s.seg_b = pick_seg(nx, ny, nz - 2);
+  s.enabled = nx >= 6 && ny >= 6 && nz >= 6;
+  if (s.enabled) {
+    const size_t la = prec == PF_FP64 ? march_lds<double>(true) : march_lds<float>(true);
+    if (la > 160 * 1024) s.enabled = false;
+  }
+  return PF_OK;
+}
+
+void fused_free(FusedState &s) { s.enabled = false; }
+size_t fused_bytes(const FusedState &) { return 0; }
+
+bool fused_supported(const FusedState &s, const pf_params &p) {
+  (void)p;
+  return s.enabled;
+}
+
+int fused_step(FusedState &s, const KParams &kt, const KParams &k, const FusedStep &fs,
+               void *phi[2], int *phi_old, bool *phi_alias, void *c[2], int *c_old,
+               bool *c_alias, void *T[2], int *T_old, bool *T_alias, void *dpp, double *probe,
+               hipStream_t stream, KTimer &timer) {
+  if (s.prec == PF_FP64)
+    return step_impl<double>(s, kt, k, fs, phi, phi_old, phi_alias, c, c_old, c_alias, T, T_old,
+                             T_alias, dpp, probe, stream, timer);
+  return step_impl<float>(s, kt, k, fs, phi, phi_old, phi_alias, c, c_old, c_alias, T, T_old,
+                          T_alias, dpp, probe, stream, timer);
+}
+
+}  // namespace pf

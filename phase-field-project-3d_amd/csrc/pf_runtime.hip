@@ -1,0 +1,486 @@
+// pf_runtime.hip -- the C-ABI context: device-resident fields, buffer swaps
+// in place of the reference's whole-array copies, the drop-in calls and the
+// fused time loop.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/pf_capi.h"
+#include "pf_common.h"
+#include "pf_fused.h"
+#include "pf_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(x)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return fail(PF_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+// One field pair (old/new) with swap-instead-of-copy bookkeeping: `old_idx`
+// names the buffer that holds the reference's *_old array; when `alias` is
+// set the reference's *_new array equals *_old (right after an old<-new copy)
+// and the other buffer holds stale data that the next kernel overwrites.
+struct pf_pair {
+  void *buf[2] = {nullptr, nullptr};
+  int old_idx = 0;
+  bool alias = false;
+  void *old_() const { return buf[old_idx]; }
+  void *new_() const { return alias ? buf[old_idx] : buf[1 - old_idx]; }
+  void *new_w() { alias = false; return buf[1 - old_idx]; }  // for writing
+  void swap() {  // old <- new
+    if (!alias) old_idx = 1 - old_idx;
+    alias = true;
+  }
+};
+
+struct pf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0, nz = 0, prec = PF_FP64;
+  size_t n = 0, esz = 8;
+  pf_pair phi, c, T;
+  void *dpp = nullptr;
+  double *diag_part = nullptr, *diag_out = nullptr;
+  int diag_nblk = 0;
+  double *pinned = nullptr;  // 8 doubles of host-visible scratch
+  double *probe_dev = nullptr;
+  size_t probe_cap = 0;
+  pf::FusedState fused;
+  pf::KTimer timer;
+  bool use_fused = true;
+  size_t bytes = 0;
+};
+
+namespace {
+
+int check_ctx(const pf_ctx *ctx, const pf_params *p) {
+  if (!ctx) return fail(PF_E_ARG, "null context");
+  if (p && (p->nx != ctx->nx || p->ny != ctx->ny || p->nz != ctx->nz))
+    return fail(PF_E_ARG, "params grid %dx%dx%d differs from the context's %dx%dx%d", p->nx, p->ny,
+                p->nz, ctx->nx, ctx->ny, ctx->nz);
+  if (p && p->precision != ctx->prec)
+    return fail(PF_E_ARG, "params precision %d differs from the context's %d", p->precision,
+                ctx->prec);
+  return PF_OK;
+}
+
+pf::KParams kparams(const pf_params &p) {
+  pf::KParams k;
+  k.nx = p.nx; k.ny = p.ny; k.nz = p.nz; k.it = p.it;
+  k.h_quintico = p.h_quintico; k.ruido_tipo = p.ruido_tipo; k.seed = p.seed;
+  k.dx = p.dx; k.dy = p.dy; k.dz = p.dz;
+  k.dt = p.dt; k.D = p.D; k.delta_H = p.delta_H; k.cp_s = p.cp_s; k.TP = p.TP;
+  k.R = p.R; k.Vm = p.Vm; k.E0 = p.E0; k.W0 = p.W0; k.M0 = p.M0; k.delta_eps = p.delta_eps;
+  k.AAA = p.AAA; k.D_liq = p.D_liq; k.D_sol = p.D_sol;
+  k.Ke = (double)p.Ke;
+  // equacao_campo_fase.f90:21-22: recomputed from TP for every sweep
+  pf_equilibrium(p.TP, p.Tm, p.Ke, p.Me, &k.Cs_eq, &k.Cl_eq);
+  return k;
+}
+
+// Host FP64 <-> device field (FP64 or FP32).
+int upload(pf_ctx *ctx, void *dst, const double *src) {
+  if (!src) {
+    HIP_TRY(hipMemsetAsync(dst, 0, ctx->n * ctx->esz, ctx->stream));
+    return PF_OK;
+  }
+  if (ctx->prec == PF_FP64) {
+    HIP_TRY(hipMemcpyAsync(dst, src, ctx->n * 8, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+  }
+  std::vector<float> tmp(ctx->n);
+  for (size_t q = 0; q < ctx->n; ++q) tmp[q] = (float)src[q];
+  HIP_TRY(hipMemcpyAsync(dst, tmp.data(), ctx->n * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int download(pf_ctx *ctx, double *dst, const void *src) {
+  if (!dst) return PF_OK;
+  if (ctx->prec == PF_FP64) {
+    HIP_TRY(hipMemcpyAsync(dst, src, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return PF_OK;
+  }
+  std::vector<float> tmp(ctx->n);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), src, ctx->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (size_t q = 0; q < ctx->n; ++q) dst[q] = (double)tmp[q];
+  return PF_OK;
+}
+
+int launch_probe(pf_ctx *ctx, const void *field, double *dst) {
+  size_t a, b;
+  pf::probe_cells(ctx->nx, ctx->ny, ctx->nz, a, b);
+  if (ctx->prec == PF_FP64)
+    HIP_TRY(pf::launch_probe<double>((const double *)field, a, b, dst, ctx->stream));
+  else
+    HIP_TRY(pf::launch_probe<float>((const float *)field, a, b, dst, ctx->stream));
+  return PF_OK;
+}
+
+int ensure_probe_cap(pf_ctx *ctx, size_t nd) {
+  if (ctx->probe_cap >= nd) return PF_OK;
+  if (ctx->probe_dev) HIP_TRY(hipFree(ctx->probe_dev));
+  ctx->probe_dev = nullptr;
+  HIP_TRY(hipMalloc(&ctx->probe_dev, nd * sizeof(double)));
+  ctx->probe_cap = nd;
+  return PF_OK;
+}
+
+// ---- one drop-in call each (used by the C-ABI and by the unfused loop) ----
+int do_temperatura(pf_ctx *ctx, const pf::KParams &k) {
+  hipError_t e;
+  ctx->timer.begin(PF_KT_TEMPERATURA, ctx->stream);
+  if (ctx->prec == PF_FP64)
+    e = pf::launch_temperatura<double>(k, (const double *)ctx->T.old_(),
+                                       (const double *)ctx->phi.old_(), (const double *)ctx->dpp,
+                                       (double *)ctx->T.new_w(), ctx->stream);
+  else
+    e = pf::launch_temperatura<float>(k, (const float *)ctx->T.old_(),
+                                      (const float *)ctx->phi.old_(), (const float *)ctx->dpp,
+                                      (float *)ctx->T.new_w(), ctx->stream);
+  ctx->timer.end(ctx->stream);
+  if (e != hipSuccess) return fail(PF_E_HIP, "temperatura launch: %s", hipGetErrorString(e));
+  return PF_OK;
+}
+
+int do_phase(pf_ctx *ctx, const pf::KParams &k) {
+  hipError_t e;
+  ctx->timer.begin(PF_KT_DROPIN, ctx->stream);
+  if (ctx->prec == PF_FP64)
+    e = pf::launch_phase<double>(k, (const double *)ctx->phi.old_(), (const double *)ctx->c.old_(),
+                                 (double *)ctx->phi.new_w(), (double *)ctx->dpp, ctx->stream);
+  else
+    e = pf::launch_phase<float>(k, (const float *)ctx->phi.old_(), (const float *)ctx->c.old_(),
+                                (float *)ctx->phi.new_w(), (float *)ctx->dpp, ctx->stream);
+  ctx->timer.end(ctx->stream);
+  if (e != hipSuccess) return fail(PF_E_HIP, "phase launch: %s", hipGetErrorString(e));
+  return PF_OK;
+}
+
+int do_solute(pf_ctx *ctx, const pf::KParams &k) {
+  hipError_t e;
+  ctx->timer.begin(PF_KT_DROPIN, ctx->stream);
+  if (ctx->prec == PF_FP64)
+    e = pf::launch_solute<double>(k, (const double *)ctx->phi.old_(), (const double *)ctx->c.old_(),
+                                  (double *)ctx->c.new_w(), ctx->stream);
+  else
+    e = pf::launch_solute<float>(k, (const float *)ctx->phi.old_(), (const float *)ctx->c.old_(),
+                                 (float *)ctx->c.new_w(), ctx->stream);
+  ctx->timer.end(ctx->stream);
+  if (e != hipSuccess) return fail(PF_E_HIP, "solute launch: %s", hipGetErrorString(e));
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *pf_version(void) { return "pfamd 0.1 (gfx950)"; }
+const char *pf_last_error(void) { return g_err.c_str(); }
+
+int pf_device_count(int32_t *n) {
+  if (!n) return fail(PF_E_ARG, "null output");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return PF_OK;
+}
+
+int pf_init(const pf_params *p, const double *phi_old, const double *phi_new, const double *c_old,
+            const double *c_new, const double *T_old, const double *T_new, const double *dpp,
+            pf_ctx **out) {
+  if (!p || !out) return fail(PF_E_ARG, "null argument");
+  *out = nullptr;
+  if (p->nx < 5 || p->ny < 5 || p->nz < 5)
+    return fail(PF_E_ARG, "grid %dx%dx%d: each extent must be >= 5", p->nx, p->ny, p->nz);
+  if (p->precision != PF_FP64 && p->precision != PF_FP32_MIXED)
+    return fail(PF_E_ARG, "unknown precision %d", p->precision);
+  if ((long long)p->nx * p->ny > (1LL << 30))
+    return fail(PF_E_ARG, "plane %dx%d too large", p->nx, p->ny);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(PF_E_NODEV, "no HIP device visible");
+  pf_ctx *ctx = new pf_ctx();
+  ctx->nx = p->nx; ctx->ny = p->ny; ctx->nz = p->nz;
+  ctx->prec = p->precision;
+  ctx->n = (size_t)p->nx * p->ny * p->nz;
+  ctx->esz = ctx->prec == PF_FP64 ? 8 : 4;
+  int rc = PF_OK;
+  auto cleanup = [&](int code) {
+    pf_finalize(ctx);
+    return code;
+  };
+  if (hipGetDevice(&ctx->device) != hipSuccess) return cleanup(fail(PF_E_HIP, "hipGetDevice"));
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(PF_E_HIP, "hipStreamCreate"));
+  void **bufs[7] = {&ctx->phi.buf[0], &ctx->phi.buf[1], &ctx->c.buf[0], &ctx->c.buf[1],
+                    &ctx->T.buf[0], &ctx->T.buf[1], &ctx->dpp};
+  const double *src[7] = {phi_old, phi_new, c_old, c_new, T_old, T_new, dpp};
+  for (int f = 0; f < 7; ++f) {
+    if (hipMalloc(bufs[f], ctx->n * ctx->esz) != hipSuccess) {
+      *bufs[f] = nullptr;
+      return cleanup(fail(PF_E_NOMEM, "hipMalloc of %zu bytes failed", ctx->n * ctx->esz));
+    }
+    ctx->bytes += ctx->n * ctx->esz;
+  }
+  ctx->diag_nblk = pf::diag_blocks(ctx->ny, ctx->nz);
+  if (hipMalloc(&ctx->diag_part, sizeof(double) * 5 * (ctx->diag_nblk + 1)) != hipSuccess)
+    return cleanup(fail(PF_E_NOMEM, "hipMalloc diag"));
+  ctx->diag_out = ctx->diag_part + 5 * ctx->diag_nblk;
+  if (hipHostMalloc(&ctx->pinned, 8 * sizeof(double)) != hipSuccess)
+    return cleanup(fail(PF_E_NOMEM, "hipHostMalloc"));
+  for (int f = 0; f < 7; ++f)
+    if ((rc = upload(ctx, *bufs[f], src[f])) != PF_OK) return cleanup(rc);
+  if ((rc = pf::fused_init(ctx->fused, ctx->nx, ctx->ny, ctx->nz, ctx->prec)) != PF_OK)
+    return cleanup(fail(rc, "fused path set-up failed"));
+  *out = ctx;
+  return PF_OK;
+}
+
+int pf_finalize(pf_ctx *ctx) {
+  if (!ctx) return PF_OK;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  pf::fused_free(ctx->fused);
+  ctx->timer.release();
+  void *bufs[7] = {ctx->phi.buf[0], ctx->phi.buf[1], ctx->c.buf[0], ctx->c.buf[1],
+                   ctx->T.buf[0], ctx->T.buf[1], ctx->dpp};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  if (ctx->diag_part) hipFree(ctx->diag_part);
+  if (ctx->probe_dev) hipFree(ctx->probe_dev);
+  if (ctx->pinned) hipHostFree(ctx->pinned);
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return PF_OK;
+}
+
+int pf_temperatura(pf_ctx *ctx, const pf_params *p) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
+  if (rc != PF_OK) return rc;
+  return do_temperatura(ctx, kparams(*p));
+}
+
+int pf_equacao_campo_fase(pf_ctx *ctx, const pf_params *p, double probe[3]) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
+  if (rc != PF_OK) return rc;
+  if ((rc = do_phase(ctx, kparams(*p))) != PF_OK) return rc;
+  if (probe) {
+    if ((rc = ensure_probe_cap(ctx, 2)) != PF_OK) return rc;
+    if ((rc = launch_probe(ctx, ctx->phi.new_(), ctx->probe_dev)) != PF_OK) return rc;
+    HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->probe_dev, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    probe[0] = ctx->pinned[0];
+    probe[1] = p->TP;
+    probe[2] = ctx->pinned[1];
+  }
+  return PF_OK;
+}
+
+int pf_phase_new_to_old(pf_ctx *ctx) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  ctx->phi.swap();
+  return PF_OK;
+}
+
+int pf_concentracao_soluto(pf_ctx *ctx, const pf_params *p) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
+  if (rc != PF_OK) return rc;
+  return do_solute(ctx, kparams(*p));
+}
+
+int pf_end_step(pf_ctx *ctx) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  ctx->phi.swap();
+  ctx->T.swap();
+  ctx->c.swap();
+  return PF_OK;
+}
+
+int pf_diagnosticos(pf_ctx *ctx, const pf_params *p, double out[5]) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && (!p || !out)) rc = fail(PF_E_ARG, "null argument");
+  if (rc != PF_OK) return rc;
+  const pf::KParams k = kparams(*p);
+  hipError_t e;
+  if (ctx->prec == PF_FP64)
+    e = pf::launch_diag<double>(k, (const double *)ctx->phi.new_(), (const double *)ctx->c.new_(),
+                                ctx->diag_part, ctx->diag_nblk, ctx->diag_out, ctx->stream);
+  else
+    e = pf::launch_diag<float>(k, (const float *)ctx->phi.new_(), (const float *)ctx->c.new_(),
+                               ctx->diag_part, ctx->diag_nblk, ctx->diag_out, ctx->stream);
+  if (e != hipSuccess) return fail(PF_E_HIP, "diag launch: %s", hipGetErrorString(e));
+  HIP_TRY(hipMemcpyAsync(ctx->pinned, ctx->diag_out, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  memcpy(out, ctx->pinned, 5 * sizeof(double));
+  return PF_OK;
+}
+
+int pf_download(pf_ctx *ctx, double *phi_new, double *c_new, double *T_new) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if ((rc = download(ctx, phi_new, ctx->phi.new_())) != PF_OK) return rc;
+  if ((rc = download(ctx, c_new, ctx->c.new_())) != PF_OK) return rc;
+  return download(ctx, T_new, ctx->T.new_());
+}
+
+int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old, double *c_new,
+                    double *T_old, double *T_new, double *dpp) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  const void *src[7] = {ctx->phi.old_(), ctx->phi.new_(), ctx->c.old_(), ctx->c.new_(),
+                        ctx->T.old_(),   ctx->T.new_(),   ctx->dpp};
+  double *dst[7] = {phi_old, phi_new, c_old, c_new, T_old, T_new, dpp};
+  for (int f = 0; f < 7; ++f)
+    if ((rc = download(ctx, dst[f], src[f])) != PF_OK) return rc;
+  return PF_OK;
+}
+
+int pf_synchronize(pf_ctx *ctx) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return PF_OK;
+}
+
+int pf_device_bytes(const pf_ctx *ctx, size_t *bytes) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (!bytes) return fail(PF_E_ARG, "null output");
+  *bytes = ctx->bytes + pf::fused_bytes(ctx->fused);
+  return PF_OK;
+}
+
+int pf_set_fused(pf_ctx *ctx, int32_t on) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  ctx->use_fused = on != 0;
+  return PF_OK;
+}
+
+int pf_timing_enable(pf_ctx *ctx, int32_t on) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->timer.kind.clear();
+  ctx->timer.on = on != 0;
+  return PF_OK;
+}
+
+int pf_timing_read(pf_ctx *ctx, double ms[PF_KT_COUNT], int64_t count[PF_KT_COUNT]) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc == PF_OK && (!ms || !count)) rc = fail(PF_E_ARG, "null output");
+  if (rc != PF_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (int q = 0; q < PF_KT_COUNT; ++q) {
+    ms[q] = 0.0;
+    count[q] = 0;
+  }
+  pf::KTimer &t = ctx->timer;
+  for (size_t r = 0; r < t.kind.size(); ++r) {
+    float e = 0.f;
+    HIP_TRY(hipEventElapsedTime(&e, t.pool[2 * r], t.pool[2 * r + 1]));
+    ms[t.kind[r]] += (double)e;
+    count[t.kind[r]] += 1;
+  }
+  t.kind.clear();
+  return PF_OK;
+}
+
+// principal.f90:96-226 without the output event.  The host scalars (time,
+// TP, Cs_eq/Cl_eq, M0) advance exactly as in the reference; the field work
+// goes to the fused kernels when the context supports them.
+int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *probes) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && (!p || !L)) rc = fail(PF_E_ARG, "null argument");
+  if (rc == PF_OK && nsteps < 0) rc = fail(PF_E_ARG, "negative step count");
+  if (rc != PF_OK) return rc;
+  if (nsteps == 0) return PF_OK;
+  const int nv = p->n_varreduras < 1 ? 1 : p->n_varreduras;
+  if (probes && (rc = ensure_probe_cap(ctx, (size_t)2 * nv * nsteps)) != PF_OK) return rc;
+  std::vector<double> tp_seen;
+  if (probes) tp_seen.reserve((size_t)nsteps);
+  const bool fused = ctx->use_fused && pf::fused_supported(ctx->fused, *p);
+  for (int s = 0; s < nsteps; ++s) {
+    const bool first = (s == 0), last = (s == nsteps - 1);
+    p->it += 1;
+    const int it = p->it;
+    L->time = (float)(p->dt + (double)L->time);  // principal.f90:96
+    const pf::KParams k_temp = kparams(*p);      // temperatura sees TP of step it-1
+    if (L->taxa_resfr > 0.0) p->TP = L->T0 - L->taxa_resfr * p->dt * (double)it;  // :104
+    pf_equilibrium(p->TP, p->Tm, p->Ke, p->Me, &p->Cs_eq, &p->Cl_eq);
+    const pf::KParams k = kparams(*p);
+    if (probes) tp_seen.push_back(p->TP);
+    double *pr = probes ? ctx->probe_dev + (size_t)2 * nv * s : nullptr;
+    if (fused) {
+      pf::FusedStep fs;
+      fs.first = first;
+      fs.last = last;
+      fs.nv = nv;
+      rc = pf::fused_step(ctx->fused, k_temp, k, fs, ctx->phi.buf, &ctx->phi.old_idx,
+                          &ctx->phi.alias, ctx->c.buf, &ctx->c.old_idx, &ctx->c.alias, ctx->T.buf,
+                          &ctx->T.old_idx, &ctx->T.alias, ctx->dpp, pr, ctx->stream,
+                          ctx->timer);
+      if (rc != PF_OK) return rc;
+    } else {
+      if ((rc = do_temperatura(ctx, k_temp)) != PF_OK) return rc;
+      for (int sw = 1; sw <= nv; ++sw) {
+        if (sw > 1) ctx->phi.swap();  // principal.f90:127-150
+        if ((rc = do_phase(ctx, k)) != PF_OK) return rc;
+        if (pr && (rc = launch_probe(ctx, ctx->phi.new_(), pr + 2 * (sw - 1))) != PF_OK) return rc;
+      }
+      if ((rc = do_solute(ctx, k)) != PF_OK) return rc;
+      ctx->phi.swap();  // principal.f90:200-226
+      ctx->T.swap();
+      ctx->c.swap();
+    }
+    // auxiliar.f90 (principal.f90:157): M0 for the next step
+    p->M0 = pf_auxiliar_m0(p->TP, p->Cl_eq, p->Cs_eq, p->R, p->Vm, p->E0, p->W0, L->Sigma, L->Beta,
+                           p->Ke, p->Me, p->D_liq, L->ajuste);
+  }
+  if (probes) {
+    std::vector<double> tmp((size_t)2 * nv * nsteps);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), ctx->probe_dev, tmp.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < nsteps; ++s)
+      for (int sw = 0; sw < nv; ++sw) {
+        const size_t q = (size_t)s * nv + sw;
+        probes[3 * q + 0] = tmp[2 * q];
+        probes[3 * q + 1] = tp_seen[s];
+        probes[3 * q + 2] = tmp[2 * q + 1];
+      }
+  }
+  return PF_OK;
+}
+
+}  // extern "C"

@@ -71,7 +71,9 @@ EXPORTS = (
     "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
+    "pf_timing_enable", "pf_timing_read", "pf_set_fused",
 )
+PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin")
 
 
 def load(path: str = LIB_PATH):
@@ -109,6 +111,9 @@ def load(path: str = LIB_PATH):
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
         "pf_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
+        "pf_timing_enable": ([vp, i32], C.c_int),
+        "pf_timing_read": ([vp, vp, vp], C.c_int),
+        "pf_set_fused": ([vp, i32], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -215,6 +220,19 @@ class Context:
         b = C.c_size_t(0)
         check(self.lib.pf_device_bytes(self.h, C.byref(b)), self.lib)
         return b.value
+
+    def set_fused(self, on: bool):
+        check(self.lib.pf_set_fused(self.h, 1 if on else 0), self.lib)
+
+    def timing(self, on: bool):
+        check(self.lib.pf_timing_enable(self.h, 1 if on else 0), self.lib)
+
+    def timing_read(self):
+        """{kind: (total_ms, launches)} since timing(True)."""
+        ms = np.zeros(len(PF_KT_NAMES))
+        cnt = np.zeros(len(PF_KT_NAMES), dtype=np.int64)
+        check(self.lib.pf_timing_read(self.h, _ptr(ms), cnt.ctypes.data_as(C.c_void_p)), self.lib)
+        return {nm: (float(m), int(c)) for nm, m, c in zip(PF_KT_NAMES, ms, cnt)}
 
     def close(self):
         if self.h:

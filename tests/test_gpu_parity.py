@@ -1,0 +1,233 @@
+"""GPU parity: the HIP kernels behind the C-ABI against the reference fixtures
+and the CPU oracle (the oracle is pinned bit-exact to the reference build).
+
+Tolerances (FP64):
+  drop-in kernels, one call      : bit-exact except where the device log
+                                   rounds differently from host libm;
+                                   |diff| <= 1e-13 * max(1, |ref|)
+  drop-in loop / fused loop      : phi, c max-abs <= 1e-10, T <= 1e-8 K over
+                                   the horizons tested, NaN positions equal
+  FP32-mixed form                : phi max-abs <= 1e-3,
+                                   c relative <= 1e-3, T <= 1e-2 K
+"""
+import glob
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import FIELDS, SCALARS, deck_from_json, load_golden, params_from_deck
+from pfamd import capi
+from pfamd.deck import Deck
+from pfamd.run import Simulation, initial_fields, setup_from_deck
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs a gfx950 GPU")]
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    fin = np.isfinite(a) & np.isfinite(b)
+    assert np.array_equal(np.isnan(a), np.isnan(b)), "NaN pattern differs"
+    if not fin.any():
+        return 0.0
+    return float(np.max(np.abs(a[fin] - b[fin]) / np.maximum(1.0, np.abs(b[fin]))))
+
+
+def _abs(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    assert np.array_equal(np.isnan(a), np.isnan(b)), "NaN pattern differs"
+    fin = np.isfinite(a) & np.isfinite(b)
+    return float(np.max(np.abs(a[fin] - b[fin]))) if fin.any() else 0.0
+
+
+UNIT = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "unit_*.npz")))
+
+
+@pytest.mark.parametrize("name", UNIT)
+def test_dropin_call_vs_reference(name, oracle_lib):
+    """One drop-in call on random fields vs the reference's own output."""
+    z = load_golden(name)
+    deck = deck_from_json(z["deck"])
+    nml = json.loads(str(z["nml"]))
+    p, _ = params_from_deck(deck, oracle_lib)
+    if "it_set" in nml:
+        p.it = int(round(nml["it_set"]))
+    if "tp_set" in nml:
+        p.TP = nml["tp_set"]
+    print(f"[dropin] {name}", flush=True)
+    with capi.Context(p, [z[f"in_{nm}"] for nm in FIELDS]) as ctx:
+        call = str(z["call"])
+        if call == "temperatura":
+            ctx.temperatura(p)
+            out = dict(zip(FIELDS, ctx.download_all()))
+            assert np.array_equal(out["T_new"], z["out_T_new"])  # no log: bit-exact
+        elif call == "fase":
+            probe = ctx.equacao_campo_fase(p, want_probe=True)
+            out = dict(zip(FIELDS, ctx.download_all()))
+            assert _rel(out["phi_new"], z["out_phi_new"]) <= 1e-13
+            assert _rel(out["dpp"], z["out_dpp"]) <= 1e-13
+            exact = np.mean(out["phi_new"] == z["out_phi_new"])
+            print(f"  phi_new bit-identical fraction {exact:.6f}")
+            assert exact > 0.99
+            np.testing.assert_allclose(probe, z["probe_print"], rtol=6e-4)
+        elif call == "soluto":
+            ctx.concentracao_soluto(p)
+            out = dict(zip(FIELDS, ctx.download_all()))
+            assert _rel(out["c_new"], z["out_c_new"]) <= 1e-13
+            print(f"  c_new bit-identical fraction {np.mean(out['c_new'] == z['out_c_new']):.6f}")
+        elif call == "diag":
+            d = ctx.diagnosticos(p)
+            ref = z["scalars"][SCALARS.index("L2"):SCALARS.index("L2") + 5]
+            np.testing.assert_allclose(d, ref, rtol=1e-13)
+
+
+RUNS = sorted(os.path.basename(p) for p in glob.glob(os.path.join(GOLDEN, "run_*.npz")))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("name", RUNS)
+def test_time_loop_vs_reference(name, fused):
+    """The reference's time loop (its own dumps) through pf_run_steps, fused
+    and one-call-per-subroutine."""
+    z = load_golden(name)
+    deck = deck_from_json(z["deck"])
+    print(f"[loop] {name} fused={fused}", flush=True)
+    sim = Simulation(deck)
+    sim.ctx.set_fused(fused)
+    done = 0
+    for st in z["dumps"]:
+        sim.run(int(st) - done)
+        done = int(st)
+        f = sim.fields()
+        s = dict(zip(SCALARS, z[f"s{st}_scalars"]))
+        assert sim.p.it == s["it"] and float(sim.L.time) == s["time"]
+        assert (sim.p.TP, sim.p.M0) == (s["TP"], s["M0"])
+        ea = _abs(f["phi_new"], z[f"s{st}_phi_new"])
+        ec = _abs(f["c_new"], z[f"s{st}_c_new"])
+        eT = _abs(f["T_new"], z[f"s{st}_T_new"])
+        print(f"  step {st}: max|dphi| {ea:.3e} max|dc| {ec:.3e} max|dT| {eT:.3e}")
+        assert ea <= 1e-10 and ec <= 1e-10 and eT <= 1e-8
+        if not fused:
+            assert _abs(f["dpp"], z[f"s{st}_dpp"]) <= 1e-10
+    sim.close()
+
+
+def _growth_deck(n=(24, 22, 30), **kw):
+    base = dict(Nx=n[0], Ny=n[1], Nz=n[2], tipo_inicial=5, R_semente=5.0, h_quintico=0,
+                ruido_tipo=1, AAA=0.0122474, taxa_resfr=0.0, TP=905.0, n_varreduras=2,
+                stoptime=0.0)
+    base.update(kw)
+    return Deck(**base)
+
+
+CASES = {
+    "seed_bottom_cubic_noise1": dict(),
+    "seed_bottom_quintic_smooth_nv1": dict(h_quintico=1, ruido_tipo=2, n_varreduras=1),
+    "planar_cooling_noise0": dict(tipo_inicial=3, taxa_resfr=19.84, ruido_tipo=0, AAA=0.03),
+    "seed_nv3": dict(n_varreduras=3, ruido_tipo=0, AAA=0.01),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fused_vs_oracle(case, oracle_lib):
+    """Fused pf_run_steps vs the oracle for 40 steps, split over calls
+    (exercises the state hand-over between calls), with per-sweep probes."""
+    O = oracle_lib
+    deck = _growth_deck(**CASES[case])
+    print(f"[fused] {case}", flush=True)
+    p, L = params_from_deck(deck, O)
+    F = O.initial_fields(deck.Nx, deck.Ny, deck.Nz, 5 if deck.tipo_inicial == 5 else
+                         deck.tipo_inicial, deck.R_semente, deck.resolved().T0, deck.C0, deck.Ke)
+    sim = Simulation(deck)
+    for chunk in (1, 13, 26):
+        pr_o = O.run_steps(p, L, chunk, F, probes=True)
+        pr_g = sim.run(chunk, probes=True)
+        np.testing.assert_allclose(pr_g, pr_o, rtol=1e-9, atol=1e-12)
+    f = sim.fields()
+    ea, ec, eT = (_abs(f["phi_new"], F.phi_new), _abs(f["c_new"], F.c_new),
+                  _abs(f["T_new"], F.T_new))
+    ed = _abs(f["dpp"], F.dpp)
+    print(f"  40 steps: |dphi| {ea:.3e} |dc| {ec:.3e} |dT| {eT:.3e} |dDPP| {ed:.3e}")
+    assert ea <= 1e-10 and ec <= 1e-10 and eT <= 1e-8 and ed <= 1e-10
+    assert (sim.p.it, sim.p.TP, sim.p.M0) == (p.it, p.TP, p.M0)
+    np.testing.assert_allclose(sim.diagnostics(), O.diagnosticos(p, F.phi_new, F.c_new),
+                               rtol=1e-12)
+    sim.close()
+
+
+def test_fused_deterministic_and_matches_dropin():
+    deck = _growth_deck(n=(70, 40, 50))
+    print("[determinism]", flush=True)
+    runs = []
+    for fused in (True, True, False):
+        sim = Simulation(deck)
+        sim.ctx.set_fused(fused)
+        sim.run(30)
+        runs.append((sim.fields(), sim.diagnostics()))
+        sim.close()
+    (a, da), (b, db), (c, dc) = runs
+    for nm in ("phi_new", "c_new", "T_new", "dpp"):
+        assert np.array_equal(a[nm], b[nm]), nm  # bitwise reproducible
+    assert np.array_equal(da, db)
+    assert _abs(a["phi_new"], c["phi_new"]) <= 1e-10
+    assert _abs(a["c_new"], c["c_new"]) <= 1e-10
+    assert _abs(a["T_new"], c["T_new"]) <= 1e-8
+
+
+def test_dropin_calls_match_loop():
+    """Calling the five drop-in subroutines from the host in principal.f90's
+    order gives the same state as pf_run_steps's unfused loop, bit for bit."""
+    deck = _growth_deck(n=(20, 18, 16))
+    a = Simulation(deck)
+    probes = []
+    a.step_dropin(7, probes)
+    b = Simulation(deck)
+    b.ctx.set_fused(False)
+    pr = b.run(7, probes=True)
+    fa, fb = a.fields(), b.fields()
+    for nm in FIELDS:
+        if nm == "c_old":
+            continue  # the reference leaves a scratch snapshot there
+        assert np.array_equal(fa[nm], fb[nm]), nm
+    np.testing.assert_array_equal(np.array(probes).reshape(-1), pr)
+    a.close()
+    b.close()
+
+
+def test_fp32_mixed_vs_oracle(oracle_lib):
+    O = oracle_lib
+    deck = _growth_deck(n=(32, 30, 40))
+    print("[fp32]", flush=True)
+    p, L = params_from_deck(deck, O)
+    d = deck.resolved()
+    F = O.initial_fields(d.Nx, d.Ny, d.Nz, 5, d.R_semente, d.T0, d.C0, d.Ke)
+    O.run_steps(p, L, 60, F)
+    sim = Simulation(deck, precision=capi.PF_FP32_MIXED)
+    sim.run(60)
+    f = sim.fields()
+    ea, eT = _abs(f["phi_new"], F.phi_new), _abs(f["T_new"], F.T_new)
+    ec = _abs(f["c_new"], F.c_new) / d.C0
+    print(f"  fp32 60 steps: |dphi| {ea:.3e} |dc|/C0 {ec:.3e} |dT| {eT:.3e}")
+    assert ea <= 1e-3 and ec <= 1e-3 and eT <= 1e-2
+    sim.close()
+
+
+def test_api_state_errors():
+    deck = _growth_deck(n=(8, 8, 8))
+    s = setup_from_deck(deck)
+    ctx = capi.Context(s.p, initial_fields(deck))
+    q = s.p.copy()
+    q.nx = 9
+    with pytest.raises(capi.PfError):
+        ctx.temperatura(q)
+    q = s.p.copy()
+    q.precision = capi.PF_FP32_MIXED
+    with pytest.raises(capi.PfError):
+        ctx.concentracao_soluto(q)
+    assert ctx.device_bytes() >= 7 * 8 ** 3 * 8
+    ctx.close()
