@@ -199,6 +199,10 @@ def main(argv=None):
         "whole_step_alg_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
         "diag_final": r64["diag"],
     }
+    gbs_copy = capi.stream_copy_gbs(2 << 30, 20)
+    out["roofline"]["measured_copy_gbs"] = round(gbs_copy, 1)
+    out["roofline"]["frac_of_measured_copy"] = round(rl["achieved_gbs"] / gbs_copy, 4)
+    print(f"[bench] STREAM copy {gbs_copy:.0f} GB/s", flush=True)
     if not a.no_fp32:
         print("[bench] FP32-mixed", flush=True)
         r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup)

@@ -200,6 +200,11 @@ int pf_synchronize(pf_ctx *ctx);
 #define PF_KT_TEMPERATURA 3 /* standalone temperatura                       */
 #define PF_KT_DROPIN 4      /* drop-in phase / solute kernels               */
 #define PF_KT_COUNT 5
+/* STREAM-style copy of `bytes` bytes (FP64 elements, coalesced, one read
+ * and one write per element) repeated `iters` times on the current device;
+ * reports the copy bandwidth (read + write bytes / average kernel time).
+ * Used as the measured HBM peak and to calibrate HBM byte counters. */
+int pf_stream_copy(size_t bytes, int32_t iters, double *gbs);
 /* on != 0 starts recording (and clears earlier records); 0 stops. */
 int pf_timing_enable(pf_ctx *ctx, int32_t on);
 /* Synchronous: sums of durations [ms] and launch counts per kind since the

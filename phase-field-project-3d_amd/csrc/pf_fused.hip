@@ -80,20 +80,24 @@ struct Node {
   R e2, xy, yz, zx;
 };
 
-// Node quantities from the centred gradient at a node (phase_body.inc:55-129).
+// Node quantities from the centred gradient at a node (phase_body.inc:55-129),
+// written with the gradient normalised by Q = |g|^2 + small:
+//   S4/Q^2 = a^2+b^2+c^2 with a = gx^2/Q ...,
+//   gx gy (gx^2-gy^2)/Q^2 = (gx gy/Q)(a-b) ...
+// which is the same quantity but never forms Q^2 (1e-46 for a flat region,
+// below the FP32 range).
 template <typename R>
 __device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
   const R gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
-  const R Q = gx2 + gy2 + gz2 + s.small;
-  const R iQ2 = (R)1 / (Q * Q);
-  const R S4 = gx2 * gx2 + gy2 * gy2 + gz2 * gz2;
-  const R eps = s.E0a + s.E0b * S4 * iQ2;
-  const R f = s.K * eps * iQ2;
+  const R rq = (R)1 / (gx2 + gy2 + gz2 + s.small);
+  const R a = gx2 * rq, b = gy2 * rq, c = gz2 * rq;
+  const R eps = s.E0a + s.E0b * (a * a + b * b + c * c);
+  const R f = s.K * eps;
   Node<R> v;
   v.e2 = eps * eps;
-  v.xy = f * gx * gy * (gx2 - gy2);
-  v.yz = f * gy * gz * (gy2 - gz2);
-  v.zx = f * gz * gx * (gz2 - gx2);
+  v.xy = f * (gx * gy * rq) * (a - b);
+  v.yz = f * (gy * gz * rq) * (b - c);
+  v.zx = f * (gz * gx * rq) * (c - a);
   return v;
 }
 

@@ -64,6 +64,31 @@ __global__ void __launch_bounds__(256) k_temperatura(KParams p, const R *__restr
 }
 
 // ---------------------------------------------------------------------------
+// One "face" point of phase_body.inc:55-129: eps and the three cross
+// coefficients from the centred gradient at a node.  FP64 keeps the
+// reference's expressions and order; FP32 uses the same quantities with the
+// gradient normalised by Q (Q^2 = 1e-46 in a flat region is below the FP32
+// range).
+template <typename R>
+__device__ __forceinline__ void node_terms(R gx, R gy, R gz, R E0, R DE, R K, R small, R &eps,
+                                           R &exy, R &eyz, R &ezx) {
+  if constexpr (sizeof(R) == 8) {
+    const R Q = gx * gx + gy * gy + gz * gz + small;
+    const R S4 = (gx * (gx * (gx * gx))) + (gy * (gy * (gy * gy))) + (gz * (gz * (gz * gz)));
+    eps = E0 * ((R)1 + DE * ((R)4 * S4 / (Q * Q) - (R)3));
+    exy = K * eps * gx * gy * (gx * gx - gy * gy) / (Q * Q);
+    eyz = K * eps * gy * gz * (gy * gy - gz * gz) / (Q * Q);
+    ezx = K * eps * gz * gx * (gz * gz - gx * gx) / (Q * Q);
+  } else {
+    const R rq = (R)1 / (gx * gx + gy * gy + gz * gz + small);
+    const R a = gx * gx * rq, b = gy * gy * rq, c = gz * gz * rq;
+    eps = E0 * ((R)1 + DE * ((R)4 * (a * a + b * b + c * c) - (R)3));
+    exy = K * eps * (gx * gy * rq) * (a - b);
+    eyz = K * eps * (gy * gz * rq) * (b - c);
+    ezx = K * eps * (gz * gx * rq) * (c - a);
+  }
+}
+
 // phase_body.inc:14-198 for one interior cell, reading phi_old from memory.
 template <typename R>
 __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict__ po,
@@ -84,40 +109,31 @@ __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict
   const R K = (R)16 * E0 * DE;
 #define SQ(x) ((x) * (x))
 #define P4(x) ((x) * ((x) * ((x) * (x))))
-#define NODE(gx, gy, gz, Q, eps)                                   \
-  const R Q = SQ(gx) + SQ(gy) + SQ(gz) + small;                    \
-  const R eps = E0 * ((R)1 + DE * ((R)4 * (P4(gx) + P4(gy) + P4(gz)) / SQ(Q) - (R)3));
   // E: node (i+1,j,k)
   const R gxE = (P(2 * sx) - PP) * h5 / dx, gyE = (PNE - PSE) * h5 / dy, gzE = (PET - PEB) * h5 / dz;
-  NODE(gxE, gyE, gzE, QE, epsE)
-  const R exyE = K * epsE * gxE * gyE * (SQ(gxE) - SQ(gyE)) / SQ(QE);
-  const R ezxE = K * epsE * gzE * gxE * (SQ(gzE) - SQ(gxE)) / SQ(QE);
+  R epsE, exyE, eyzE, ezxE;
+  node_terms<R>(gxE, gyE, gzE, E0, DE, K, small, epsE, exyE, eyzE, ezxE);
   // W: node (i-1,j,k)
   const R gxW = (PP - P(-2 * sx)) * h5 / dx, gyW = (PNW - PSW) * h5 / dy, gzW = (PWT - PWB) * h5 / dz;
-  NODE(gxW, gyW, gzW, QW, epsW)
-  const R exyW = K * epsW * gxW * gyW * (SQ(gxW) - SQ(gyW)) / SQ(QW);
-  const R ezxW = K * epsW * gzW * gxW * (SQ(gzW) - SQ(gxW)) / SQ(QW);
+  R epsW, exyW, eyzW, ezxW;
+  node_terms<R>(gxW, gyW, gzW, E0, DE, K, small, epsW, exyW, eyzW, ezxW);
   // N: node (i,j+1,k)
   const R gxN = (PNE - PNW) * h5 / dx, gyN = (P(2 * sy) - PP) * h5 / dy, gzN = (PNT - PNB) * h5 / dz;
-  NODE(gxN, gyN, gzN, QN, epsN)
-  const R exyN = K * epsN * gxN * gyN * (SQ(gxN) - SQ(gyN)) / SQ(QN);
-  const R eyzN = K * epsN * gyN * gzN * (SQ(gyN) - SQ(gzN)) / SQ(QN);
+  R epsN, exyN, eyzN, ezxN;
+  node_terms<R>(gxN, gyN, gzN, E0, DE, K, small, epsN, exyN, eyzN, ezxN);
   // S: node (i,j-1,k)
   const R gxS = (PSE - PSW) * h5 / dx, gyS = (PP - P(-2 * sy)) * h5 / dy, gzS = (PST - PSB) * h5 / dz;
-  NODE(gxS, gyS, gzS, QS, epsS)
-  const R exyS = K * epsS * gxS * gyS * (SQ(gxS) - SQ(gyS)) / SQ(QS);
-  const R eyzS = K * epsS * gyS * gzS * (SQ(gyS) - SQ(gzS)) / SQ(QS);
+  R epsS, exyS, eyzS, ezxS;
+  node_terms<R>(gxS, gyS, gzS, E0, DE, K, small, epsS, exyS, eyzS, ezxS);
   // T: node (i,j,k+1)
   const R gxT = (PET - PWT) * h5 / dx, gyT = (PNT - PST) * h5 / dy, gzT = (P(2 * sz) - PP) * h5 / dz;
-  NODE(gxT, gyT, gzT, QT, epsT)
-  const R eyzT = K * epsT * gyT * gzT * (SQ(gyT) - SQ(gzT)) / SQ(QT);
-  const R ezxT = K * epsT * gzT * gxT * (SQ(gzT) - SQ(gxT)) / SQ(QT);
+  R epsT, exyT, eyzT, ezxT;
+  node_terms<R>(gxT, gyT, gzT, E0, DE, K, small, epsT, exyT, eyzT, ezxT);
   // B: node (i,j,k-1)
   const R gxB = (PEB - PWB) * h5 / dx, gyB = (PNB - PSB) * h5 / dy, gzB = (PP - P(-2 * sz)) * h5 / dz;
-  NODE(gxB, gyB, gzB, QB, epsB)
-  const R eyzB = K * epsB * gyB * gzB * (SQ(gyB) - SQ(gzB)) / SQ(QB);
-  const R ezxB = K * epsB * gzB * gxB * (SQ(gzB) - SQ(gxB)) / SQ(QB);
-#undef NODE
+  R epsB, exyB, eyzB, ezxB;
+  node_terms<R>(gxB, gyB, gzB, E0, DE, K, small, epsB, exyB, eyzB, ezxB);
+  (void)eyzE; (void)eyzW; (void)ezxN; (void)ezxS; (void)exyT; (void)exyB;
   R AA = (SQ(epsE) * (PE - PP) + SQ(epsW) * (PW - PP)) / (dx * dx) +
          (SQ(epsN) * (PN - PP) + SQ(epsS) * (PS - PP)) / (dy * dy) +
          (SQ(epsT) * (PT - PP) + SQ(epsB) * (PB - PP)) / (dz * dz);

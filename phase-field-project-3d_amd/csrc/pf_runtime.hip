@@ -194,9 +194,49 @@ int do_solute(pf_ctx *ctx, const pf::KParams &k) {
   return PF_OK;
 }
 
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a,
+                                                     double2 *__restrict__ b, size_t n2) {
+  for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n2;
+       q += (size_t)gridDim.x * blockDim.x)
+    b[q] = a[q];
+}
+
 }  // namespace
 
 extern "C" {
+
+int pf_stream_copy(size_t bytes, int32_t iters, double *gbs) {
+  if (!gbs || bytes < 1024 || iters < 1) return fail(PF_E_ARG, "bad stream-copy arguments");
+  const size_t n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PF_OK;
+  if (hipMalloc(&a, n2 * 16) != hipSuccess || hipMalloc(&b, n2 * 16) != hipSuccess) {
+    rc = fail(PF_E_NOMEM, "stream-copy buffers");
+  } else if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+             hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(PF_E_HIP, "stream-copy stream/events");
+  } else {
+    hipMemsetAsync(a, 0, n2 * 16, s);
+    const dim3 grd(8192), blk(256);
+    hipLaunchKernelGGL(k_stream_copy, grd, blk, 0, s, a, b, n2);  // warm-up
+    hipEventRecord(e0, s);
+    for (int q = 0; q < iters; ++q) hipLaunchKernelGGL(k_stream_copy, grd, blk, 0, s, a, b, n2);
+    hipEventRecord(e1, s);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = fail(PF_E_HIP, "stream-copy timing");
+    else
+      *gbs = 2.0 * (double)(n2 * 16) * iters / ((double)ms * 1e-3) / 1e9;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
+}
 
 const char *pf_version(void) { return "pfamd 0.1 (gfx950)"; }
 const char *pf_last_error(void) { return g_err.c_str(); }

@@ -71,7 +71,7 @@ EXPORTS = (
     "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
-    "pf_timing_enable", "pf_timing_read", "pf_set_fused",
+    "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy",
 )
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin")
 
@@ -114,6 +114,7 @@ def load(path: str = LIB_PATH):
         "pf_timing_enable": ([vp, i32], C.c_int),
         "pf_timing_read": ([vp, vp, vp], C.c_int),
         "pf_set_fused": ([vp, i32], C.c_int),
+        "pf_stream_copy": ([C.c_size_t, i32, C.POINTER(f64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -135,6 +136,13 @@ def device_count(lib=None) -> int:
     n = C.c_int32(0)
     check(lib.pf_device_count(C.byref(n)), lib)
     return n.value
+
+
+def stream_copy_gbs(nbytes: int, iters: int = 20, lib=None) -> float:
+    lib = lib or load()
+    g = C.c_double(0.0)
+    check(lib.pf_stream_copy(int(nbytes), int(iters), C.byref(g)), lib)
+    return g.value
 
 
 def w0_e0(lam, ajuste, sigma, lib=None):

@@ -26,9 +26,7 @@ def oracle_lib():
 
 
 def gpu_available() -> bool:
-    try:
-        from pfamd import capi
-
-        return capi.device_count() > 0
-    except Exception:
-        return False
+    """True on a machine with an AMD GPU driver (/dev/kfd).  Deliberately
+    independent of libpfamd.so: on a GPU box a library that fails to load
+    must fail the GPU tests, not skip them."""
+    return os.path.exists("/dev/kfd")
