@@ -61,7 +61,7 @@ def _sync(sim):
         pass
 
 
-def run_form(precision, steps, warmup, timing=True):
+def run_form(precision, steps, warmup, timing=True, autotune=True):
     """Warm-up, then exactly `steps` timed steps; returns timings."""
     from pfamd.run import Simulation
 
@@ -72,6 +72,10 @@ def run_form(precision, steps, warmup, timing=True):
     t_init = time.time() - t0
     if warmup:
         sim.run(warmup)
+    # pick the sweep tiling on this grid (times every candidate once, before
+    # the timed region; results do not depend on it)
+    cands = sim.ctx.autotune(sim.p, reps=3) if autotune else []
+    tiling = sim.ctx.tiling()
     _sync(sim)
     sim.ctx.timing(timing)
     done = 0
@@ -90,7 +94,7 @@ def run_form(precision, steps, warmup, timing=True):
     nbytes = sim.ctx.device_bytes()
     sim.close()
     return {"wall_s": wall, "kernels": kt, "diag": diag.tolist(), "init_s": t_init,
-            "device_bytes": nbytes}
+            "device_bytes": nbytes, "tiling": tiling, "autotune": cands}
 
 
 def roofline(res, precision_bytes):
@@ -157,6 +161,19 @@ def cpu_baseline():
                       f"OMP_NUM_THREADS={threads}, {t:.2f} s"}
 
 
+def log(msg):
+    """Progress lines go to stderr; stdout carries only the JSON result line."""
+    print(msg, file=sys.stderr, flush=True)
+
+
+def tiling_summary(res):
+    from pfamd.capi import TILE_SHAPES
+
+    ta, tb, sa, sb = res["tiling"]
+    return {"sweep_A": {"tile": "x".join(map(str, TILE_SHAPES[ta])), "z_planes_per_group": sa},
+            "sweep_B": {"tile": "x".join(map(str, TILE_SHAPES[tb])), "z_planes_per_group": sb}}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,11 +191,12 @@ def main(argv=None):
     if capi.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
-    print(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps", flush=True)
+    log(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps")
     r64 = run_form(capi.PF_FP64, a.steps, a.warmup)
     ncell = NX * NY * NZ
     mlups = ncell * a.steps / r64["wall_s"] / 1e6
-    print(f"[bench] FP64 {mlups:.1f} MLUPS, kernels {r64['kernels']}", flush=True)
+    log(f"[bench] FP64 autotune {r64['autotune']}")
+    log(f"[bench] FP64 {mlups:.1f} MLUPS, tiling {r64['tiling']}, kernels {r64['kernels']}")
     rl = roofline(r64, 8)
     out = {
         "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
@@ -198,24 +216,28 @@ def main(argv=None):
         "kernels_launches": {k: v[1] for k, v in r64["kernels"].items()},
         "whole_step_alg_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
         "diag_final": r64["diag"],
+        "tiling": tiling_summary(r64),
     }
     gbs_copy = capi.stream_copy_gbs(2 << 30, 20)
     out["roofline"]["measured_copy_gbs"] = round(gbs_copy, 1)
     out["roofline"]["frac_of_measured_copy"] = round(rl["achieved_gbs"] / gbs_copy, 4)
-    print(f"[bench] STREAM copy {gbs_copy:.0f} GB/s", flush=True)
+    log(f"[bench] STREAM copy {gbs_copy:.0f} GB/s")
     if not a.no_fp32:
-        print("[bench] FP32-mixed", flush=True)
+        log("[bench] FP32-mixed")
         r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup)
         m32 = ncell * a.steps / r32["wall_s"] / 1e6
+        log(f"[bench] FP32 autotune {r32['autotune']}")
+        log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['tiling']}, kernels {r32['kernels']}")
         rl32 = roofline(r32, 4)
         out["fp32_mixed"] = {"value": round(m32, 2), "unit": "MLUPS",
                              "ms_per_step": r32["wall_s"] / a.steps * 1e3,
                              "speedup_vs_fp64": round(m32 / mlups, 3),
                              "roofline_kernel": rl32["kernel"],
                              "roofline_achieved_gbs": round(rl32["achieved_gbs"], 1),
+                             "tiling": tiling_summary(r32),
                              "diag_final": r32["diag"]}
     if not a.no_cpu_baseline:
-        print("[bench] CPU baseline (reference OpenMP path)", flush=True)
+        log("[bench] CPU baseline (reference OpenMP path)")
         cb = cpu_baseline()
         out["cpu_baseline"] = cb
         out["gpu_vs_cpu"] = round(mlups / cb["value"], 1)

@@ -187,6 +187,22 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps,
 /* on = 0 makes pf_run_steps issue the drop-in kernels one reference call at
  * a time instead of the fused sweeps (default 1). */
 int pf_set_fused(pf_ctx *ctx, int32_t on);
+/* Work-group tiling of the fused sweeps (no counterpart in the reference,
+ * whose loops are collapsed over the whole grid, equacao_campo_fase.f90:27).
+ * Tile shapes: 0 = 64x8, 1 = 32x8, 2 = 64x4, 3 = 32x16 cells per work-group;
+ * the z-march is split into segments so that a sweep launches about
+ * target_groups work-groups.  The tiling never changes results: every cell
+ * is computed by the same arithmetic in any tiling. */
+int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_groups);
+/* out = {tile of sweep A, tile of sweep B, z-planes per segment A, B}. */
+int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
+/* Times each tile shape x segment split of both sweeps on the context's
+ * current fields (reps launches each after one warm-up, writing into scratch
+ * buffers it allocates and frees: the state is untouched) and keeps the
+ * fastest per sweep.  If ms is non-NULL it receives 4 doubles per candidate
+ * {sweep (0 = A, 1 = B), tile, target_groups, average ms}, at most 64
+ * candidates, their number in *ncand.  Synchronous. */
+int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms);
 /* Wait for all queued work of the context. */
 int pf_synchronize(pf_ctx *ctx);
 

@@ -18,9 +18,14 @@
 
 namespace pf {
 
+// Tile shapes of the sweep kernels: 0 = 64x8, 1 = 32x8, 2 = 64x4, 3 = 32x16.
+constexpr int PF_NTILES = 4;
+
 struct FusedState {
   int nx = 0, ny = 0, nz = 0, prec = PF_FP64;
-  int seg_a = 0, seg_b = 0;  // z-planes per work-group segment (autotuned)
+  int tile_a = 0, tile_b = 0;  // tile shape of sweep A / sweep B
+  int target = 2048;           // work-groups per sweep the z split aims for
+  int seg_a = 0, seg_b = 0;    // z-planes per work-group segment
   bool enabled = false;
 };
 
@@ -58,6 +63,18 @@ struct FusedStep {
 };
 
 int fused_init(FusedState &s, int nx, int ny, int nz, int prec);
+// Tile shape per sweep and the work-group count the z split aims for.
+int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups);
+// Times every tile shape x z-split candidate of both sweeps on the given
+// fields (writing into scratch buffers of its own) and keeps the fastest.
+constexpr int PF_AUTOTUNE_MAX = 64;
+struct AutotuneResult {
+  int n = 0;
+  struct { int sweep, tile, target; float ms; } cand[PF_AUTOTUNE_MAX];
+  float best_ms[2] = {0, 0};
+};
+int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, void *T_old,
+                   int reps, hipStream_t stream, AutotuneResult *res);
 void fused_free(FusedState &s);
 size_t fused_bytes(const FusedState &s);
 bool fused_supported(const FusedState &s, const pf_params &p);

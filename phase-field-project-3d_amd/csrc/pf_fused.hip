@@ -17,9 +17,10 @@
 // sweep B reads phi, c and writes phi, c, T.
 //
 // Arithmetic follows the reference's formulas with algebraic reuse
-// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g')), so
-// results agree with the reference to rounding, not bit for bit; the drop-in
-// kernels keep the bit-faithful form.
+// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g'),
+// divisions as a refined reciprocal), so results agree with the reference
+// to rounding, not bit for bit; the drop-in kernels keep the bit-faithful
+// form.
 #include <hip/hip_runtime.h>
 
 #include "pf_common.h"
@@ -29,16 +30,42 @@
 namespace pf {
 namespace {
 
-constexpr int TX = 64, TY = 8, NT = TX * TY;
-constexpr int PX = TX + 4, PY = TY + 4, PPL = PX * PY;  // phi plane, halo 2
-constexpr int QX = TX + 2, QY = TY + 2, QPL = QX * QY;  // node plane, halo 1
-constexpr int NRING = 2 * QX + 2 * TY;                   // ring nodes per plane
 constexpr int NSLOT = 5;
+
+// Tile shape: TX x TY cells per work-group (one thread per column).
+template <int TX_, int TY_>
+struct Tile {
+  static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
+  static constexpr int PX = TX + 4, PY = TY + 4, PPL = PX * PY;  // phi plane, halo 2
+  static constexpr int QX = TX + 2, QY = TY + 2, QPL = QX * QY;  // node plane, halo 1
+  static constexpr int NRING = 2 * QX + 2 * TY;                  // ring nodes per plane
+  static constexpr int NLD = (PPL + NT - 1) / NT;                 // phi loads per thread
+  static_assert(NRING <= NT, "one ring node per thread at most");
+  static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
+};
 
 template <typename R>
 __device__ __forceinline__ R dlog(R x) { return log(x); }
 template <>
 __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+
+// 1/x.  FP64: the hardware estimate refined by two Newton steps (about 1 ulp,
+// half the instructions of a correctly rounded division); for x = 0, +-inf or
+// NaN the estimate itself is returned (inf, 0, NaN), as a division would give.
+// FP32: the hardware reciprocal (1 ulp).
+template <typename R>
+__device__ __forceinline__ R rcpn(R x);
+template <>
+__device__ __forceinline__ double rcpn<double>(double x) {
+  const double r0 = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r0, 1.0);
+  double r = fma(r0, e, r0);
+  e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  return r != r ? r0 : r;
+}
+template <>
+__device__ __forceinline__ float rcpn<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Scalars of one sweep, in the arithmetic type.
 template <typename R>
@@ -49,9 +76,9 @@ struct SC {
   R E0a, E0b, K;       // eps = E0a + E0b*S4/Q^2, K = 16 E0 delta
   R small;
   R Ke, W0, RTV;       // RTV = R*TP/Vm
-  R omCs, omCl;        // 1-Cs_eq, 1-Cl_eq
+  R omCs, iomCl;       // 1-Cs_eq, 1/(1-Cl_eq)
   R M0dt, AAA;
-  R Dliq, Dsol, dt;
+  R Dliq2dt, Dsol2dt;  // 2*D*dt for the solute faces
   R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
 };
 
@@ -68,9 +95,9 @@ SC<R> make_sc(const KParams &p) {
   s.K = (R)(16.0 * p.E0 * p.delta_eps);
   s.small = (R)1.0e-23;
   s.Ke = (R)p.Ke; s.W0 = (R)p.W0; s.RTV = (R)(p.R * p.TP / p.Vm);
-  s.omCs = (R)(1.0 - p.Cs_eq); s.omCl = (R)(1.0 - p.Cl_eq);
+  s.omCs = (R)(1.0 - p.Cs_eq); s.iomCl = (R)(1.0 / (1.0 - p.Cl_eq));
   s.M0dt = (R)(p.M0 * p.dt); s.AAA = (R)p.AAA;
-  s.Dliq = (R)p.D_liq; s.Dsol = (R)p.D_sol; s.dt = (R)p.dt;
+  s.Dliq2dt = (R)(2.0 * p.D_liq * p.dt); s.Dsol2dt = (R)(2.0 * p.D_sol * p.dt);
   s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
   return s;
 }
@@ -89,7 +116,7 @@ struct Node {
 template <typename R>
 __device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
   const R gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
-  const R rq = (R)1 / (gx2 + gy2 + gz2 + s.small);
+  const R rq = rcpn<R>(gx2 + gy2 + gz2 + s.small);
   const R a = gx2 * rq, b = gy2 * rq, c = gz2 * rq;
   const R eps = s.E0a + s.E0b * (a * a + b * b + c * c);
   const R f = s.K * eps;
@@ -113,18 +140,18 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
 }
 
 // Solute node quantities: 1/gamma (gamma without the centre's D_C) and
-// mu = ln(CL/(1-CL)) (conc_body.inc:61-118, 139-144).
+// mu = ln(CL/(1-CL)) = ln(c/(den-c)) (conc_body.inc:61-118, 139-144).
 template <typename R>
 __device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &mu,
                                             R &CL) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
-  CL = cc / den;
+  CL = cc * rcpn<R>(den);
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
-  ig = (R)1 / g;
-  mu = dlog<R>(cc / (den - cc));
+  ig = rcpn<R>(g);
+  mu = dlog<R>(cc * rcpn<R>(den - cc));
 }
 
 struct Geo {
@@ -146,34 +173,39 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
   return true;
 }
 
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
 // SWEEP_B == false: sweep A (phase only).  SWEEP_B == true: phase + solute +
-// (WRITE_T ? next-step T : DPP).
-template <typename R, bool SWEEP_B>
-__global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po,
+// (write_T ? next-step T : DPP).
+template <typename R, bool SWEEP_B, typename TL>
+__global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po,
                                               const R *__restrict__ co, R *__restrict__ pn,
                                               R *__restrict__ cn, const R *__restrict__ Tcur,
                                               R *__restrict__ Tnext, R *__restrict__ dpp,
                                               int write_T) {
+  constexpr int TX = TL::TX, NT = TL::NT, PX = TL::PX, PPL = TL::PPL;
+  constexpr int QX = TL::QX, QY = TL::QY, QPL = TL::QPL, NRING = TL::NRING, NLD = TL::NLD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  R *sphi = reinterpret_cast<R *>(smem_raw);         // NSLOT * PPL
-  R *snode = sphi + NSLOT * PPL;                     // 4 * QPL
-  R *sgm = snode + 4 * QPL;                          // 2 * QPL (sweep B)
+  R *sphi = reinterpret_cast<R *>(smem_raw);  // NSLOT * PPL
+  R *snode = sphi + NSLOT * PPL;              // 4 * QPL
+  R *sgm = snode + 4 * QPL;                   // 2 * QPL (sweep B)
 
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
-  const int i0 = 2 + bx * TX, j0 = 2 + by * TY;  // first cell of the tile
+  const int i0 = 2 + bx * TX, j0 = 2 + by * TL::TY;  // first cell of the tile
   const int i = i0 + tx, j = j0 + ty;
   const bool own = (i <= nx - 1) && (j <= ny - 1);
-  const bool ph_cell = own && i >= 3 && i <= nx - 2 && j >= 3 && j <= ny - 2;
+  const bool ph_col = own && i >= 3 && i <= nx - 2 && j >= 3 && j <= ny - 2;
   const int ks = g.k_lo + sg * g.seg;
   const int ke = min(ks + g.seg - 1, g.k_hi);
-  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  const size_t sz = (size_t)nx * ny;
   const int quint = kp.h_quintico;
+  const bool noise = s.AAA != (R)0;
 
   // ring node handled by this thread (tid < NRING): its node-frame coords
-  int rlx = -1, rly = -1;
+  int rlx = 0, rly = 0;
   if (tid < NRING) {
     if (tid < QX) { rlx = tid; rly = 0; }
     else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
@@ -181,31 +213,34 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
   }
   const int olx = tx + 1, oly = ty + 1;  // own node-frame coords
 
-  // phi plane loader: each thread moves elements e = tid, tid+NT of the
-  // (PY x PX) halo-2 window; global cells outside the grid are clamped.
-  R pre[2];
+  // In-plane offsets (32 bit; nx*ny < 2^31) of the phi window elements this
+  // thread loads, of its own column and of its ring column; clamped so every
+  // load stays inside the grid (clamped values feed only unused nodes).
+  int loff[NLD];
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = tid + u * NT;
+    const int gi = clampi(i0 - 2 + (e % PX), 1, nx), gj = clampi(j0 - 2 + (e / PX), 1, ny);
+    loff[u] = (gi - 1) + (gj - 1) * nx;
+  }
+  const int ooff = (clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx;
+  const int roff = (clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx;
+  auto plane = [&](int q) -> size_t { return (size_t)(clampi(q, 1, nz) - 1) * sz; };
+
+  R pre[NLD];
   auto gload = [&](int q) {
-    const int qc = q < 1 ? 1 : (q > nz ? nz : q);
+    const R *src = po + plane(q);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int e = tid + u * NT;
-      if (e < PPL) {
-        int gi = i0 - 2 + (e % PX), gj = j0 - 2 + (e / PX);
-        gi = gi < 1 ? 1 : (gi > nx ? nx : gi);
-        gj = gj < 1 ? 1 : (gj > ny ? ny : gj);
-        pre[u] = po[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(qc - 1) * sz];
-      }
-    }
+    for (int u = 0; u < NLD; ++u)
+      if (tid + u * NT < PPL) pre[u] = src[loff[u]];
   };
+  auto slot = [&](int q) -> R * { return sphi + (q % NSLOT) * PPL; };  // q >= 0
   auto lstore = [&](int q) {
-    R *dst = sphi + (((q % NSLOT) + NSLOT) % NSLOT) * PPL;
+    R *dst = slot(q);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int e = tid + u * NT;
-      if (e < PPL) dst[e] = pre[u];
-    }
+    for (int u = 0; u < NLD; ++u)
+      if (tid + u * NT < PPL) dst[tid + u * NT] = pre[u];
   };
-  auto slot = [&](int q) -> const R * { return sphi + (((q % NSLOT) + NSLOT) % NSLOT) * PPL; };
 
   // node at node-frame (lx,ly) of plane q-1 from phi planes q-2, q-1, q
   auto node_at = [&](int q, int lx, int ly) -> Node<R> {
@@ -221,7 +256,7 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
   // solute quantities of planes k-1, k, k+1
   Node<R> nM{}, nZ{}, nP{};
   R igM = 0, muM = 0, igZ = 0, muZ = 0, clZ = 0, igP = 0, muP = 0, clP = 0;
-  R cZ = 0, cP_ = 0;  // c at planes k and k+1 (own column)
+  R cZ = 0;
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
 
   const int q_first = ks - 2, q_last = ke + 2;
@@ -232,28 +267,21 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
     __syncthreads();
     const int kn = q - 1;  // plane whose node / solute quantities are built now
     Node<R> nr{};
-    R rig = 0, rmu = 0;
-    R cnew_col = 0;
+    R rig = 0, rmu = 0, cnew_col = 0;
     if (q >= ks) {
-      // nodes are only used for planes [2, nz-1]
-      const bool nvalid = kn >= 2 && kn <= nz - 1;
-      if (nvalid) {
+      if (kn >= 2 && kn <= nz - 1) {  // nodes are only used for planes [2, nz-1]
         nP = node_at(q, olx, oly);
         if (tid < NRING) nr = node_at(q, rlx, rly);
       }
       if constexpr (SWEEP_B) {
         if (kn >= 1 && kn <= nz) {
           const R *P0 = slot(q - 1);
-          {
-            const int gi = min(max(i, 1), nx), gj = min(max(j, 1), ny);
-            cnew_col = co[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(kn - 1) * sz];
-            solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP, clP);
-          }
+          const size_t pk = (size_t)(kn - 1) * sz;
+          cnew_col = co[pk + ooff];
+          solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP, clP);
           if (tid < NRING) {
-            const int gi = min(max(i0 - 1 + rlx, 1), nx), gj = min(max(j0 - 1 + rly, 1), ny);
-            const R cr = co[(size_t)(gi - 1) + (size_t)(gj - 1) * sy + (size_t)(kn - 1) * sz];
             R dummy;
-            solute_node(P0[(rly + 1) * PX + rlx + 1], cr, quint, s, rig, rmu, dummy);
+            solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu, dummy);
           }
         }
       }
@@ -263,11 +291,11 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = P0[c];
-      const size_t gidx = (size_t)(i - 1) + (size_t)(j - 1) * sy + (size_t)(k - 1) * sz;
+      const size_t gidx = (size_t)(k - 1) * sz + ooff;
       R dppv = (R)0, pnew = PP;
       R cPP;
       if constexpr (SWEEP_B) cPP = cZ; else cPP = co[gidx];
-      if (ph_cell && k >= 3 && k <= nz - 2) {
+      if (ph_col && k >= 3 && k <= nz - 2) {
         const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
         const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
                 PSW = P0[c - PX - 1];
@@ -300,16 +328,19 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
         h_of(PP, quint, h, hl);
         R CLP;
         if constexpr (SWEEP_B) CLP = clZ;
-        else CLP = cPP / (s.Ke * h + ((R)1 - h));
+        else CLP = cPP * rcpn<R>(s.Ke * h + ((R)1 - h));
         const R CSP = s.Ke * CLP;
-        const R GG = hl * s.RTV * dlog<R>(fabs((s.omCs * ((R)1 - CLP)) / (s.omCl * ((R)1 - CSP)))) -
-                     s.W0 * gl;
-        const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
-                              ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
-                              : (float)hash_unit(hbase);
-        const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
-                                                              : ((R)2 * (R)ran - (R)1) * s.AAA;
-        const R ruidos = amp * (R)16 * (PP * PP) * (om * om);
+        const R arg = (s.omCs * ((R)1 - CLP)) * s.iomCl * rcpn<R>((R)1 - CSP);
+        const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
+        R ruidos = (R)0;
+        if (noise) {
+          const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
+                                ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                                : (float)hash_unit(hbase);
+          const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
+                                                                : ((R)2 * (R)ran - (R)1) * s.AAA;
+          ruidos = amp * (R)16 * (PP * PP) * (om * om);
+        }
         const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
         dppv = pstar - PP;
         const R cl = pstar > (R)1 ? (R)1 : pstar;
@@ -317,16 +348,18 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
         pn[gidx] = pnew;
       }
       if constexpr (SWEEP_B) {
-        // solute at (i,j,k), k in [2,nz-1] (conc_body.inc:92-144)
+        // solute at (i,j,k), k in [2,nz-1] (conc_body.inc:92-144): face
+        // mobility 2 D_C/(1/g_P + 1/g_X), flux along mu differences
         const int nc = oly * QX + olx;
         const R *ig = sgm, *mu = sgm + QPL;
-        const R DC = PP <= (R)0.95 ? s.Dliq : s.Dsol;
+        const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R igc = igZ, muc = muZ;
-        const R fx = (mu[nc + 1] - muc) / (ig[nc + 1] + igc) + (mu[nc - 1] - muc) / (ig[nc - 1] + igc);
-        const R fy = (mu[nc + QX] - muc) / (ig[nc + QX] + igc) +
-                     (mu[nc - QX] - muc) / (ig[nc - QX] + igc);
-        const R fz = (muP - muc) / (igP + igc) + (muM - muc) / (igM + igc);
-        cn[gidx] = cPP + s.dt * ((R)2 * DC) * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        const R fx = (mu[nc + 1] - muc) * rcpn<R>(ig[nc + 1] + igc) +
+                     (mu[nc - 1] - muc) * rcpn<R>(ig[nc - 1] + igc);
+        const R fy = (mu[nc + QX] - muc) * rcpn<R>(ig[nc + QX] + igc) +
+                     (mu[nc - QX] - muc) * rcpn<R>(ig[nc - QX] + igc);
+        const R fz = (muP - muc) * rcpn<R>(igP + igc) + (muM - muc) * rcpn<R>(igM + igc);
+        cn[gidx] = cPP + D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
         if (write_T) {
           // temp_body.inc:31-41 of the next step: TP + D dt lap(T) + fonte
           R hh, hl2;
@@ -335,11 +368,11 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
           if (s.Ddt != (R)0) {
             const R TP2 = (R)2 * s.TP;
             lap = (Tcur[gidx + 1] - TP2 + Tcur[gidx - 1]) * s.idx2 +
-                  (Tcur[gidx + sy] - TP2 + Tcur[gidx - sy]) * s.idy2 +
+                  (Tcur[gidx + nx] - TP2 + Tcur[gidx - nx]) * s.idy2 +
                   (Tcur[gidx + sz] - TP2 + Tcur[gidx - sz]) * s.idz2;
           }
           Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
-        } else if (ph_cell && k >= 3 && k <= nz - 2) {
+        } else if (ph_col && k >= 3 && k <= nz - 2) {
           dpp[gidx] = dppv;
         }
       } else {
@@ -350,22 +383,18 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
     __syncthreads();
     // publish plane kn's node / solute quantities and rotate the column
     if (q >= ks) {
-      const bool nvalid = kn >= 2 && kn <= nz - 1;
-      if (nvalid) {
-        const int oc = oly * QX + olx;
+      const int oc = oly * QX + olx, rc = rly * QX + rlx;
+      if (kn >= 2 && kn <= nz - 1) {
         snode[oc] = nP.e2; snode[QPL + oc] = nP.xy; snode[2 * QPL + oc] = nP.yz;
         snode[3 * QPL + oc] = nP.zx;
         if (tid < NRING) {
-          const int rc = rly * QX + rlx;
           snode[rc] = nr.e2; snode[QPL + rc] = nr.xy; snode[2 * QPL + rc] = nr.yz;
           snode[3 * QPL + rc] = nr.zx;
         }
       }
       if constexpr (SWEEP_B) {
-        const int oc = oly * QX + olx;
         sgm[oc] = igP; sgm[QPL + oc] = muP;
         if (tid < NRING) {
-          const int rc = rly * QX + rlx;
           sgm[rc] = rig; sgm[QPL + rc] = rmu;
         }
       }
@@ -375,20 +404,26 @@ __global__ void __launch_bounds__(NT) k_march(Geo g, KParams kp, SC<R> s, const 
       igM = igZ; muM = muZ; igZ = igP; muZ = muP; clZ = clP;
       cZ = cnew_col;
     }
-    (void)cP_;
   }
 }
 
-template <typename R>
+template <typename R, typename TL>
 size_t march_lds(bool sweep_b) {
-  return sizeof(R) * (size_t)(NSLOT * PPL + 4 * QPL + (sweep_b ? 2 * QPL : 0));
+  return sizeof(R) * (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
 }
 
-Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg) {
+using Tile64x8 = Tile<64, 8>;
+using Tile32x8 = Tile<32, 8>;
+using Tile64x4 = Tile<64, 4>;
+using Tile32x16 = Tile<32, 16>;
+constexpr int kTileX[PF_NTILES] = {64, 32, 64, 32};
+constexpr int kTileY[PF_NTILES] = {8, 8, 4, 16};
+
+Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg, int tile) {
   Geo g;
   g.nx = nx; g.ny = ny; g.nz = nz;
-  g.nbx = (nx - 2 + TX - 1) / TX;
-  g.nby = (ny - 2 + TY - 1) / TY;
+  g.nbx = (nx - 2 + kTileX[tile] - 1) / kTileX[tile];
+  g.nby = (ny - 2 + kTileY[tile] - 1) / kTileY[tile];
   g.k_lo = k_lo; g.k_hi = k_hi;
   const int ncell = k_hi - k_lo + 1;
   g.seg = seg < 1 ? 1 : (seg > ncell ? ncell : seg);
@@ -400,30 +435,50 @@ Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg) {
 
 // Segment length: enough work-groups to fill 256 CUs a few times over while
 // keeping the z warm-up (4 extra planes per segment) small.
-int pick_seg(int nx, int ny, int ncell) {
-  const int tiles = ((nx - 2 + TX - 1) / TX) * ((ny - 2 + TY - 1) / TY);
-  const int target = 2048;
+int pick_seg(int nx, int ny, int ncell, int tile, int target) {
+  const int tiles = ((nx - 2 + kTileX[tile] - 1) / kTileX[tile]) *
+                    ((ny - 2 + kTileY[tile] - 1) / kTileY[tile]);
   int nseg = (target + tiles - 1) / tiles;
   if (nseg < 1) nseg = 1;
   int seg = (ncell + nseg - 1) / nseg;
-  if (seg < 24) seg = 24;
+  if (seg < 16) seg = 16;
   return seg;
 }
 
-template <typename R>
-hipError_t launch_march(bool sweep_b, const Geo &g, const KParams &kp, const R *po, const R *co,
-                        R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                        hipStream_t st) {
-  const SC<R> s = make_sc<R>(kp);
-  const dim3 blk(TX, TY, 1), grd(8 * g.per_xcd, 1, 1);
-  const size_t lds = march_lds<R>(sweep_b);
+template <typename R, typename TL>
+hipError_t launch_march_t(bool sweep_b, const Geo &g, const KParams &kp, const SC<R> &s,
+                          const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
+                          int write_T, hipStream_t st) {
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
+  const size_t lds = march_lds<R, TL>(sweep_b);
   if (sweep_b)
-    hipLaunchKernelGGL((k_march<R, true>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
+    hipLaunchKernelGGL((k_march<R, true, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
                        Tnext, dpp, write_T);
   else
-    hipLaunchKernelGGL((k_march<R, false>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
+    hipLaunchKernelGGL((k_march<R, false, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
                        Tnext, dpp, write_T);
   return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp, const R *po,
+                        const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
+                        hipStream_t st) {
+  const SC<R> s = make_sc<R>(kp);
+  switch (tile) {
+    case 1:
+      return launch_march_t<R, Tile32x8>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
+                                         write_T, st);
+    case 2:
+      return launch_march_t<R, Tile64x4>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
+                                         write_T, st);
+    case 3:
+      return launch_march_t<R, Tile32x16>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
+                                          write_T, st);
+    default:
+      return launch_march_t<R, Tile64x8>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
+                                         write_T, st);
+  }
 }
 
 template <typename R>
@@ -443,8 +498,8 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
     tm.end(st);
     if (e != hipSuccess) return PF_E_HIP;
   }
-  const Geo ga = make_geo(nx, ny, nz, 3, nz - 2, S.seg_a);
-  const Geo gb = make_geo(nx, ny, nz, 2, nz - 1, S.seg_b);
+  const Geo ga = make_geo(nx, ny, nz, 3, nz - 2, S.seg_a, S.tile_a);
+  const Geo gb = make_geo(nx, ny, nz, 2, nz - 1, S.seg_b, S.tile_b);
   size_t pa, pb;
   probe_cells(nx, ny, nz, pa, pb);
   for (int sw = 1; sw <= fs.nv; ++sw) {
@@ -457,7 +512,7 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
     R *pn = (R *)phi[pnw];
     if (sw < fs.nv) {
       tm.begin(PF_KT_SWEEP_A, st);
-      e = launch_march<R>(false, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr, nullptr,
+      e = launch_march<R>(false, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr, nullptr,
                           nullptr, 0, st);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
@@ -471,7 +526,7 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
       const int t_cur = *T_alias ? *T_old : 1 - *T_old;
       const int t_nxt = 1 - t_cur;
       tm.begin(PF_KT_SWEEP_B, st);
-      e = launch_march<R>(true, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
+      e = launch_march<R>(true, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
                           (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1, st);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
@@ -504,14 +559,95 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
 
 int fused_init(FusedState &s, int nx, int ny, int nz, int prec) {
   s.nx = nx; s.ny = ny; s.nz = nz; s.prec = prec;
-  s.seg_a = pick_seg(nx, ny, nz - 4);
-  s.seg_b = pick_seg(nx, ny, nz - 2);
   s.enabled = nx >= 6 && ny >= 6 && nz >= 6;
-  if (s.enabled) {
-    const size_t la = prec == PF_FP64 ? march_lds<double>(true) : march_lds<float>(true);
-    if (la > 160 * 1024) s.enabled = false;
-  }
+  // defaults until pf_autotune measures: two 4-wave work-groups per CU for
+  // FP64 (its sweep B needs > 128 VGPRs), one 8-wave group for FP32
+  fused_set_tile(s, prec == PF_FP64 ? 1 : 0, prec == PF_FP64 ? 1 : 0, 2048);
   return PF_OK;
+}
+
+int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups) {
+  if (tile_a < 0 || tile_a >= PF_NTILES || tile_b < 0 || tile_b >= PF_NTILES ||
+      target_groups < 1)
+    return PF_E_ARG;
+  s.tile_a = tile_a;
+  s.tile_b = tile_b;
+  s.target = target_groups;
+  s.seg_a = pick_seg(s.nx, s.ny, s.nz - 4, tile_a, target_groups);
+  s.seg_b = pick_seg(s.nx, s.ny, s.nz - 2, tile_b, target_groups);
+  return PF_OK;
+}
+
+namespace {
+
+template <typename R>
+int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, void *T_old,
+                  int reps, hipStream_t st, AutotuneResult *res) {
+  const int nx = S.nx, ny = S.ny, nz = S.nz;
+  const size_t n = (size_t)nx * ny * nz;
+  R *out = nullptr;  // scratch outputs: phi, c, T (the context's fields are not touched)
+  if (hipMalloc((void **)&out, 3 * n * sizeof(R)) != hipSuccess) return PF_E_NOMEM;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = PF_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PF_E_HIP;
+  static const int targets[] = {1024, 2048, 4096, 8192};
+  const R *po = (const R *)phi_old, *co = (const R *)c_old, *Tc = (const R *)T_old;
+  float best[2] = {1e30f, 1e30f};
+  int cand = 0;
+  for (int sweep_b = 0; sweep_b < 2 && rc == PF_OK; ++sweep_b) {
+    for (int tile = 0; tile < PF_NTILES && rc == PF_OK; ++tile) {
+      for (int target : targets) {
+        const int seg = pick_seg(nx, ny, sweep_b ? nz - 2 : nz - 4, tile, target);
+        const Geo g = sweep_b ? make_geo(nx, ny, nz, 2, nz - 1, seg, tile)
+                              : make_geo(nx, ny, nz, 3, nz - 2, seg, tile);
+        hipError_t e = hipSuccess;
+        for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is the warm-up
+          if (r == 1) hipEventRecord(e0, st);
+          e = launch_march<R>(sweep_b != 0, tile, g, k, po, co, out, out + n, Tc, out + 2 * n,
+                              nullptr, 1, st);
+        }
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) { rc = PF_E_HIP; break; }
+        ms /= (float)reps;
+        if (res && cand < PF_AUTOTUNE_MAX) {
+          res->cand[cand].sweep = sweep_b;
+          res->cand[cand].tile = tile;
+          res->cand[cand].target = target;
+          res->cand[cand].ms = ms;
+        }
+        ++cand;
+        if (ms < best[sweep_b]) {
+          best[sweep_b] = ms;
+          if (sweep_b) { S.tile_b = tile; S.seg_b = seg; }
+          else { S.tile_a = tile; S.seg_a = seg; }
+        }
+      }
+    }
+  }
+  if (res) {
+    res->n = cand < PF_AUTOTUNE_MAX ? cand : PF_AUTOTUNE_MAX;
+    res->best_ms[0] = best[0];
+    res->best_ms[1] = best[1];
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  hipStreamSynchronize(st);
+  hipFree(out);
+  return rc;
+}
+
+}  // namespace
+
+int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, void *T_old,
+                   int reps, hipStream_t stream, AutotuneResult *res) {
+  if (!s.enabled) return PF_E_STATE;
+  if (reps < 1) return PF_E_ARG;
+  if (s.prec == PF_FP64)
+    return autotune_impl<double>(s, k, phi_old, c_old, T_old, reps, stream, res);
+  return autotune_impl<float>(s, k, phi_old, c_old, T_old, reps, stream, res);
 }
 
 void fused_free(FusedState &s) { s.enabled = false; }

@@ -198,30 +198,48 @@ __global__ void __launch_bounds__(256) k_phase_interior(KParams p, const R *__re
   dpp[c] = d;
 }
 
-// Cell t of the boundary shell of plane k (thickness w = 2 for the phase
-// decoration, 1 for the solute faces); planes within w of the z faces are
-// whole.  Returns false when t is past the plane's shell.
-__device__ __forceinline__ bool shell_cell(int t, int k, int nx, int ny, int nz, int w, int &i,
-                                           int &j) {
-  if (k <= w || k > nz - w) {
-    if (t >= nx * ny) return false;
-    i = t % nx + 1;
-    j = t / nx + 1;
+// Cell t (0-based) of the boundary shell of thickness w (2 for the phase
+// decoration, 1 for the solute faces), enumerated plane by plane: the w
+// planes at each z face are whole; every other plane contributes its w rows
+// at each y face (x-contiguous) and the w cells at each x face of its other
+// rows.  Returns false when t is past the shell.
+__host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny, int nz, int w, int &i,
+                                           int &j, int &k) {
+  const long long plane = (long long)nx * ny;
+  const long long full = (long long)w * plane;           // the w planes at one z face
+  const long long per = 2LL * w * nx + 2LL * w * (ny - 2 * w);  // one middle plane
+  const long long mid = per * (nz - 2 * w);
+  if (t < 0 || t >= 2 * full + mid) return false;
+  long long r;
+  if (t < full || t >= full + mid) {
+    const long long u = t < full ? t : t - full - mid;
+    const int kk = (int)(u / plane);
+    k = t < full ? kk + 1 : nz - w + 1 + kk;
+    r = u - (long long)kk * plane;
+    i = (int)(r % nx) + 1;
+    j = (int)(r / nx) + 1;
     return true;
   }
-  const int rows = 2 * w * nx;  // the w rows at each y face
-  if (t < rows) {
-    const int r = t / nx;
-    i = t % nx + 1;
-    j = r < w ? r + 1 : ny - (2 * w - 1 - r);
+  const long long u = t - full;
+  const int km = (int)(u / per);
+  k = w + 1 + km;
+  r = u - (long long)km * per;
+  const long long rows = 2LL * w * nx;
+  if (r < rows) {
+    const int q = (int)(r / nx);
+    i = (int)(r % nx) + 1;
+    j = q < w ? q + 1 : ny - (2 * w - 1 - q);
     return true;
   }
-  const int q = t - rows;  // the w columns at each x face of the other rows
-  if (q >= 2 * w * (ny - 2 * w)) return false;
+  const int q = (int)(r - rows);
   const int col = q % (2 * w);
   j = q / (2 * w) + w + 1;
   i = col < w ? col + 1 : nx - (2 * w - 1 - col);
   return true;
+}
+
+__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w) {
+  return 2LL * w * nx * ny + (2LL * w * nx + 2LL * w * (ny - 2 * w)) * (nz - 2 * w);
 }
 
 // The 12-plane decoration for the cells outside [3,n-2]^3.  Reads phi_new
@@ -230,10 +248,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_phase_boundary(KParams p, const R *__restrict__ po,
                                                         R *__restrict__ pn) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const int k = blockIdx.y + 1;
-  int i, j;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; shell_cell(t, k, nx, ny, nz, 2, i, j);
-       t += gridDim.x * blockDim.x) {
+  int i, j, k;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       shell_flat(t, nx, ny, nz, 2, i, j, k); t += (long long)gridDim.x * blockDim.x) {
     const int si = phase_map(i, nx), sj = phase_map(j, ny), sk = phase_map(k, nz);
     const bool inner = si >= 3 && si <= nx - 2 && sj >= 3 && sj <= ny - 2 && sk >= 3 && sk <= nz - 2;
     const size_t s = cidx(si, sj, sk, nx, ny);
@@ -306,10 +323,9 @@ template <typename R>
 __global__ void __launch_bounds__(256) k_solute_boundary(KParams p, const R *__restrict__ co,
                                                          R *__restrict__ cn) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const int k = blockIdx.y + 1;
-  int i, j;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; shell_cell(t, k, nx, ny, nz, 1, i, j);
-       t += gridDim.x * blockDim.x) {
+  int i, j, k;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       shell_flat(t, nx, ny, nz, 1, i, j, k); t += (long long)gridDim.x * blockDim.x) {
     int si = i, sj = j, sk = k;
     solute_source(si, sj, sk, nx, ny, nz);
     const bool inner = si > 1 && si < nx && sj > 1 && sj < ny && sk > 1 && sk < nz;
@@ -422,15 +438,21 @@ hipError_t launch_temperatura(const KParams &p, const R *Told, const R *phi, con
   return hipGetLastError();
 }
 
+static unsigned shell_blocks(const KParams &p, int w) {
+  const long long n = shell_count(p.nx, p.ny, p.nz, w);
+  const long long b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+
 template <typename R>
 hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s) {
-  hipLaunchKernelGGL(k_phase_boundary<R>, dim3(4, (unsigned)p.nz), dim3(256), 0, s, p, po, pn);
+  hipLaunchKernelGGL(k_phase_boundary<R>, dim3(shell_blocks(p, 2)), dim3(256), 0, s, p, po, pn);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s) {
-  hipLaunchKernelGGL(k_solute_boundary<R>, dim3(4, (unsigned)p.nz), dim3(256), 0, s, p, co, cn);
+  hipLaunchKernelGGL(k_solute_boundary<R>, dim3(shell_blocks(p, 1)), dim3(256), 0, s, p, co, cn);
   return hipGetLastError();
 }
 

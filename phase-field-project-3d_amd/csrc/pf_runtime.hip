@@ -427,6 +427,50 @@ int pf_set_fused(pf_ctx *ctx, int32_t on) {
   return PF_OK;
 }
 
+int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_groups) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (pf::fused_set_tile(ctx->fused, tile_a, tile_b, target_groups) != PF_OK)
+    return fail(PF_E_ARG, "tile shapes must be in [0,%d) and target_groups >= 1",
+                pf::PF_NTILES);
+  return PF_OK;
+}
+
+int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail(PF_E_ARG, "null output");
+  out[0] = ctx->fused.tile_a;
+  out[1] = ctx->fused.tile_b;
+  out[2] = ctx->fused.seg_a;
+  out[3] = ctx->fused.seg_b;
+  return PF_OK;
+}
+
+int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
+  if (rc == PF_OK && reps < 1) rc = fail(PF_E_ARG, "reps must be >= 1");
+  if (rc != PF_OK) return rc;
+  if (!ctx->fused.enabled) return fail(PF_E_STATE, "grid too small for the fused sweeps");
+  HIP_TRY(hipSetDevice(ctx->device));
+  pf::AutotuneResult res;
+  rc = pf::fused_autotune(ctx->fused, kparams(*p), ctx->phi.old_(), ctx->c.old_(),
+                          ctx->T.old_(), reps, ctx->stream, &res);
+  if (rc == PF_E_NOMEM) return fail(rc, "no device memory for the autotune scratch fields");
+  if (rc != PF_OK) return fail(rc, "autotune launch failed: %s",
+                               hipGetErrorString(hipGetLastError()));
+  if (ncand) *ncand = res.n;
+  if (ms)
+    for (int q = 0; q < res.n; ++q) {
+      ms[4 * q + 0] = res.cand[q].sweep;
+      ms[4 * q + 1] = res.cand[q].tile;
+      ms[4 * q + 2] = res.cand[q].target;
+      ms[4 * q + 3] = res.cand[q].ms;
+    }
+  return PF_OK;
+}
+
 int pf_timing_enable(pf_ctx *ctx, int32_t on) {
   int rc = check_ctx(ctx, nullptr);
   if (rc != PF_OK) return rc;

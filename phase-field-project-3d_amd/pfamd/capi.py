@@ -71,8 +71,10 @@ EXPORTS = (
     "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
-    "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy",
+    "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
+    "pf_get_tiling", "pf_autotune",
 )
+TILE_SHAPES = ((64, 8), (32, 8), (64, 4), (32, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin")
 
 
@@ -108,6 +110,9 @@ def load(path: str = LIB_PATH):
         "pf_diagnosticos": ([vp, P, vp], C.c_int),
         "pf_download": ([vp, vp, vp, vp], C.c_int),
         "pf_download_all": ([vp] + [vp] * 7, C.c_int),
+        "pf_set_tiling": ([vp, i32, i32, i32], C.c_int),
+        "pf_get_tiling": ([vp, vp], C.c_int),
+        "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
         "pf_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
@@ -231,6 +236,24 @@ class Context:
 
     def set_fused(self, on: bool):
         check(self.lib.pf_set_fused(self.h, 1 if on else 0), self.lib)
+
+    def set_tiling(self, tile_a: int, tile_b: int, target_groups: int = 2048):
+        check(self.lib.pf_set_tiling(self.h, tile_a, tile_b, target_groups), self.lib)
+
+    def tiling(self):
+        """(tile_a, tile_b, z-planes per segment A, B)."""
+        out = np.zeros(4, dtype=np.int32)
+        check(self.lib.pf_get_tiling(self.h, out.ctypes.data_as(C.c_void_p)), self.lib)
+        return tuple(int(v) for v in out)
+
+    def autotune(self, p, reps: int = 3):
+        """Times every tiling candidate; keeps the fastest.  Returns a list of
+        (sweep, tile, target_groups, ms)."""
+        n = C.c_int32(0)
+        ms = np.zeros(4 * 64)
+        check(self.lib.pf_autotune(self.h, C.byref(p), reps, C.byref(n), _ptr(ms)), self.lib)
+        return [("AB"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]), float(ms[4 * q + 3]))
+                for q in range(n.value)]
 
     def timing(self, on: bool):
         check(self.lib.pf_timing_enable(self.h, 1 if on else 0), self.lib)

@@ -199,6 +199,42 @@ def test_fused_deterministic_and_matches_dropin():
     assert _abs(a["T_new"], c["T_new"]) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", [capi.PF_FP64, capi.PF_FP32_MIXED])
+def test_tiling_does_not_change_results(precision):
+    """Every tile shape and z split computes each cell with the same
+    arithmetic, so results are bit-identical across tilings; the autotuner
+    leaves the state untouched and picks one of them."""
+    deck = _growth_deck(n=(70, 41, 50))
+    print(f"[tiling] precision {precision}", flush=True)
+    ref = None
+    for tile, target in ((0, 2048), (1, 2048), (2, 256), (3, 4096), (1, 1)):
+        sim = Simulation(deck, precision=precision)
+        sim.ctx.set_tiling(tile, tile, target)
+        sim.run(12)
+        out = (sim.fields(), sim.diagnostics())
+        sim.close()
+        if ref is None:
+            ref = out
+            continue
+        for nm in ("phi_new", "c_new", "T_new", "dpp"):
+            assert np.array_equal(out[0][nm], ref[0][nm]), (tile, target, nm)
+        assert np.array_equal(out[1], ref[1])
+    sim = Simulation(deck, precision=precision)
+    sim.run(5)
+    before = sim.fields()
+    cands = sim.ctx.autotune(sim.p, reps=2)
+    assert len(cands) == 2 * 4 * 4 and all(c[3] > 0 for c in cands)
+    after = sim.fields()
+    for nm in FIELDS:
+        assert np.array_equal(before[nm], after[nm]), nm
+    ta, tb, sa, sb = sim.ctx.tiling()
+    assert 0 <= ta < 4 and 0 <= tb < 4 and sa >= 1 and sb >= 1
+    sim.run(7)
+    for nm in ("phi_new", "c_new", "T_new", "dpp"):
+        assert np.array_equal(sim.fields()[nm], ref[0][nm]), nm
+    sim.close()
+
+
 def test_dropin_calls_match_loop():
     """Calling the five drop-in subroutines from the host in principal.f90's
     order gives the same state as pf_run_steps's unfused loop, bit for bit."""
