@@ -169,9 +169,11 @@ def log(msg):
 def tiling_summary(res):
     from pfamd.capi import TILE_SHAPES
 
-    ta, tb, sa, sb = res["tiling"]
-    return {"sweep_A": {"tile": "x".join(map(str, TILE_SHAPES[ta])), "z_planes_per_group": sa},
-            "sweep_B": {"tile": "x".join(map(str, TILE_SHAPES[tb])), "z_planes_per_group": sb}}
+    out = {}
+    for nm, t, seg in (("sweep_A", res["tiling"][0], res["tiling"][2]),
+                       ("sweep_B", res["tiling"][1], res["tiling"][3])):
+        out[nm] = {"tile": "x".join(map(str, TILE_SHAPES[t])), "z_planes_per_group": seg}
+    return out
 
 
 def main(argv=None):
@@ -181,6 +183,8 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tiling (for profiling runs)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (instances of this "
@@ -192,7 +196,7 @@ def main(argv=None):
         print("no GPU visible", file=sys.stderr)
         return 1
     log(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps")
-    r64 = run_form(capi.PF_FP64, a.steps, a.warmup)
+    r64 = run_form(capi.PF_FP64, a.steps, a.warmup, autotune=not a.no_autotune)
     ncell = NX * NY * NZ
     mlups = ncell * a.steps / r64["wall_s"] / 1e6
     log(f"[bench] FP64 autotune {r64['autotune']}")
@@ -224,7 +228,7 @@ def main(argv=None):
     log(f"[bench] STREAM copy {gbs_copy:.0f} GB/s")
     if not a.no_fp32:
         log("[bench] FP32-mixed")
-        r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup)
+        r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup, autotune=not a.no_autotune)
         m32 = ncell * a.steps / r32["wall_s"] / 1e6
         log(f"[bench] FP32 autotune {r32['autotune']}")
         log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['tiling']}, kernels {r32['kernels']}")

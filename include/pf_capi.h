@@ -189,8 +189,8 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps,
 int pf_set_fused(pf_ctx *ctx, int32_t on);
 /* Work-group tiling of the fused sweeps (no counterpart in the reference,
  * whose loops are collapsed over the whole grid, equacao_campo_fase.f90:27).
- * Tile shapes: 0 = 64x8, 1 = 32x8, 2 = 64x4, 3 = 32x16 cells per work-group;
- * the z-march is split into segments so that a sweep launches about
+ * Tile shapes: 0 = 64x8, 1 = 32x8, 2 = 32x16 cells per work-group; the
+ * z-march is split into segments so that a sweep launches about
  * target_groups work-groups.  The tiling never changes results: every cell
  * is computed by the same arithmetic in any tiling. */
 int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_groups);

@@ -17,15 +17,17 @@
 // sweep B reads phi, c and writes phi, c, T.
 //
 // Arithmetic follows the reference's formulas with algebraic reuse
-// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g'),
-// divisions as a refined reciprocal), so results agree with the reference
-// to rounding, not bit for bit; the drop-in kernels keep the bit-faithful
-// form.
+// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g'), the
+// free-energy ratio with CL = c/den folded in, divisions as a refined
+// reciprocal, the ~1 ulp logarithm of pf_math.h), so results agree with the
+// reference to rounding, not bit for bit; the drop-in kernels keep the
+// bit-faithful form.
 #include <hip/hip_runtime.h>
 
 #include "pf_common.h"
 #include "pf_fused.h"
 #include "pf_kernels.h"
+#include "pf_math.h"
 
 namespace pf {
 namespace {
@@ -44,26 +46,19 @@ struct Tile {
   static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
 };
 
+// log and 1/x in the arithmetic type.  FP64: the ~1 ulp forms of
+// pf_math.h.  FP32: the hardware log2 / reciprocal (about 1 ulp).
 template <typename R>
-__device__ __forceinline__ R dlog(R x) { return log(x); }
+__device__ __forceinline__ R dlog(R x);
+template <>
+__device__ __forceinline__ double dlog<double>(double x) { return log_fast(x); }
 template <>
 __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
 
-// 1/x.  FP64: the hardware estimate refined by two Newton steps (about 1 ulp,
-// half the instructions of a correctly rounded division); for x = 0, +-inf or
-// NaN the estimate itself is returned (inf, 0, NaN), as a division would give.
-// FP32: the hardware reciprocal (1 ulp).
 template <typename R>
 __device__ __forceinline__ R rcpn(R x);
 template <>
-__device__ __forceinline__ double rcpn<double>(double x) {
-  const double r0 = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r0, 1.0);
-  double r = fma(r0, e, r0);
-  e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  return r != r ? r0 : r;
-}
+__device__ __forceinline__ double rcpn<double>(double x) { return rcp_refined(x); }
 template <>
 __device__ __forceinline__ float rcpn<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -76,7 +71,7 @@ struct SC {
   R E0a, E0b, K;       // eps = E0a + E0b*S4/Q^2, K = 16 E0 delta
   R small;
   R Ke, W0, RTV;       // RTV = R*TP/Vm
-  R omCs, iomCl;       // 1-Cs_eq, 1/(1-Cl_eq)
+  R omCsCl;            // (1-Cs_eq)/(1-Cl_eq)
   R M0dt, AAA;
   R Dliq2dt, Dsol2dt;  // 2*D*dt for the solute faces
   R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
@@ -95,7 +90,7 @@ SC<R> make_sc(const KParams &p) {
   s.K = (R)(16.0 * p.E0 * p.delta_eps);
   s.small = (R)1.0e-23;
   s.Ke = (R)p.Ke; s.W0 = (R)p.W0; s.RTV = (R)(p.R * p.TP / p.Vm);
-  s.omCs = (R)(1.0 - p.Cs_eq); s.iomCl = (R)(1.0 / (1.0 - p.Cl_eq));
+  s.omCsCl = (R)((1.0 - p.Cs_eq) / (1.0 - p.Cl_eq));
   s.M0dt = (R)(p.M0 * p.dt); s.AAA = (R)p.AAA;
   s.Dliq2dt = (R)(2.0 * p.D_liq * p.dt); s.Dsol2dt = (R)(2.0 * p.D_sol * p.dt);
   s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
@@ -142,12 +137,11 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
 // Solute node quantities: 1/gamma (gamma without the centre's D_C) and
 // mu = ln(CL/(1-CL)) = ln(c/(den-c)) (conc_body.inc:61-118, 139-144).
 template <typename R>
-__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &mu,
-                                            R &CL) {
+__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &mu) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
-  CL = cc * rcpn<R>(den);
+  const R CL = cc * rcpn<R>(den);
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
   ig = rcpn<R>(g);
@@ -255,7 +249,7 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
   // own-column registers: nodes of planes k-1 (M), k (Z), k+1 (P) and the
   // solute quantities of planes k-1, k, k+1
   Node<R> nM{}, nZ{}, nP{};
-  R igM = 0, muM = 0, igZ = 0, muZ = 0, clZ = 0, igP = 0, muP = 0, clP = 0;
+  R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R cZ = 0;
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
 
@@ -278,11 +272,9 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
           const R *P0 = slot(q - 1);
           const size_t pk = (size_t)(kn - 1) * sz;
           cnew_col = co[pk + ooff];
-          solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP, clP);
-          if (tid < NRING) {
-            R dummy;
-            solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu, dummy);
-          }
+          solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
+          if (tid < NRING)
+            solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu);
         }
       }
     }
@@ -326,11 +318,9 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
         const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
         R h, hl;
         h_of(PP, quint, h, hl);
-        R CLP;
-        if constexpr (SWEEP_B) CLP = clZ;
-        else CLP = cPP * rcpn<R>(s.Ke * h + ((R)1 - h));
-        const R CSP = s.Ke * CLP;
-        const R arg = (s.omCs * ((R)1 - CLP)) * s.iomCl * rcpn<R>((R)1 - CSP);
+        // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
+        const R den = s.Ke * h + ((R)1 - h);
+        const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
         const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
         R ruidos = (R)0;
         if (noise) {
@@ -401,7 +391,7 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
     }
     nM = nZ; nZ = nP;
     if constexpr (SWEEP_B) {
-      igM = igZ; muM = muZ; igZ = igP; muZ = muP; clZ = clP;
+      igM = igZ; muM = muZ; igZ = igP; muZ = muP;
       cZ = cnew_col;
     }
   }
@@ -414,10 +404,9 @@ size_t march_lds(bool sweep_b) {
 
 using Tile64x8 = Tile<64, 8>;
 using Tile32x8 = Tile<32, 8>;
-using Tile64x4 = Tile<64, 4>;
 using Tile32x16 = Tile<32, 16>;
-constexpr int kTileX[PF_NTILES] = {64, 32, 64, 32};
-constexpr int kTileY[PF_NTILES] = {8, 8, 4, 16};
+constexpr int kTileX[PF_NVARIANTS] = {64, 32, 32};
+constexpr int kTileY[PF_NVARIANTS] = {8, 8, 16};
 
 Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg, int tile) {
   Geo g;
@@ -465,20 +454,14 @@ hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp,
                         const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
                         hipStream_t st) {
   const SC<R> s = make_sc<R>(kp);
+#define PF_MARCH(TL) \
+  return launch_march_t<R, TL>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, st)
   switch (tile) {
-    case 1:
-      return launch_march_t<R, Tile32x8>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
-                                         write_T, st);
-    case 2:
-      return launch_march_t<R, Tile64x4>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
-                                         write_T, st);
-    case 3:
-      return launch_march_t<R, Tile32x16>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
-                                          write_T, st);
-    default:
-      return launch_march_t<R, Tile64x8>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
-                                         write_T, st);
+    case 1: PF_MARCH(Tile32x8);
+    case 2: PF_MARCH(Tile32x16);
+    default: PF_MARCH(Tile64x8);
   }
+#undef PF_MARCH
 }
 
 template <typename R>
@@ -560,14 +543,15 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
 int fused_init(FusedState &s, int nx, int ny, int nz, int prec) {
   s.nx = nx; s.ny = ny; s.nz = nz; s.prec = prec;
   s.enabled = nx >= 6 && ny >= 6 && nz >= 6;
-  // defaults until pf_autotune measures: two 4-wave work-groups per CU for
-  // FP64 (its sweep B needs > 128 VGPRs), one 8-wave group for FP32
-  fused_set_tile(s, prec == PF_FP64 ? 1 : 0, prec == PF_FP64 ? 1 : 0, 2048);
+  // defaults until pf_autotune measures: the shapes that were fastest for
+  // both precisions on 256^2 x 528 (32x8 for the phase-only sweep, 32x16
+  // for the phase + solute sweep, about 1024 work-groups)
+  fused_set_tile(s, 1, 2, 1024);
   return PF_OK;
 }
 
 int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups) {
-  if (tile_a < 0 || tile_a >= PF_NTILES || tile_b < 0 || tile_b >= PF_NTILES ||
+  if (tile_a < 0 || tile_a >= PF_NVARIANTS || tile_b < 0 || tile_b >= PF_NVARIANTS ||
       target_groups < 1)
     return PF_E_ARG;
   s.tile_a = tile_a;
@@ -595,7 +579,7 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
   float best[2] = {1e30f, 1e30f};
   int cand = 0;
   for (int sweep_b = 0; sweep_b < 2 && rc == PF_OK; ++sweep_b) {
-    for (int tile = 0; tile < PF_NTILES && rc == PF_OK; ++tile) {
+    for (int tile = 0; tile < PF_NVARIANTS && rc == PF_OK; ++tile) {
       for (int target : targets) {
         const int seg = pick_seg(nx, ny, sweep_b ? nz - 2 : nz - 4, tile, target);
         const Geo g = sweep_b ? make_geo(nx, ny, nz, 2, nz - 1, seg, tile)

@@ -432,7 +432,7 @@ int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_gr
   if (rc != PF_OK) return rc;
   if (pf::fused_set_tile(ctx->fused, tile_a, tile_b, target_groups) != PF_OK)
     return fail(PF_E_ARG, "tile shapes must be in [0,%d) and target_groups >= 1",
-                pf::PF_NTILES);
+                pf::PF_NVARIANTS);
   return PF_OK;
 }
 

@@ -74,7 +74,7 @@ EXPORTS = (
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune",
 )
-TILE_SHAPES = ((64, 8), (32, 8), (64, 4), (32, 16))
+TILE_SHAPES = ((64, 8), (32, 8), (32, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin")
 
 
@@ -237,7 +237,7 @@ class Context:
     def set_fused(self, on: bool):
         check(self.lib.pf_set_fused(self.h, 1 if on else 0), self.lib)
 
-    def set_tiling(self, tile_a: int, tile_b: int, target_groups: int = 2048):
+    def set_tiling(self, tile_a: int, tile_b: int, target_groups: int = 1024):
         check(self.lib.pf_set_tiling(self.h, tile_a, tile_b, target_groups), self.lib)
 
     def tiling(self):
