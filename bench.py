@@ -113,6 +113,67 @@ def roofline(res, precision_bytes):
     return best
 
 
+def precision_decks():
+    """The precision study's configurations on the 64x64x128 grid, one seed
+    of radius 5 at the bottom centre, noise off: the cooling deck (T0 = 922 K
+    is above the liquidus, so the seed melts back) and an active-growth
+    variant held at TP = 905 K."""
+    from pfamd.deck import Deck
+
+    base = dict(Nx=64, Ny=64, Nz=128, tipo_inicial=5, R_semente=5.0, h_quintico=0, ruido_tipo=0,
+                AAA=0.0, n_varreduras=2, stoptime=0.0)
+    return {"cooling 19.84 K/s (melt-back)": Deck(taxa_resfr=19.84, **base),
+            "held at TP=905 K (growth)": Deck(taxa_resfr=0.0, TP=905.0, **base)}
+
+
+def precision_study(checkpoints=(5000, 10000, 20000)):
+    """FP32-mixed against FP64: per-field max-abs, RMS and relative L2
+    differences and the interior solute mass at each checkpoint (both runs
+    on the GPU, in lockstep)."""
+    import numpy as np
+
+    from pfamd import capi
+    from pfamd.run import Simulation
+
+    out = {}
+    for name, deck in precision_decks().items():
+        d = deck.resolved()
+        shape = (d.Nz, d.Ny, d.Nx)
+        sims = {"fp64": Simulation(deck), "fp32": Simulation(deck, precision=capi.PF_FP32_MIXED)}
+
+        def interior_mass(c):
+            return float(np.sum(c.reshape(shape)[1:-1, 1:-1, 1:-1], dtype=np.float64))
+
+        m0 = interior_mass(sims["fp64"].fields()["c_new"])
+        rows, done = [], 0
+        for cp in checkpoints:
+            for sim in sims.values():
+                sim.run(cp - done)
+            done = cp
+            f64, f32 = sims["fp64"].fields(), sims["fp32"].fields()
+            row = {"step": cp}
+            for nm, key in (("phi", "phi_new"), ("c", "c_new"), ("T", "T_new")):
+                a, b = f32[key], f64[key]
+                nb = float(np.linalg.norm(b))
+                row[f"{nm}_maxabs"] = float(np.max(np.abs(a - b)))
+                row[f"{nm}_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
+                row[f"{nm}_rel_l2"] = float(np.linalg.norm(a - b) / nb) if nb > 1e-6 else None
+            M64, M32 = interior_mass(f64["c_new"]), interior_mass(f32["c_new"])
+            led = sims["fp32"].ctx.mass_ledger()
+            row["phi_max_fp64"] = float(np.max(f64["phi_new"]))
+            row["mass_fp64_change"] = (M64 - m0) / m0
+            row["mass_fp32_vs_fp64"] = (M32 - M64) / M64
+            row["mass_fp32_vs_ledger"] = (M32 - led[0]) / led[0]
+            rows.append(row)
+            log(f"[bench] precision study {name} step {cp}: {row}")
+        for sim in sims.values():
+            sim.close()
+        out[name] = rows
+    return {"config": "64x64x128, seed r=5 at the bottom centre, noise off, 2 sweeps/step; "
+                      "FP32-mixed (mass correction every 100 steps) vs FP64 on the GPU",
+            "checkpoints": out}
+
+
 def cpu_baseline():
     """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
     kernels compiled from source with flang -O2 -fopenmp), timed on this
@@ -183,6 +244,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling (for profiling runs)")
     a = ap.parse_args(argv)
@@ -240,6 +302,9 @@ def main(argv=None):
                              "roofline_achieved_gbs": round(rl32["achieved_gbs"], 1),
                              "tiling": tiling_summary(r32),
                              "diag_final": r32["diag"]}
+    if not a.no_fp32 and not a.no_precision_study:
+        log("[bench] precision study (FP32-mixed vs FP64, 64x64x128)")
+        out["fp32_mixed"]["precision_study"] = precision_study()
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (reference OpenMP path)")
         cb = cpu_baseline()

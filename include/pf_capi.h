@@ -203,6 +203,22 @@ int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
  * {sweep (0 = A, 1 = B), tile, target_groups, average ms}, at most 64
  * candidates, their number in *ncand.  Synchronous. */
 int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms);
+/* Solute-mass correction of the fused path (no counterpart in the
+ * reference, which is FP64 throughout).  The context keeps an FP64 ledger of
+ * the interior ([2,n-1]^3) solute mass: the FP64 sum of the field when a run
+ * starts plus the FP64 sum of every increment the solute update computes.
+ * Every `every` steps (p->it a multiple of it) the FP64 interior sum of the
+ * stored field is compared with the ledger and the interior is rescaled by
+ * ledger/sum (the FP64 product rounded stochastically to FP32, unbiased,
+ * drawn from a counter hash of cell and step), removing the drift that FP32
+ * storage rounding accumulates.  The ledger follows the scheme's own
+ * increments: the reference's solute update is not conservative by itself
+ * (the face mobility takes the centre cell's D), and that change is kept.  Default: 100 for PF_FP32_MIXED, 0 (off) for PF_FP64.
+ * Setting it restarts the ledger from the field at the next pf_run_steps. */
+int pf_set_mass_correction(pf_ctx *ctx, int32_t every);
+/* Synchronous: out = {ledger, interior sum at the last correction, last
+ * factor ledger/sum}; zeros before the first run with the correction on. */
+int pf_mass_ledger(pf_ctx *ctx, double out[3]);
 /* Wait for all queued work of the context. */
 int pf_synchronize(pf_ctx *ctx);
 
@@ -215,7 +231,8 @@ int pf_synchronize(pf_ctx *ctx);
 #define PF_KT_BOUNDARY 2    /* boundary decorations                         */
 #define PF_KT_TEMPERATURA 3 /* standalone temperatura                       */
 #define PF_KT_DROPIN 4      /* drop-in phase / solute kernels               */
-#define PF_KT_COUNT 5
+#define PF_KT_MASS 5        /* solute-mass correction (FP32-mixed form)     */
+#define PF_KT_COUNT 6
 /* STREAM-style copy of `bytes` bytes (FP64 elements, coalesced, one read
  * and one write per element) repeated `iters` times on the current device;
  * reports the copy bandwidth (read + write bytes / average kernel time).

@@ -28,6 +28,13 @@ struct FusedState {
   int target = 2048;           // work-groups per sweep the z split aims for
   int seg_a = 0, seg_b = 0;    // z-planes per work-group segment
   bool enabled = false;
+  // solute-mass ledger (pf_fused.hip): period of the correction in steps
+  // (0 = off), device scalars {ledger, interior sum, factor}, one FP64 slot
+  // per sweep-B work-group, partial interior sums
+  int mass_every = 0;
+  bool ledger_valid = false;
+  double *mass = nullptr, *mslot = nullptr, *mpart = nullptr;
+  int nslot = 0;
 };
 
 // HIP-event pairs recorded around launches on one stream, by kernel kind.
@@ -76,6 +83,14 @@ struct AutotuneResult {
 };
 int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, void *T_old,
                    int reps, hipStream_t stream, AutotuneResult *res);
+// Solute-mass ledger: prepare before a run (sizes slots, starts the ledger
+// from the current field when it is not valid), fold the slots into the
+// ledger after a run, correct the field (every mass_every steps), read
+// {ledger, interior sum at the last correction, last factor}.
+int fused_mass_prepare(FusedState &s, const void *c_old, hipStream_t stream);
+int fused_mass_fold(FusedState &s, hipStream_t stream);
+int fused_mass_correct(FusedState &s, const KParams &k, void *c, hipStream_t stream);
+int fused_mass_read(FusedState &s, double out[3], hipStream_t stream);
 void fused_free(FusedState &s);
 size_t fused_bytes(const FusedState &s);
 bool fused_supported(const FusedState &s, const pf_params &p);

@@ -24,6 +24,8 @@
 // bit-faithful form.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pf_common.h"
 #include "pf_fused.h"
 #include "pf_kernels.h"
@@ -134,10 +136,18 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
   }
 }
 
-// Solute node quantities: 1/gamma (gamma without the centre's D_C) and
-// mu = ln(CL/(1-CL)) = ln(c/(den-c)) (conc_body.inc:61-118, 139-144).
+// Solute node quantities (conc_body.inc:61-118, 139-144): 1/gamma (gamma
+// without the centre's D_C) and q, the node's share of the chemical
+// potential difference taken across each face by dmu():
+//   FP64: q = mu = ln(CL/(1-CL)) = ln(c/(den-c)), dmu = mu_X - mu_P;
+//   FP32: q = CL, dmu = ln((CL_X (1-CL_P)) / (CL_P (1-CL_X)))
+//         = log1p((CL_X - CL_P) / (CL_P (1-CL_X)))   (same quantity).
+// In FP32 the difference of two ~-4 logarithms is noise of about one FP32
+// ulp of mu, which in near-uniform liquid is as large as the true
+// difference and would random-walk c by ~1 ulp per step; the log1p form
+// takes the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
 template <typename R>
-__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &mu) {
+__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &q) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
@@ -145,7 +155,22 @@ __device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
   ig = rcpn<R>(g);
-  mu = dlog<R>(cc * rcpn<R>(den - cc));
+  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc));
+  else q = CL;
+}
+
+template <typename R>
+__device__ __forceinline__ R dmu(R qX, R qP);
+template <>
+__device__ __forceinline__ double dmu<double>(double qX, double qP) { return qX - qP; }
+template <>
+__device__ __forceinline__ float dmu<float>(float qX, float qP) {
+  const float d = (qX - qP) * rcpn<float>(qP * (1.0f - qX));
+  // log1p(d): Taylor series to d^6 for |d| < 1/32 (truncation < 2e-10
+  // relative), else log(1 + d)
+  const float ser =
+      d * (1.0f + d * (-0.5f + d * (1.0f / 3.0f + d * (-0.25f + d * (0.2f + d * (-1.0f / 6.0f))))));
+  return fabsf(d) < 0.03125f ? ser : dlog<float>(1.0f + d);
 }
 
 struct Geo {
@@ -170,13 +195,19 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // SWEEP_B == false: sweep A (phase only).  SWEEP_B == true: phase + solute +
-// (write_T ? next-step T : DPP).
+// (write_T ? next-step T : DPP); if mslot is non-null, the work-group adds
+// the FP64 sum of its cells' solute increments to mslot[blockIdx.x] (the
+// solute-mass ledger; one slot per work-group, so no atomics and a fixed
+// summation order).
+// FP32 sweeps fit four waves per SIMD (<= 128 VGPRs); the hint keeps the
+// compiler there.  FP64 sweeps need more registers than three waves allow.
 template <typename R, bool SWEEP_B, typename TL>
-__global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po,
+__global__ void __launch_bounds__(TL::NT)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1))) k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po,
                                               const R *__restrict__ co, R *__restrict__ pn,
                                               R *__restrict__ cn, const R *__restrict__ Tcur,
                                               R *__restrict__ Tnext, R *__restrict__ dpp,
-                                              int write_T) {
+                                              int write_T, double *__restrict__ mslot) {
   constexpr int TX = TL::TX, NT = TL::NT, PX = TL::PX, PPL = TL::PPL;
   constexpr int QX = TL::QX, QY = TL::QY, QPL = TL::QPL, NRING = TL::NRING, NLD = TL::NLD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -252,6 +283,10 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R cZ = 0;
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
+  // solute increments of this column for the ledger, summed in the
+  // arithmetic type over the column's z-segment (a few dozen cells) and in
+  // FP64 from there on (FP64 here would cost the FP32 form a wave per SIMD)
+  R macc = (R)0;
 
   const int q_first = ks - 2, q_last = ke + 2;
   gload(q_first);
@@ -344,12 +379,14 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
         const R *ig = sgm, *mu = sgm + QPL;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R igc = igZ, muc = muZ;
-        const R fx = (mu[nc + 1] - muc) * rcpn<R>(ig[nc + 1] + igc) +
-                     (mu[nc - 1] - muc) * rcpn<R>(ig[nc - 1] + igc);
-        const R fy = (mu[nc + QX] - muc) * rcpn<R>(ig[nc + QX] + igc) +
-                     (mu[nc - QX] - muc) * rcpn<R>(ig[nc - QX] + igc);
-        const R fz = (muP - muc) * rcpn<R>(igP + igc) + (muM - muc) * rcpn<R>(igM + igc);
-        cn[gidx] = cPP + D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
+                     dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
+        const R fy = dmu<R>(mu[nc + QX], muc) * rcpn<R>(ig[nc + QX] + igc) +
+                     dmu<R>(mu[nc - QX], muc) * rcpn<R>(ig[nc - QX] + igc);
+        const R fz = dmu<R>(muP, muc) * rcpn<R>(igP + igc) + dmu<R>(muM, muc) * rcpn<R>(igM + igc);
+        const R inc = D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        cn[gidx] = cPP + inc;
+        macc += inc;
         if (write_T) {
           // temp_body.inc:31-41 of the next step: TP + D dt lap(T) + fonte
           R hh, hl2;
@@ -395,6 +432,22 @@ __global__ void __launch_bounds__(TL::NT) k_march(Geo g, KParams kp, SC<R> s, co
       cZ = cnew_col;
     }
   }
+  if constexpr (SWEEP_B) {
+    if (mslot) {  // uniform over the work-group
+      __shared__ double wsum[NT / 64];
+      double v = (double)macc;
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if ((tid & 63) == 0) wsum[tid >> 6] = v;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NT / 64; ++w) t += wsum[w];
+        mslot[blockIdx.x] += t;
+      }
+    }
+  } else {
+    (void)mslot; (void)macc;
+  }
 }
 
 template <typename R, typename TL>
@@ -437,31 +490,38 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
 template <typename R, typename TL>
 hipError_t launch_march_t(bool sweep_b, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
-                          int write_T, hipStream_t st) {
+                          int write_T, double *mslot, hipStream_t st) {
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
   const size_t lds = march_lds<R, TL>(sweep_b);
   if (sweep_b)
     hipLaunchKernelGGL((k_march<R, true, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
-                       Tnext, dpp, write_T);
+                       Tnext, dpp, write_T, mslot);
   else
     hipLaunchKernelGGL((k_march<R, false, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
-                       Tnext, dpp, write_T);
+                       Tnext, dpp, write_T, nullptr);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp, const R *po,
                         const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                        hipStream_t st) {
+                        double *mslot, hipStream_t st) {
   const SC<R> s = make_sc<R>(kp);
 #define PF_MARCH(TL) \
-  return launch_march_t<R, TL>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, st)
+  return launch_march_t<R, TL>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, \
+                               mslot, st)
   switch (tile) {
     case 1: PF_MARCH(Tile32x8);
     case 2: PF_MARCH(Tile32x16);
     default: PF_MARCH(Tile64x8);
   }
 #undef PF_MARCH
+}
+
+// The ledger is kept while it is valid and has a slot for every sweep-B
+// work-group (fused_mass_prepare sizes the slots before a run).
+bool mass_on(const FusedState &S, const Geo &gb) {
+  return S.mass_every > 0 && S.ledger_valid && S.mslot && 8 * gb.per_xcd <= S.nslot;
 }
 
 template <typename R>
@@ -495,8 +555,8 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
     R *pn = (R *)phi[pnw];
     if (sw < fs.nv) {
       tm.begin(PF_KT_SWEEP_A, st);
-      e = launch_march<R>(false, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr, nullptr,
-                          nullptr, 0, st);
+      e = launch_march<R>(false, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr,
+                          nullptr, nullptr, 0, nullptr, st);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
       tm.begin(PF_KT_BOUNDARY, st);
@@ -510,7 +570,8 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
       const int t_nxt = 1 - t_cur;
       tm.begin(PF_KT_SWEEP_B, st);
       e = launch_march<R>(true, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
-                          (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1, st);
+                          (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1,
+                          mass_on(S, gb) ? S.mslot : nullptr, st);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
       tm.begin(PF_KT_BOUNDARY, st);
@@ -547,6 +608,7 @@ int fused_init(FusedState &s, int nx, int ny, int nz, int prec) {
   // both precisions on 256^2 x 528 (32x8 for the phase-only sweep, 32x16
   // for the phase + solute sweep, about 1024 work-groups)
   fused_set_tile(s, 1, 2, 1024);
+  s.mass_every = prec == PF_FP32_MIXED ? 100 : 0;
   return PF_OK;
 }
 
@@ -588,7 +650,7 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
         for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is the warm-up
           if (r == 1) hipEventRecord(e0, st);
           e = launch_march<R>(sweep_b != 0, tile, g, k, po, co, out, out + n, Tc, out + 2 * n,
-                              nullptr, 1, st);
+                              nullptr, 1, nullptr, st);
         }
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventRecord(e1, st);
@@ -634,8 +696,212 @@ int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, 
   return autotune_impl<float>(s, k, phi_old, c_old, T_old, reps, stream, res);
 }
 
-void fused_free(FusedState &s) { s.enabled = false; }
-size_t fused_bytes(const FusedState &) { return 0; }
+// ---- solute-mass ledger (FP32-mixed form) -----------------------------------
+// The ledger is the FP64 interior solute mass the scheme's own increments
+// imply: its initial FP64 sum plus the FP64 sum of every FP32 increment
+// computed by sweep B (before the increment is rounded into the FP32 field).
+// Every mass_every steps the FP64 sum of the stored field over the same
+// interior [2,n-1]^3 is compared with it and the interior is rescaled by
+// ledger/sum (FP64 product, unbiased stochastic rounding to FP32), which
+// removes the drift that FP32 storage rounding accumulates.  The face planes
+// are then refreshed from their interior sources: left ~1 ulp above the
+// rescaled interior (the factor is mostly < 1), they would feed solute back
+// through the domain faces at every correction.
+
+namespace {
+
+constexpr int kMassBlocks = 1024;
+
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) sh[t] += sh[t + w];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_mass_interior(const R *__restrict__ c, int nx, int ny,
+                                                       int nz, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int nrow = (ny - 2) * (nz - 2);
+  const size_t sz = (size_t)nx * ny;
+  double acc = 0.0;
+  for (int r = blockIdx.x; r < nrow; r += gridDim.x) {
+    const int j = 2 + r % (ny - 2), k = 2 + r / (ny - 2);
+    const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
+    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) acc += (double)c[row + i - 1];
+  }
+  const double t = block_sum256(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// mode 0: ledger := interior sum (slots discarded); 1: ledger += slots;
+// 2: ledger += slots, mass[1] := interior sum, mass[2] := ledger / sum.
+// Slots are zeroed in every mode.
+__global__ void __launch_bounds__(256) k_mass_update(const double *__restrict__ part, int npart,
+                                                     double *__restrict__ mslot, int nslot,
+                                                     double *__restrict__ mass, int mode) {
+  __shared__ double sh[256];
+  double a = 0.0, b = 0.0;
+  for (int q = threadIdx.x; q < npart; q += 256) a += part[q];
+  for (int q = threadIdx.x; q < nslot; q += 256) {
+    b += mslot[q];
+    mslot[q] = 0.0;
+  }
+  const double S = block_sum256(a, sh);
+  __syncthreads();
+  const double L = block_sum256(b, sh);
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      mass[0] = S;
+    } else {
+      mass[0] += L;
+      if (mode == 2) {
+        mass[1] = S;
+        mass[2] = mass[0] / S;
+      }
+    }
+  }
+}
+
+// Rescales the interior cells by mass[2].  FP32: the FP64 product is
+// rounded stochastically to one of its two neighbouring floats with the
+// probability that makes the expectation exact (a counter hash of cell and
+// step, salt 4, draws the choice), because the factor is typically 1 -+ 1e-9,
+// far below one FP32 ulp, and round-to-nearest would leave every cell as it
+// was.  The summed rounding error is then a zero-mean random walk of about
+// sqrt(n) * ulp / 2.
+template <typename R>
+__global__ void __launch_bounds__(256) k_mass_scale(R *__restrict__ c, int nx, int ny, int nz,
+                                                    int it, uint64_t seed,
+                                                    const double *__restrict__ mass) {
+  const double f = mass[2];
+  const int nrow = (ny - 2) * (nz - 2);
+  const size_t sz = (size_t)nx * ny;
+  for (int r = blockIdx.x; r < nrow; r += gridDim.x) {
+    const int j = 2 + r % (ny - 2), k = 2 + r / (ny - 2);
+    const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
+    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) {
+      const size_t t = row + i - 1;
+      const double x = (double)c[t] * f;
+      if constexpr (sizeof(R) == 8) {
+        c[t] = (R)x;
+      } else {
+        const float nearest = (float)x;
+        float lo = nearest, hi = nearest;
+        if ((double)nearest > x) lo = nextafterf(nearest, -__builtin_huge_valf());
+        else if ((double)nearest < x) hi = nextafterf(nearest, __builtin_huge_valf());
+        float out = lo;
+        if (hi != lo) {
+          const double p = (x - (double)lo) / ((double)hi - (double)lo);
+          out = hash_unit(hash_base(i, j, k, it, 4, seed)) < p ? hi : lo;
+        }
+        c[t] = out;
+      }
+    }
+  }
+}
+
+template <typename R>
+hipError_t mass_interior(const FusedState &S, const void *c, hipStream_t st) {
+  hipLaunchKernelGGL(k_mass_interior<R>, dim3(kMassBlocks), dim3(256), 0, st, (const R *)c, S.nx,
+                     S.ny, S.nz, S.mpart);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
+  if (S.mass_every <= 0 || !S.enabled) return PF_OK;
+  const Geo gb = make_geo(S.nx, S.ny, S.nz, 2, S.nz - 1, S.seg_b, S.tile_b);
+  const int need = 8 * gb.per_xcd;
+  if (!S.mass) {
+    if (hipMalloc((void **)&S.mass, 4 * sizeof(double)) != hipSuccess) return PF_E_NOMEM;
+    if (hipMalloc((void **)&S.mpart, kMassBlocks * sizeof(double)) != hipSuccess)
+      return PF_E_NOMEM;
+    if (hipMemsetAsync(S.mass, 0, 4 * sizeof(double), st) != hipSuccess) return PF_E_HIP;
+  }
+  if (need > S.nslot) {  // slots are folded (zero) between runs: safe to replace
+    if (S.mslot) {
+      if (hipStreamSynchronize(st) != hipSuccess) return PF_E_HIP;
+      hipFree(S.mslot);
+      S.mslot = nullptr;
+      S.nslot = 0;
+    }
+    if (hipMalloc((void **)&S.mslot, (size_t)need * sizeof(double)) != hipSuccess)
+      return PF_E_NOMEM;
+    if (hipMemsetAsync(S.mslot, 0, (size_t)need * sizeof(double), st) != hipSuccess)
+      return PF_E_HIP;
+    S.nslot = need;
+  }
+  if (!S.ledger_valid) {
+    hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c_old, st)
+                                     : mass_interior<float>(S, c_old, st);
+    if (e != hipSuccess) return PF_E_HIP;
+    hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, kMassBlocks, S.mslot,
+                       S.nslot, S.mass, 0);
+    if (hipGetLastError() != hipSuccess) return PF_E_HIP;
+    S.ledger_valid = true;
+  }
+  return PF_OK;
+}
+
+int fused_mass_fold(FusedState &S, hipStream_t st) {
+  if (!S.ledger_valid || !S.mslot) return PF_OK;
+  hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, 0, S.mslot, S.nslot,
+                     S.mass, 1);
+  return hipGetLastError() == hipSuccess ? PF_OK : PF_E_HIP;
+}
+
+int fused_mass_correct(FusedState &S, const KParams &k, void *c, hipStream_t st) {
+  if (!S.ledger_valid || !S.mslot) return PF_OK;
+  const int it = k.it;
+  const uint64_t seed = k.seed;
+  hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c, st)
+                                   : mass_interior<float>(S, c, st);
+  if (e != hipSuccess) return PF_E_HIP;
+  hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, kMassBlocks, S.mslot,
+                     S.nslot, S.mass, 2);
+  if (hipGetLastError() != hipSuccess) return PF_E_HIP;
+  if (S.prec == PF_FP64) {
+    hipLaunchKernelGGL(k_mass_scale<double>, dim3(kMassBlocks), dim3(256), 0, st, (double *)c, S.nx,
+                       S.ny, S.nz, it, seed, S.mass);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_solute_refresh<double>(k, (double *)c, st);
+  } else {
+    hipLaunchKernelGGL(k_mass_scale<float>, dim3(kMassBlocks), dim3(256), 0, st, (float *)c, S.nx,
+                       S.ny, S.nz, it, seed, S.mass);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = launch_solute_refresh<float>(k, (float *)c, st);
+  }
+  return e == hipSuccess ? PF_OK : PF_E_HIP;
+}
+
+int fused_mass_read(FusedState &S, double out[3], hipStream_t st) {
+  out[0] = out[1] = out[2] = 0.0;
+  if (!S.mass) return PF_OK;
+  if (hipMemcpyAsync(out, S.mass, 3 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return PF_E_HIP;
+  return PF_OK;
+}
+
+void fused_free(FusedState &s) {
+  s.enabled = false;
+  if (s.mass) hipFree(s.mass);
+  if (s.mpart) hipFree(s.mpart);
+  if (s.mslot) hipFree(s.mslot);
+  s.mass = s.mpart = s.mslot = nullptr;
+  s.nslot = 0;
+  s.ledger_valid = false;
+}
+size_t fused_bytes(const FusedState &s) {
+  return s.mass ? (4 + kMassBlocks + (size_t)s.nslot) * sizeof(double) : 0;
+}
 
 bool fused_supported(const FusedState &s, const pf_params &p) {
   (void)p;

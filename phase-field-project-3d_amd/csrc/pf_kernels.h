@@ -24,6 +24,9 @@ template <typename R>
 hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s);
 template <typename R>
 hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s);
+// Face cells take their interior source cell's value again (in place).
+template <typename R>
+hipError_t launch_solute_refresh(const KParams &p, R *c, hipStream_t s);
 
 // The two cells equacao_campo_fase.f90:174 prints (0-based offsets):
 // (nx/2, 10, max(nz/2,1)) and (nx/2, min(100,ny-1), max(nz/2,1)).

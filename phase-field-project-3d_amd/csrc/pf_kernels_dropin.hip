@@ -334,6 +334,24 @@ __global__ void __launch_bounds__(256) k_solute_boundary(KParams p, const R *__r
   }
 }
 
+// Boundary cells whose solute source is an interior cell take that cell's
+// value again (in place: interior cells are only read).  Used after the
+// interior of c has been rescaled, so that the face planes stay copies of
+// their interior neighbours; edge and corner cells, whose sources are other
+// boundary cells, border no interior cell and are left as they are.
+template <typename R>
+__global__ void __launch_bounds__(256) k_solute_refresh(KParams p, R *__restrict__ c) {
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  int i, j, k;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       shell_flat(t, nx, ny, nz, 1, i, j, k); t += (long long)gridDim.x * blockDim.x) {
+    int si = i, sj = j, sk = k;
+    solute_source(si, sj, sk, nx, ny, nz);
+    if (si > 1 && si < nx && sj > 1 && sj < ny && sk > 1 && sk < nz)
+      c[cidx(i, j, k, nx, ny)] = c[cidx(si, sj, sk, nx, ny)];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // diag_body.inc:8-22: block b folds the rows (j,k) = b, b+G, ... (threads
 // striding over i) into a partial {sum phi^2, max, min, sum c, F}; the second
@@ -457,6 +475,12 @@ hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStrea
 }
 
 template <typename R>
+hipError_t launch_solute_refresh(const KParams &p, R *c, hipStream_t s) {
+  hipLaunchKernelGGL(k_solute_refresh<R>, dim3(shell_blocks(p, 1)), dim3(256), 0, s, p, c);
+  return hipGetLastError();
+}
+
+template <typename R>
 __global__ void k_probe(const R *__restrict__ f, size_t a, size_t b, double *__restrict__ out) {
   if (threadIdx.x == 0) {
     out[0] = (double)f[a];
@@ -510,6 +534,7 @@ hipError_t launch_diag(const KParams &p, const R *ph, const R *cc, double *part,
                                      double *, hipStream_t);                                    \
   template hipError_t launch_phase_boundary<R>(const KParams &, const R *, R *, hipStream_t);    \
   template hipError_t launch_solute_boundary<R>(const KParams &, const R *, R *, hipStream_t);   \
+  template hipError_t launch_solute_refresh<R>(const KParams &, R *, hipStream_t);               \
   template hipError_t launch_probe<R>(const R *, size_t, size_t, double *, hipStream_t);
 PF_INST(double)
 PF_INST(float)

@@ -181,6 +181,7 @@ int do_phase(pf_ctx *ctx, const pf::KParams &k) {
 }
 
 int do_solute(pf_ctx *ctx, const pf::KParams &k) {
+  ctx->fused.ledger_valid = false;  // c changes outside the ledger
   hipError_t e;
   ctx->timer.begin(PF_KT_DROPIN, ctx->stream);
   if (ctx->prec == PF_FP64)
@@ -471,6 +472,24 @@ int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, d
   return PF_OK;
 }
 
+int pf_set_mass_correction(pf_ctx *ctx, int32_t every) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (every < 0) return fail(PF_E_ARG, "negative correction period");
+  ctx->fused.mass_every = every;
+  ctx->fused.ledger_valid = false;  // restarts from the field at the next run
+  return PF_OK;
+}
+
+int pf_mass_ledger(pf_ctx *ctx, double out[3]) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail(PF_E_ARG, "null output");
+  if (pf::fused_mass_read(ctx->fused, out, ctx->stream) != PF_OK)
+    return fail(PF_E_HIP, "reading the ledger failed");
+  return PF_OK;
+}
+
 int pf_timing_enable(pf_ctx *ctx, int32_t on) {
   int rc = check_ctx(ctx, nullptr);
   if (rc != PF_OK) return rc;
@@ -514,6 +533,13 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
   std::vector<double> tp_seen;
   if (probes) tp_seen.reserve((size_t)nsteps);
   const bool fused = ctx->use_fused && pf::fused_supported(ctx->fused, *p);
+  pf::FusedState &fst = ctx->fused;
+  if (fused) {
+    if ((rc = pf::fused_mass_prepare(fst, ctx->c.old_(), ctx->stream)) != PF_OK)
+      return fail(rc, "solute-mass ledger set-up failed");
+  } else {
+    fst.ledger_valid = false;  // the unfused kernels do not keep the ledger
+  }
   for (int s = 0; s < nsteps; ++s) {
     const bool first = (s == 0), last = (s == nsteps - 1);
     p->it += 1;
@@ -535,6 +561,12 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
                           &ctx->T.old_idx, &ctx->T.alias, ctx->dpp, pr, ctx->stream,
                           ctx->timer);
       if (rc != PF_OK) return rc;
+      if (fst.mass_every > 0 && it % fst.mass_every == 0) {  // c_old holds this step's c
+        ctx->timer.begin(PF_KT_MASS, ctx->stream);
+        rc = pf::fused_mass_correct(fst, k, ctx->c.buf[ctx->c.old_idx], ctx->stream);
+        ctx->timer.end(ctx->stream);
+        if (rc != PF_OK) return fail(rc, "solute-mass correction failed");
+      }
     } else {
       if ((rc = do_temperatura(ctx, k_temp)) != PF_OK) return rc;
       for (int sw = 1; sw <= nv; ++sw) {
@@ -551,6 +583,8 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
     p->M0 = pf_auxiliar_m0(p->TP, p->Cl_eq, p->Cs_eq, p->R, p->Vm, p->E0, p->W0, L->Sigma, L->Beta,
                            p->Ke, p->Me, p->D_liq, L->ajuste);
   }
+  if (fused && (rc = pf::fused_mass_fold(fst, ctx->stream)) != PF_OK)
+    return fail(rc, "solute-mass ledger fold failed");
   if (probes) {
     std::vector<double> tmp((size_t)2 * nv * nsteps);
     HIP_TRY(hipMemcpyAsync(tmp.data(), ctx->probe_dev, tmp.size() * sizeof(double),

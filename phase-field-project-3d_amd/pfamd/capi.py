@@ -72,10 +72,10 @@ EXPORTS = (
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
-    "pf_get_tiling", "pf_autotune",
+    "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16))
-PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin")
+PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass")
 
 
 def load(path: str = LIB_PATH):
@@ -113,6 +113,8 @@ def load(path: str = LIB_PATH):
         "pf_set_tiling": ([vp, i32, i32, i32], C.c_int),
         "pf_get_tiling": ([vp, vp], C.c_int),
         "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
+        "pf_set_mass_correction": ([vp, i32], C.c_int),
+        "pf_mass_ledger": ([vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
         "pf_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
@@ -254,6 +256,15 @@ class Context:
         check(self.lib.pf_autotune(self.h, C.byref(p), reps, C.byref(n), _ptr(ms)), self.lib)
         return [("AB"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]), float(ms[4 * q + 3]))
                 for q in range(n.value)]
+
+    def set_mass_correction(self, every: int):
+        check(self.lib.pf_set_mass_correction(self.h, int(every)), self.lib)
+
+    def mass_ledger(self):
+        """(ledger, interior sum at the last correction, last factor)."""
+        out = np.zeros(3)
+        check(self.lib.pf_mass_ledger(self.h, _ptr(out)), self.lib)
+        return tuple(float(v) for v in out)
 
     def timing(self, on: bool):
         check(self.lib.pf_timing_enable(self.h, 1 if on else 0), self.lib)
