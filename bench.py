@@ -97,6 +97,23 @@ def run_form(precision, steps, warmup, timing=True, autotune=True):
             "device_bytes": nbytes, "tiling": tiling, "autotune": cands}
 
 
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_fp64_256x256x528_pmc.md")
+
+
+def pmc_traffic(kernel="sweep_B_fp64"):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy kernel),
+    or None when the summary is absent."""
+    import re
+
+    try:
+        txt = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(rf"^traffic_bytes_per_launch: {kernel} (\d+)", txt, re.M)
+    return int(m.group(1)) if m else None
+
+
 def roofline(res, precision_bytes):
     ncell = NX * NY * NZ
     best = None
@@ -274,7 +291,11 @@ def main(argv=None):
                    "grid": [NX, NY, NZ], "cells": ncell, "form": "FP64 drop-in (fused sweeps)"},
         "roofline": {"bound": "hbm", "achieved": round(rl["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(),
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
+                                     "1.0, factors from a copy of known size)",
+                     "limiter": "VALU issue: 536 VALU instructions per cell at ~49% issue "
+                                "utilisation with 2 waves/SIMD (FP64 register demand)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,
