@@ -222,6 +222,59 @@ int pf_mass_ledger(pf_ctx *ctx, double out[3]);
 /* Wait for all queued work of the context. */
 int pf_synchronize(pf_ctx *ctx);
 
+/* ---- batched form ----------------------------------------------------------
+ * Many independent instances (each a pf_init-style problem with its own grid,
+ * precision, deck scalars and step count) advanced together: one launch per
+ * sweep covers every instance, the loop scalars of principal.f90:96-157 (it,
+ * REAL(4) time, TP, Cs/Cl, auxiliar M0) advance on the device per instance,
+ * and K steps are captured once in a hipGraph and replayed.  No counterpart
+ * in the reference, which runs one problem per process.  An instance's
+ * result does not depend on the other instances or on the tiling: it equals
+ * pf_run_steps on the same instance (tested), except that DPP is not kept.
+ * Instances of one batch share n_varreduras; every extent must be >= 6. */
+typedef struct pf_batch pf_batch;
+/* hbm_budget: bytes of instance fields the batch may hold (0 = 90% of the
+ * device's free memory now).  Instances use 6 fields (FP64: 48 B/cell,
+ * FP32-mixed: 24 B/cell). */
+int pf_batch_create(size_t hbm_budget, pf_batch **out);
+int pf_batch_destroy(pf_batch *b);
+/* Admits an instance that will run nsteps steps from the given state
+ * (phi_old, c_old, T_old as for pf_init; DPP = 0); runs its first
+ * temperatura.  PF_E_NOMEM when it does not fit the budget.  *slot is the
+ * instance's handle until pf_batch_remove. */
+int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nsteps,
+                 const double *phi_old, const double *c_old, const double *T_old,
+                 int32_t *slot);
+/* Frees an instance (synchronous). */
+int pf_batch_remove(pf_batch *b, int32_t slot);
+/* Enqueues nsteps steps of every instance that has steps left (each stops at
+ * its own count) and returns without waiting. */
+int pf_batch_run(pf_batch *b, int32_t nsteps);
+/* Steps per captured graph (default 16; 0 = plain launches). */
+int pf_batch_set_graph_steps(pf_batch *b, int32_t k);
+int pf_batch_synchronize(pf_batch *b);
+/* Host-side step accounting of an instance (no device wait). */
+int pf_batch_status(pf_batch *b, int32_t slot, int32_t *steps_done, int32_t *steps_left);
+/* Synchronous: the instance's scalars as pf_run_steps leaves them (p->it,
+ * TP, M0, Cs_eq, Cl_eq; L->time). */
+int pf_batch_scalars(pf_batch *b, int32_t slot, pf_params *p, pf_loop *L);
+/* Synchronous: phi_new, c_new, T_new of the last completed step (any may be
+ * NULL). */
+int pf_batch_download(pf_batch *b, int32_t slot, double *phi, double *c, double *T);
+/* Synchronous: diagnosticos of the instance (as pf_diagnosticos). */
+int pf_batch_diagnosticos(pf_batch *b, int32_t slot, double out[5]);
+int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget);
+/* Size classes: precision x plane-area bucket (<= 48^2, <= 96^2, <= 160^2,
+ * larger); each class has a tile shape (0 = 64x8, 1 = 32x8, 2 = 32x16) and a
+ * z-split target (work-groups per instance). */
+int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny);
+int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape, int32_t target_groups);
+int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape,
+                        int32_t *target_groups);
+/* Times every shape x z-split for each size class present on the batch's own
+ * instances (no step is taken) and keeps the fastest.  Synchronous. */
+int pf_batch_autotune(pf_batch *b, int32_t reps);
+
 /* ---- measurement ----------------------------------------------------------
  * Per-kernel durations from HIP events recorded around each launch on the
  * context's stream (no counterpart in the reference, whose only timer is the

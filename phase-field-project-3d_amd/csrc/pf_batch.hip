@@ -1,0 +1,1018 @@
+// pf_batch.hip -- the batched form: many independent instances of uneven
+// grid sizes advanced together, one launch per sweep for all of them.
+//
+// Every instance keeps its fields and its loop scalars (it, TP, Cs/Cl, M0,
+// the REAL(4) time, buffer roles, steps left) on the device.  A step is
+//   k_batch_begin          per instance: principal.f90:96-104 + the kernel
+//                          scalars of the sweeps (one thread per instance)
+//   (nv-1) x [sweep A, phase decoration]      one launch per tile shape
+//   sweep B, phase + solute decorations       (work-group -> tile table)
+//   k_batch_end            per instance: auxiliar M0 (principal.f90:157),
+//                          buffer roles, steps left
+// so K steps need no host work and are captured once in a hipGraph that is
+// replayed.  An instance that has run its steps goes inactive on the device
+// (its work-groups exit at once) until the host removes it.
+//
+// The sweeps are the single-instance z-march (pf_march.h) with the same
+// arithmetic, and the per-instance scalars are computed with the host loop's
+// functions (pf_common.h, contraction off), so an instance's result does
+// not depend on what else is in the batch, nor on the tiling.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "../../include/pf_capi.h"
+#include "pf_common.h"
+#include "pf_kernels.h"
+#include "pf_march.h"
+
+namespace pf {
+namespace {
+
+using Tile64x8 = Tile<64, 8>;
+using Tile32x8 = Tile<32, 8>;
+using Tile32x16 = Tile<32, 16>;
+constexpr int kNShapes = 3;
+constexpr int kShapeX[kNShapes] = {64, 32, 32};
+constexpr int kShapeY[kNShapes] = {8, 8, 16};
+
+// Device-resident state of one instance (arithmetic type R).
+template <typename R>
+struct InstDev {
+  Geo ga, gb;           // sweep geometry (k ranges, tiles, z segments)
+  KParams kt, k;        // temperatura scalars (TP of the previous step), sweep scalars
+  SC<R> s;              // derived sweep scalars of this step
+  R *phi[2], *c[2], *T[2];
+  int po, co, to;       // buffer of phi_old / c_old / T of the current step
+  int nv;               // phase sweeps per step
+  int active, steps_left, steps_done;
+  // host-loop scalars (principal.f90:96-157)
+  double T0, taxa, Tm, Me;
+  float Sigma, Beta, ajuste, time, Ke;
+  int pad;
+};
+
+struct TileRef {
+  int inst, bx, by, sg;
+};
+struct Chunk {
+  int inst;
+  long long first;
+};
+constexpr int kChunkCells = 1024;
+
+// ---- per-step scalar kernels ------------------------------------------------
+
+template <typename R>
+__global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n) {
+#pragma clang fp contract(off)
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  InstDev<R> &I = inst[q];
+  if (!I.active) return;
+  I.k.it += 1;
+  const int it = I.k.it;
+  I.time = (float)(I.k.dt + (double)I.time);  // principal.f90:96
+  I.kt = I.k;                                 // temperatura sees TP of step it-1
+  if (I.taxa > 0.0) I.k.TP = I.T0 - I.taxa * I.k.dt * (double)it;  // :104
+  equilibrium(I.k.TP, I.Tm, I.Ke, I.Me, I.k.Cs_eq, I.k.Cl_eq);
+  I.s = make_sc<R>(I.k);
+}
+
+template <typename R>
+__global__ void k_batch_end(InstDev<R> *__restrict__ inst, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  InstDev<R> &I = inst[q];
+  if (!I.active) return;
+  // auxiliar.f90 (principal.f90:157): M0 for the next step
+  I.k.M0 = auxiliar_m0(I.k.TP, I.k.Cl_eq, I.k.Cs_eq, I.k.R, I.k.Vm, I.k.E0, I.k.W0, I.Sigma,
+                       I.Beta, I.Ke, I.Me, I.k.D_liq, I.ajuste);
+  I.po ^= (I.nv & 1);  // principal.f90:127-150, 200-226 as buffer roles
+  I.co ^= 1;
+  I.to ^= 1;
+  I.steps_done += 1;
+  I.steps_left -= 1;
+  if (I.steps_left <= 0) I.active = 0;
+}
+
+// ---- batched sweeps and decorations ----------------------------------------
+
+template <typename R, bool SWEEP_B, typename TL>
+__global__ void __launch_bounds__(TL::NT)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
+    k_march_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
+                  const InstDev<R> *__restrict__ inst, int sweep) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);  // each XCD gets a contiguous run
+  if (t >= ntiles) return;
+  const TileRef d = tiles[t];
+  const InstDev<R> &I = inst[d.inst];
+  if (!I.active) return;
+  const int pin = I.po ^ ((sweep - 1) & 1), pout = pin ^ 1;
+  if constexpr (SWEEP_B)
+    march_tile<R, true, TL>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], I.c[I.co ^ 1],
+                            I.T[I.to], I.T[I.to ^ 1], nullptr, 1, nullptr, d.bx, d.by, d.sg);
+  else
+    march_tile<R, false, TL>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], nullptr,
+                             nullptr, nullptr, nullptr, 0, nullptr, d.bx, d.by, d.sg);
+}
+
+// which = 0: phase decoration of sweep `sweep` (shell of width 2);
+// which = 1: solute faces of the step (width 1).
+template <typename R>
+__global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ chunks, int nchunk,
+                                                    const InstDev<R> *__restrict__ inst,
+                                                    int which, int sweep) {
+  if ((int)blockIdx.x >= nchunk) return;
+  const Chunk ch = chunks[blockIdx.x];
+  const InstDev<R> &I = inst[ch.inst];
+  if (!I.active) return;
+  const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
+  const int w = which == 0 ? 2 : 1;
+  const long long cnt = shell_count(nx, ny, nz, w);
+  const long long end = ch.first + kChunkCells < cnt ? ch.first + kChunkCells : cnt;
+  const int pin = I.po ^ ((sweep - 1) & 1);
+  for (long long t = ch.first + threadIdx.x; t < end; t += 256) {
+    int i, j, k;
+    shell_flat(t, nx, ny, nz, w, i, j, k);
+    if (which == 0) phase_deco_cell(i, j, k, nx, ny, nz, (const R *)I.phi[pin], I.phi[pin ^ 1]);
+    else solute_deco_cell(i, j, k, nx, ny, nz, (const R *)I.c[I.co], I.c[I.co ^ 1]);
+  }
+}
+
+template <typename R, typename TL>
+hipError_t launch_batch_march(bool sweep_b, const TileRef *tiles, int ntiles,
+                              const InstDev<R> *inst, int sweep, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  const int per_xcd = (ntiles + 7) / 8;
+  const size_t lds =
+      sizeof(R) * (size_t)(5 * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
+  if (sweep_b)
+    hipLaunchKernelGGL((k_march_batch<R, true, TL>), grd, blk, lds, st, tiles, ntiles, per_xcd,
+                       inst, sweep);
+  else
+    hipLaunchKernelGGL((k_march_batch<R, false, TL>), grd, blk, lds, st, tiles, ntiles, per_xcd,
+                       inst, sweep);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_batch_shape(int shape, bool sweep_b, const TileRef *tiles, int ntiles,
+                              const InstDev<R> *inst, int sweep, hipStream_t st) {
+  switch (shape) {
+    case 1: return launch_batch_march<R, Tile32x8>(sweep_b, tiles, ntiles, inst, sweep, st);
+    case 2: return launch_batch_march<R, Tile32x16>(sweep_b, tiles, ntiles, inst, sweep, st);
+    default: return launch_batch_march<R, Tile64x8>(sweep_b, tiles, ntiles, inst, sweep, st);
+  }
+}
+
+// Geometry of a sweep for a tile shape and segment length.
+Geo batch_geo(int nx, int ny, int nz, bool sweep_b, int shape, int seg) {
+  Geo g;
+  g.nx = nx; g.ny = ny; g.nz = nz;
+  g.nbx = (nx - 2 + kShapeX[shape] - 1) / kShapeX[shape];
+  g.nby = (ny - 2 + kShapeY[shape] - 1) / kShapeY[shape];
+  g.k_lo = sweep_b ? 2 : 3;
+  g.k_hi = sweep_b ? nz - 1 : nz - 2;
+  const int ncell = g.k_hi - g.k_lo + 1;
+  g.seg = std::max(1, std::min(seg, ncell));
+  g.nseg = (ncell + g.seg - 1) / g.seg;
+  g.ntiles = g.nbx * g.nby * g.nseg;
+  g.per_xcd = (g.ntiles + 7) / 8;
+  return g;
+}
+
+// Size class of an instance: its precision and the bucket of its xy plane
+// (tile shape and z-march depth are chosen per class).
+int size_class(int prec, int nx, int ny) {
+  const long long a = (long long)nx * ny;
+  const int bucket = a <= 48 * 48 ? 0 : a <= 96 * 96 ? 1 : a <= 160 * 160 ? 2 : 3;
+  return prec * 4 + bucket;
+}
+constexpr int kNClasses = 8;
+
+}  // namespace
+}  // namespace pf
+
+// ---- host side ----------------------------------------------------------------
+
+struct pf_batch_slot {
+  bool used = false;
+  int prec = PF_FP64;
+  int idx = -1;  // index in the per-precision device array
+  pf_params p{};
+  pf_loop L{};
+  int nx = 0, ny = 0, nz = 0, nv = 2;
+  size_t n = 0, bytes = 0;
+  void *buf[6] = {};  // phi0, phi1, c0, c1, T0, T1
+  int steps_total = 0, steps_left = 0;
+};
+
+struct pf_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0, used = 0;
+  int nv = -1;  // shared n_varreduras of the instances
+  std::vector<pf_batch_slot> slots;
+  // device arrays of instance state, per precision
+  void *inst_dev[2] = {nullptr, nullptr};
+  int inst_cap[2] = {0, 0};
+  std::vector<int> slot_of[2];  // per precision: device index -> slot
+  // tiling per size class: tile shape and segment target
+  int cls_shape[pf::kNClasses];
+  int cls_target[pf::kNClasses];
+  // tables (rebuilt when instances change)
+  bool dirty = true;
+  pf::TileRef *tiles_dev = nullptr;
+  size_t tiles_cap = 0;
+  pf::Chunk *chunks_dev = nullptr;
+  size_t chunks_cap = 0;
+  struct Group {
+    int prec, shape;
+    bool sweep_b;
+    size_t off;
+    int n;
+  };
+  std::vector<Group> groups;
+  struct Deco {
+    int prec, which;
+    size_t off;
+    int n;
+  };
+  std::vector<Deco> decos;
+  // captured graph of graph_steps steps
+  int graph_steps = 16;
+  hipGraphExec_t graph = nullptr;
+  bool graph_valid = false;
+  double *diag_part = nullptr;
+  int diag_cap = 0;
+  double *pinned = nullptr;
+};
+
+namespace {
+
+using pf::fail_msg;
+
+#define HIP_OK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail_msg(PF_E_HIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+size_t inst_size(int prec) {
+  return prec == PF_FP64 ? sizeof(pf::InstDev<double>) : sizeof(pf::InstDev<float>);
+}
+
+int check_batch(const pf_batch *b) {
+  if (!b) return fail_msg(PF_E_ARG, "null batch");
+  return PF_OK;
+}
+
+int check_slot(const pf_batch *b, int32_t slot) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (slot < 0 || slot >= (int)b->slots.size() || !b->slots[slot].used)
+    return fail_msg(PF_E_ARG, "no instance in slot %d", slot);
+  return PF_OK;
+}
+
+template <typename R>
+pf::InstDev<R> make_inst(const pf_batch_slot &s) {
+  pf::InstDev<R> I;
+  memset(&I, 0, sizeof I);
+  I.k = pf::kparams_from(s.p);
+  I.kt = I.k;
+  I.s = pf::make_sc<R>(I.k);
+  for (int f = 0; f < 2; ++f) {
+    I.phi[f] = (R *)s.buf[f];
+    I.c[f] = (R *)s.buf[2 + f];
+    I.T[f] = (R *)s.buf[4 + f];
+  }
+  I.po = 0;
+  I.co = 0;
+  I.to = 1;  // T of step 1 (temperatura at admission) goes to T[1]
+  I.nv = s.nv;
+  I.active = s.steps_left > 0;
+  I.steps_left = s.steps_left;
+  I.steps_done = 0;
+  I.T0 = s.L.T0;
+  I.taxa = s.L.taxa_resfr;
+  I.Tm = s.p.Tm;
+  I.Me = s.p.Me;
+  I.Sigma = s.L.Sigma;
+  I.Beta = s.L.Beta;
+  I.ajuste = s.L.ajuste;
+  I.time = s.L.time;
+  I.Ke = s.p.Ke;
+  return I;
+}
+
+// Rebuilds the device instance arrays' geometry and the tile / chunk tables
+// after instances were added or removed or the tiling changed.
+template <typename R>
+void set_geo(pf::InstDev<R> &I, const pf_batch &b, const pf_batch_slot &s) {
+  const int cls = pf::size_class(s.prec, s.nx, s.ny);
+  const int shape = b.cls_shape[cls];
+  // z segments: about cls_target work-groups for the instance's plane
+  const int plane_tiles = ((s.nx - 2 + pf::kShapeX[shape] - 1) / pf::kShapeX[shape]) *
+                          ((s.ny - 2 + pf::kShapeY[shape] - 1) / pf::kShapeY[shape]);
+  int nseg = std::max(1, (b.cls_target[cls] + plane_tiles - 1) / plane_tiles);
+  int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
+  I.ga = pf::batch_geo(s.nx, s.ny, s.nz, false, shape, seg);
+  I.gb = pf::batch_geo(s.nx, s.ny, s.nz, true, shape, seg);
+}
+
+int rebuild(pf_batch *b) {
+  // tile tables: one group per (precision, shape, sweep)
+  std::vector<pf::TileRef> tiles;
+  std::vector<pf::Chunk> chunks;
+  b->groups.clear();
+  b->decos.clear();
+  for (int prec = 0; prec < 2; ++prec) {
+    const auto &ids = b->slot_of[prec];
+    if (ids.empty()) continue;
+    // read geometry from the slots (host mirror) in device-index order
+    for (int sweep_b = 0; sweep_b < 2; ++sweep_b)
+      for (int shape = 0; shape < pf::kNShapes; ++shape) {
+        const size_t off = tiles.size();
+        for (size_t q = 0; q < ids.size(); ++q) {
+          const pf_batch_slot &s = b->slots[ids[q]];
+          const int cls = pf::size_class(s.prec, s.nx, s.ny);
+          if (b->cls_shape[cls] != shape) continue;
+          pf::InstDev<double> tmp;  // geometry only
+          set_geo(tmp, *b, s);
+          const pf::Geo &g = sweep_b ? tmp.gb : tmp.ga;
+          for (int sg = 0; sg < g.nseg; ++sg)
+            for (int by = 0; by < g.nby; ++by)
+              for (int bx = 0; bx < g.nbx; ++bx) tiles.push_back({(int)q, bx, by, sg});
+        }
+        const int n = (int)(tiles.size() - off);
+        if (n) b->groups.push_back({prec, shape, sweep_b != 0, off, n});
+      }
+    for (int which = 0; which < 2; ++which) {
+      const size_t off = chunks.size();
+      for (size_t q = 0; q < ids.size(); ++q) {
+        const pf_batch_slot &s = b->slots[ids[q]];
+        const long long cnt = pf::shell_count(s.nx, s.ny, s.nz, which == 0 ? 2 : 1);
+        for (long long f = 0; f < cnt; f += pf::kChunkCells) chunks.push_back({(int)q, f});
+      }
+      b->decos.push_back({prec, which, off, (int)(chunks.size() - off)});
+    }
+  }
+  if (tiles.size() > b->tiles_cap) {
+    if (b->tiles_dev) HIP_OK(hipFree(b->tiles_dev));
+    b->tiles_dev = nullptr;
+    b->tiles_cap = 0;
+    HIP_OK(hipMalloc((void **)&b->tiles_dev, tiles.size() * sizeof(pf::TileRef)));
+    b->tiles_cap = tiles.size();
+  }
+  if (chunks.size() > b->chunks_cap) {
+    if (b->chunks_dev) HIP_OK(hipFree(b->chunks_dev));
+    b->chunks_dev = nullptr;
+    b->chunks_cap = 0;
+    HIP_OK(hipMalloc((void **)&b->chunks_dev, chunks.size() * sizeof(pf::Chunk)));
+    b->chunks_cap = chunks.size();
+  }
+  if (!tiles.empty())
+    HIP_OK(hipMemcpyAsync(b->tiles_dev, tiles.data(), tiles.size() * sizeof(pf::TileRef),
+                          hipMemcpyHostToDevice, b->stream));
+  if (!chunks.empty())
+    HIP_OK(hipMemcpyAsync(b->chunks_dev, chunks.data(), chunks.size() * sizeof(pf::Chunk),
+                          hipMemcpyHostToDevice, b->stream));
+  // the geometry of every instance on the device
+  for (int prec = 0; prec < 2; ++prec)
+    for (size_t q = 0; q < b->slot_of[prec].size(); ++q) {
+      const pf_batch_slot &s = b->slots[b->slot_of[prec][q]];
+      if (prec == PF_FP64) {
+        pf::InstDev<double> tmp;
+        set_geo(tmp, *b, s);
+        char *base = (char *)b->inst_dev[prec] + q * sizeof(pf::InstDev<double>);
+        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<double>, ga), &tmp.ga, 2 * sizeof(pf::Geo),
+                              hipMemcpyHostToDevice, b->stream));
+      } else {
+        pf::InstDev<float> tmp;
+        set_geo(tmp, *b, s);
+        char *base = (char *)b->inst_dev[prec] + q * sizeof(pf::InstDev<float>);
+        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<float>, ga), &tmp.ga, 2 * sizeof(pf::Geo),
+                              hipMemcpyHostToDevice, b->stream));
+      }
+    }
+  HIP_OK(hipStreamSynchronize(b->stream));
+  b->dirty = false;
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->graph_valid = false;
+  return PF_OK;
+}
+
+// Enqueues one step of every active instance.
+int enqueue_step(pf_batch *b) {
+  hipStream_t st = b->stream;
+  for (int prec = 0; prec < 2; ++prec) {
+    const int n = (int)b->slot_of[prec].size();
+    if (!n) continue;
+    const unsigned nb = (unsigned)((n + 63) / 64);
+    if (prec == PF_FP64)
+      hipLaunchKernelGGL(pf::k_batch_begin<double>, dim3(nb), dim3(64), 0, st,
+                         (pf::InstDev<double> *)b->inst_dev[prec], n);
+    else
+      hipLaunchKernelGGL(pf::k_batch_begin<float>, dim3(nb), dim3(64), 0, st,
+                         (pf::InstDev<float> *)b->inst_dev[prec], n);
+    HIP_OK(hipGetLastError());
+  }
+  auto sweeps = [&](bool sweep_b, int sweep) -> int {
+    for (const auto &g : b->groups) {
+      if (g.sweep_b != sweep_b) continue;
+      hipError_t e;
+      if (g.prec == PF_FP64)
+        e = pf::launch_batch_shape<double>(g.shape, sweep_b, b->tiles_dev + g.off, g.n,
+                                           (const pf::InstDev<double> *)b->inst_dev[0], sweep, st);
+      else
+        e = pf::launch_batch_shape<float>(g.shape, sweep_b, b->tiles_dev + g.off, g.n,
+                                          (const pf::InstDev<float> *)b->inst_dev[1], sweep, st);
+      if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
+    }
+    return PF_OK;
+  };
+  auto decos = [&](int which, int sweep) -> int {
+    for (const auto &d : b->decos) {
+      if (d.which != which || d.n == 0) continue;
+      if (d.prec == PF_FP64)
+        hipLaunchKernelGGL(pf::k_deco_batch<double>, dim3(d.n), dim3(256), 0, st,
+                           b->chunks_dev + d.off, d.n,
+                           (const pf::InstDev<double> *)b->inst_dev[0], which, sweep);
+      else
+        hipLaunchKernelGGL(pf::k_deco_batch<float>, dim3(d.n), dim3(256), 0, st,
+                           b->chunks_dev + d.off, d.n,
+                           (const pf::InstDev<float> *)b->inst_dev[1], which, sweep);
+      HIP_OK(hipGetLastError());
+    }
+    return PF_OK;
+  };
+  int rc;
+  const int nv = b->nv < 1 ? 1 : b->nv;
+  for (int sw = 1; sw < nv; ++sw) {
+    if ((rc = sweeps(false, sw)) != PF_OK) return rc;
+    if ((rc = decos(0, sw)) != PF_OK) return rc;
+  }
+  if ((rc = sweeps(true, nv)) != PF_OK) return rc;
+  if ((rc = decos(0, nv)) != PF_OK) return rc;
+  if ((rc = decos(1, nv)) != PF_OK) return rc;
+  for (int prec = 0; prec < 2; ++prec) {
+    const int n = (int)b->slot_of[prec].size();
+    if (!n) continue;
+    const unsigned nb = (unsigned)((n + 63) / 64);
+    if (prec == PF_FP64)
+      hipLaunchKernelGGL(pf::k_batch_end<double>, dim3(nb), dim3(64), 0, st,
+                         (pf::InstDev<double> *)b->inst_dev[prec], n);
+    else
+      hipLaunchKernelGGL(pf::k_batch_end<float>, dim3(nb), dim3(64), 0, st,
+                         (pf::InstDev<float> *)b->inst_dev[prec], n);
+    HIP_OK(hipGetLastError());
+  }
+  return PF_OK;
+}
+
+int capture_graph(pf_batch *b) {
+  if (b->graph) {
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->graph_valid = false;
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = PF_OK;
+  for (int q = 0; q < b->graph_steps && rc == PF_OK; ++q) rc = enqueue_step(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc != PF_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail_msg(PF_E_HIP, "graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    b->graph = nullptr;
+    return fail_msg(PF_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  b->graph_valid = true;
+  return PF_OK;
+}
+
+template <typename R>
+int download_field(pf_batch *b, const void *src, size_t n, double *dst) {
+  if (!dst) return PF_OK;
+  if (sizeof(R) == 8) {
+    HIP_OK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return PF_OK;
+  }
+  std::vector<float> tmp(n);
+  HIP_OK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, b->stream));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  for (size_t q = 0; q < n; ++q) dst[q] = (double)tmp[q];
+  return PF_OK;
+}
+
+int upload_field(pf_batch *b, void *dst, size_t n, int prec, const double *src) {
+  if (prec == PF_FP64) {
+    HIP_OK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return PF_OK;
+  }
+  std::vector<float> tmp(n);
+  for (size_t q = 0; q < n; ++q) tmp[q] = (float)src[q];
+  HIP_OK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return PF_OK;
+}
+
+// Reads the device state of a slot into a host InstDev (FP64 layout for
+// the scalars shared by both precisions).
+template <typename R>
+int read_inst(pf_batch *b, const pf_batch_slot &s, pf::InstDev<R> &I) {
+  HIP_OK(hipMemcpyAsync(&I, (char *)b->inst_dev[s.prec] + (size_t)s.idx * sizeof(I), sizeof(I),
+                        hipMemcpyDeviceToHost, b->stream));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return PF_OK;
+}
+
+int ensure_inst_cap(pf_batch *b, int prec, int need) {
+  if (need <= b->inst_cap[prec]) return PF_OK;
+  const int cap = std::max(need, std::max(8, 2 * b->inst_cap[prec]));
+  void *nw = nullptr;
+  HIP_OK(hipMalloc(&nw, (size_t)cap * inst_size(prec)));
+  if (b->inst_dev[prec]) {
+    HIP_OK(hipMemcpyAsync(nw, b->inst_dev[prec], (size_t)b->slot_of[prec].size() * inst_size(prec),
+                          hipMemcpyDeviceToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipFree(b->inst_dev[prec]));
+  }
+  b->inst_dev[prec] = nw;
+  b->inst_cap[prec] = cap;
+  return PF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pf_batch_create(size_t hbm_budget, pf_batch **out) {
+  if (!out) return fail_msg(PF_E_ARG, "null output");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail_msg(PF_E_NODEV, "no HIP device visible");
+  pf_batch *b = new pf_batch();
+  for (int c = 0; c < pf::kNClasses; ++c) {
+    b->cls_shape[c] = 1;  // 32x8 until pf_batch_autotune measures
+    b->cls_target[c] = 512;
+  }
+  hipGetDevice(&b->device);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail_msg(PF_E_HIP, "hipStreamCreate");
+  }
+  size_t fr = 0, tot = 0;
+  hipMemGetInfo(&fr, &tot);
+  b->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
+  if (hipHostMalloc((void **)&b->pinned, 8 * sizeof(double)) != hipSuccess) b->pinned = nullptr;
+  *out = b;
+  return PF_OK;
+}
+
+int pf_batch_destroy(pf_batch *b) {
+  if (!b) return PF_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->graph) hipGraphExecDestroy(b->graph);
+  for (auto &s : b->slots)
+    for (void *p : s.buf)
+      if (p) hipFree(p);
+  for (int prec = 0; prec < 2; ++prec)
+    if (b->inst_dev[prec]) hipFree(b->inst_dev[prec]);
+  if (b->tiles_dev) hipFree(b->tiles_dev);
+  if (b->chunks_dev) hipFree(b->chunks_dev);
+  if (b->diag_part) hipFree(b->diag_part);
+  if (b->pinned) hipHostFree(b->pinned);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return PF_OK;
+}
+
+int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nsteps,
+                 const double *phi, const double *c, const double *T, int32_t *slot) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (!p || !L || !phi || !c || !T || !slot) return fail_msg(PF_E_ARG, "null argument");
+  *slot = -1;
+  if (p->nx < 6 || p->ny < 6 || p->nz < 6)
+    return fail_msg(PF_E_ARG, "batched grids need every extent >= 6 (got %dx%dx%d)", p->nx, p->ny,
+                    p->nz);
+  if ((long long)p->nx * p->ny > (1LL << 30)) return fail_msg(PF_E_ARG, "plane too large");
+  if (p->precision != PF_FP64 && p->precision != PF_FP32_MIXED)
+    return fail_msg(PF_E_ARG, "unknown precision %d", p->precision);
+  if (nsteps < 0) return fail_msg(PF_E_ARG, "negative step count");
+  const int nv = p->n_varreduras < 1 ? 1 : p->n_varreduras;
+  bool any = false;
+  for (auto &s : b->slots) any = any || s.used;
+  if (any && nv != b->nv)
+    return fail_msg(PF_E_ARG, "instances of one batch share n_varreduras (%d, got %d)", b->nv, nv);
+  pf_batch_slot s;
+  s.used = true;
+  s.prec = p->precision;
+  s.p = *p;
+  s.L = *L;
+  s.nx = p->nx; s.ny = p->ny; s.nz = p->nz;
+  s.nv = nv;
+  s.n = (size_t)p->nx * p->ny * p->nz;
+  const size_t esz = s.prec == PF_FP64 ? 8 : 4;
+  s.bytes = 6 * s.n * esz;
+  if (b->used + s.bytes > b->budget)
+    return fail_msg(PF_E_NOMEM, "instance needs %zu bytes, %zu of %zu in use", s.bytes, b->used,
+                    b->budget);
+  s.steps_total = s.steps_left = nsteps;
+  for (int f = 0; f < 6; ++f) {
+    if (hipMalloc(&s.buf[f], s.n * esz) != hipSuccess) {
+      for (int g = 0; g < f; ++g) hipFree(s.buf[g]);
+      return fail_msg(PF_E_NOMEM, "hipMalloc of %zu bytes failed", s.n * esz);
+    }
+  }
+  auto cleanup = [&](int code) {
+    for (void *q : s.buf)
+      if (q) hipFree(q);
+    return code;
+  };
+  // phi_old, c_old, T_old into both buffers of each pair (the new buffers are
+  // fully rewritten before they are read; T faces stay TOLD in both)
+  if ((rc = upload_field(b, s.buf[0], s.n, s.prec, phi)) != PF_OK) return cleanup(rc);
+  if ((rc = upload_field(b, s.buf[2], s.n, s.prec, c)) != PF_OK) return cleanup(rc);
+  if ((rc = upload_field(b, s.buf[4], s.n, s.prec, T)) != PF_OK) return cleanup(rc);
+  if ((rc = upload_field(b, s.buf[5], s.n, s.prec, T)) != PF_OK) return cleanup(rc);
+  // temperatura of step 1 (principal.f90:98) with DPP = 0: c[1] is dead
+  // until the first solute update, so it serves as the zero DPP field
+  if (hipMemsetAsync(s.buf[3], 0, s.n * esz, b->stream) != hipSuccess)
+    return cleanup(fail_msg(PF_E_HIP, "hipMemsetAsync"));
+  const pf::KParams kt = pf::kparams_from(*p);
+  hipError_t e;
+  if (s.prec == PF_FP64)
+    e = pf::launch_temperatura<double>(kt, (const double *)s.buf[4], (const double *)s.buf[0],
+                                       (const double *)s.buf[3], (double *)s.buf[5], b->stream);
+  else
+    e = pf::launch_temperatura<float>(kt, (const float *)s.buf[4], (const float *)s.buf[0],
+                                      (const float *)s.buf[3], (float *)s.buf[5], b->stream);
+  if (e != hipSuccess) return cleanup(fail_msg(PF_E_HIP, "temperatura: %s", hipGetErrorString(e)));
+  // device state
+  const int prec = s.prec;
+  if ((rc = ensure_inst_cap(b, prec, (int)b->slot_of[prec].size() + 1)) != PF_OK)
+    return cleanup(rc);
+  int id = -1;
+  for (size_t q = 0; q < b->slots.size(); ++q)
+    if (!b->slots[q].used) {
+      id = (int)q;
+      break;
+    }
+  if (id < 0) {
+    id = (int)b->slots.size();
+    b->slots.push_back(pf_batch_slot());
+  }
+  s.idx = (int)b->slot_of[prec].size();
+  if (prec == PF_FP64) {
+    pf::InstDev<double> I = make_inst<double>(s);
+    set_geo(I, *b, s);
+    e = hipMemcpyAsync((char *)b->inst_dev[prec] + (size_t)s.idx * sizeof I, &I, sizeof I,
+                       hipMemcpyHostToDevice, b->stream);
+  } else {
+    pf::InstDev<float> I = make_inst<float>(s);
+    set_geo(I, *b, s);
+    e = hipMemcpyAsync((char *)b->inst_dev[prec] + (size_t)s.idx * sizeof I, &I, sizeof I,
+                       hipMemcpyHostToDevice, b->stream);
+  }
+  if (e != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+    return cleanup(fail_msg(PF_E_HIP, "instance upload failed"));
+  b->slot_of[prec].push_back(id);
+  b->slots[id] = s;
+  b->used += s.bytes;
+  b->nv = nv;
+  b->dirty = true;
+  *slot = id;
+  return PF_OK;
+}
+
+int pf_batch_remove(pf_batch *b, int32_t slot) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  pf_batch_slot &s = b->slots[slot];
+  const int prec = s.prec;
+  auto &ids = b->slot_of[prec];
+  // move the last device entry into the removed one's place
+  const int last = (int)ids.size() - 1;
+  if (s.idx != last) {
+    const size_t sz = inst_size(prec);
+    HIP_OK(hipMemcpyAsync((char *)b->inst_dev[prec] + (size_t)s.idx * sz,
+                          (char *)b->inst_dev[prec] + (size_t)last * sz, sz,
+                          hipMemcpyDeviceToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    ids[s.idx] = ids[last];
+    b->slots[ids[s.idx]].idx = s.idx;
+  }
+  ids.pop_back();
+  for (void *p : s.buf)
+    if (p) HIP_OK(hipFree(p));
+  b->used -= s.bytes;
+  s = pf_batch_slot();
+  b->dirty = true;
+  return PF_OK;
+}
+
+int pf_batch_run(pf_batch *b, int32_t nsteps) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (nsteps < 0) return fail_msg(PF_E_ARG, "negative step count");
+  if (b->slot_of[0].empty() && b->slot_of[1].empty()) return PF_OK;
+  if (b->dirty && (rc = rebuild(b)) != PF_OK) return rc;
+  int left = nsteps;
+  if (b->graph_steps > 0 && left >= b->graph_steps) {
+    if (!b->graph_valid && (rc = capture_graph(b)) != PF_OK) return rc;
+    while (left >= b->graph_steps) {
+      HIP_OK(hipGraphLaunch(b->graph, b->stream));
+      left -= b->graph_steps;
+    }
+  }
+  for (; left > 0; --left)
+    if ((rc = enqueue_step(b)) != PF_OK) return rc;
+  for (auto &s : b->slots)
+    if (s.used) s.steps_left = std::max(0, s.steps_left - nsteps);
+  return PF_OK;
+}
+
+int pf_batch_set_graph_steps(pf_batch *b, int32_t k) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (k < 0) return fail_msg(PF_E_ARG, "negative graph length");
+  b->graph_steps = k;
+  if (b->graph) {
+    HIP_OK(hipStreamSynchronize(b->stream));
+    hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->graph_valid = false;
+  return PF_OK;
+}
+
+int pf_batch_synchronize(pf_batch *b) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return PF_OK;
+}
+
+int pf_batch_status(pf_batch *b, int32_t slot, int32_t *steps_done, int32_t *steps_left) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  const pf_batch_slot &s = b->slots[slot];
+  if (steps_done) *steps_done = s.steps_total - s.steps_left;
+  if (steps_left) *steps_left = s.steps_left;
+  return PF_OK;
+}
+
+int pf_batch_scalars(pf_batch *b, int32_t slot, pf_params *p, pf_loop *L) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  const pf_batch_slot &s = b->slots[slot];
+  pf::KParams k;
+  float time;
+  if (s.prec == PF_FP64) {
+    pf::InstDev<double> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    k = I.k;
+    time = I.time;
+  } else {
+    pf::InstDev<float> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    k = I.k;
+    time = I.time;
+  }
+  if (p) {
+    *p = s.p;
+    p->it = k.it;
+    p->TP = k.TP;
+    p->M0 = k.M0;
+    p->Cs_eq = k.Cs_eq;
+    p->Cl_eq = k.Cl_eq;
+  }
+  if (L) {
+    *L = s.L;
+    L->time = time;
+  }
+  return PF_OK;
+}
+
+int pf_batch_download(pf_batch *b, int32_t slot, double *phi, double *c, double *T) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  const pf_batch_slot &s = b->slots[slot];
+  int po, co, to;
+  if (s.prec == PF_FP64) {
+    pf::InstDev<double> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    po = I.po; co = I.co; to = I.to;
+  } else {
+    pf::InstDev<float> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    po = I.po; co = I.co; to = I.to;
+  }
+  // after a step: phi[po], c[co] are the newest; T[to] holds the next step's
+  // temperature, T[to ^ 1] the reference's TNEW of the last step
+  if (s.prec == PF_FP64) {
+    if ((rc = download_field<double>(b, s.buf[po], s.n, phi)) != PF_OK) return rc;
+    if ((rc = download_field<double>(b, s.buf[2 + co], s.n, c)) != PF_OK) return rc;
+    return download_field<double>(b, s.buf[4 + (to ^ 1)], s.n, T);
+  }
+  if ((rc = download_field<float>(b, s.buf[po], s.n, phi)) != PF_OK) return rc;
+  if ((rc = download_field<float>(b, s.buf[2 + co], s.n, c)) != PF_OK) return rc;
+  return download_field<float>(b, s.buf[4 + (to ^ 1)], s.n, T);
+}
+
+int pf_batch_diagnosticos(pf_batch *b, int32_t slot, double out[5]) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail_msg(PF_E_ARG, "null output");
+  const pf_batch_slot &s = b->slots[slot];
+  const int nblk = pf::diag_blocks(s.ny, s.nz);
+  if (nblk + 1 > b->diag_cap) {
+    if (b->diag_part) HIP_OK(hipFree(b->diag_part));
+    b->diag_part = nullptr;
+    b->diag_cap = 0;
+    HIP_OK(hipMalloc((void **)&b->diag_part, sizeof(double) * 5 * (size_t)(nblk + 1)));
+    b->diag_cap = nblk + 1;
+  }
+  pf_params p;
+  if ((rc = pf_batch_scalars(b, slot, &p, nullptr)) != PF_OK) return rc;
+  int po, co;
+  if (s.prec == PF_FP64) {
+    pf::InstDev<double> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    po = I.po; co = I.co;
+  } else {
+    pf::InstDev<float> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    po = I.po; co = I.co;
+  }
+  const pf::KParams k = pf::kparams_from(p);
+  double *outd = b->diag_part + 5 * nblk;
+  hipError_t e;
+  if (s.prec == PF_FP64)
+    e = pf::launch_diag<double>(k, (const double *)s.buf[po], (const double *)s.buf[2 + co],
+                                b->diag_part, nblk, outd, b->stream);
+  else
+    e = pf::launch_diag<float>(k, (const float *)s.buf[po], (const float *)s.buf[2 + co],
+                               b->diag_part, nblk, outd, b->stream);
+  if (e != hipSuccess) return fail_msg(PF_E_HIP, "diagnostics: %s", hipGetErrorString(e));
+  HIP_OK(hipMemcpyAsync(out, outd, 5 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return PF_OK;
+}
+
+int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (used) *used = b->used;
+  if (budget) *budget = b->budget;
+  return PF_OK;
+}
+
+int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape, int32_t target_groups) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (size_class < 0 || size_class >= pf::kNClasses || shape < 0 || shape >= pf::kNShapes ||
+      target_groups < 1)
+    return fail_msg(PF_E_ARG, "size class in [0,%d), shape in [0,%d), target >= 1",
+                    pf::kNClasses, pf::kNShapes);
+  HIP_OK(hipStreamSynchronize(b->stream));
+  b->cls_shape[size_class] = shape;
+  b->cls_target[size_class] = target_groups;
+  b->dirty = true;
+  return PF_OK;
+}
+
+int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape,
+                        int32_t *target_groups) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (size_class < 0 || size_class >= pf::kNClasses) return fail_msg(PF_E_ARG, "bad size class");
+  if (shape) *shape = b->cls_shape[size_class];
+  if (target_groups) *target_groups = b->cls_target[size_class];
+  return PF_OK;
+}
+
+int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny) {
+  return pf::size_class(precision == PF_FP32_MIXED ? 1 : 0, nx, ny);
+}
+
+// Times each tile shape x segment target for every size class present, on
+// the batch's own instances of that class alone, and keeps the fastest.
+// Both sweeps are timed with the roles of the first phase sweep (read the
+// current phi, c, T; write the step's dead buffers), so the state is
+// unchanged: no step is taken.
+int pf_batch_autotune(pf_batch *b, int32_t reps) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (reps < 1) return fail_msg(PF_E_ARG, "reps must be >= 1");
+  static const int targets[] = {128, 512, 2048};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  pf::TileRef *tmp_dev = nullptr;
+  size_t tmp_cap = 0;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  for (int cls = 0; cls < pf::kNClasses && rc == PF_OK; ++cls) {
+    const int prec = cls / 4;
+    std::vector<int> members;  // device indices of this class
+    for (size_t q = 0; q < b->slot_of[prec].size(); ++q) {
+      const pf_batch_slot &s = b->slots[b->slot_of[prec][q]];
+      if (pf::size_class(s.prec, s.nx, s.ny) == cls) members.push_back((int)q);
+    }
+    if (members.empty()) continue;
+    float best = 1e30f;
+    int bs = b->cls_shape[cls], bt = b->cls_target[cls];
+    for (int shape = 0; shape < pf::kNShapes && rc == PF_OK; ++shape)
+      for (int target : targets) {
+        b->cls_shape[cls] = shape;
+        b->cls_target[cls] = target;
+        if ((rc = rebuild(b)) != PF_OK) break;  // device geometry of the candidate
+        std::vector<pf::TileRef> tl[2];
+        for (int sweep_b = 0; sweep_b < 2; ++sweep_b)
+          for (int q : members) {
+            pf::InstDev<double> g;
+            set_geo(g, *b, b->slots[b->slot_of[prec][q]]);
+            const pf::Geo &gg = sweep_b ? g.gb : g.ga;
+            for (int sg = 0; sg < gg.nseg; ++sg)
+              for (int by = 0; by < gg.nby; ++by)
+                for (int bx = 0; bx < gg.nbx; ++bx) tl[sweep_b].push_back({q, bx, by, sg});
+          }
+        const size_t need = tl[0].size() + tl[1].size();
+        if (need > tmp_cap) {
+          if (tmp_dev) hipFree(tmp_dev);
+          tmp_dev = nullptr;
+          tmp_cap = 0;
+          if (hipMalloc((void **)&tmp_dev, need * sizeof(pf::TileRef)) != hipSuccess) {
+            rc = fail_msg(PF_E_NOMEM, "autotune table");
+            break;
+          }
+          tmp_cap = need;
+        }
+        hipMemcpyAsync(tmp_dev, tl[0].data(), tl[0].size() * sizeof(pf::TileRef),
+                       hipMemcpyHostToDevice, b->stream);
+        hipMemcpyAsync(tmp_dev + tl[0].size(), tl[1].data(), tl[1].size() * sizeof(pf::TileRef),
+                       hipMemcpyHostToDevice, b->stream);
+        hipError_t e = hipSuccess;
+        for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+          if (r == 1) e = hipEventRecord(e0, b->stream);
+          for (int sweep_b = 0; sweep_b < 2 && e == hipSuccess; ++sweep_b) {
+            const pf::TileRef *t = tmp_dev + (sweep_b ? tl[0].size() : 0);
+            const int n = (int)tl[sweep_b].size();
+            if (prec == PF_FP64)
+              e = pf::launch_batch_shape<double>(shape, sweep_b != 0, t, n,
+                                                 (const pf::InstDev<double> *)b->inst_dev[0], 1,
+                                                 b->stream);
+            else
+              e = pf::launch_batch_shape<float>(shape, sweep_b != 0, t, n,
+                                                (const pf::InstDev<float> *)b->inst_dev[1], 1,
+                                                b->stream);
+          }
+        }
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(e));
+          break;
+        }
+        ms /= (float)reps;
+        if (ms < best) {
+          best = ms;
+          bs = shape;
+          bt = target;
+        }
+      }
+    b->cls_shape[cls] = bs;
+    b->cls_target[cls] = bt;
+  }
+  hipStreamSynchronize(b->stream);
+  if (tmp_dev) hipFree(tmp_dev);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  b->dirty = true;
+  return rc;
+}
+
+}  // extern "C"

@@ -94,6 +94,110 @@ PF_HD void solute_source(int &i, int &j, int &k, int nx, int ny, int nz) {
   }
 }
 
+// source/equacao_campo_fase.f90:21-22.
+PF_HD void equilibrium(double TP, double Tm, float Ke, double Me, double &Cs_eq, double &Cl_eq) {
+#pragma clang fp contract(off)
+  Cs_eq = ((Tm - TP) * (double)Ke) / Me;
+  Cl_eq = (Tm - TP) / Me;
+}
+
+// source/auxiliar.f90:14-42: trapezoid rule over phi_i = 0.11..0.89 with the
+// hard-coded end values, then 1/M0 from the thin-interface relation, in the
+// reference's operation order with its REAL(4) literals and (1-Ke).  Shared
+// by the host loop and the batched form's per-instance device update, so
+// both give the same bits.
+PF_HD double auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm, double E0,
+                         double W0, float Sigma, float Beta, float Ke, double Me, double D_liq,
+                         float ajuste) {
+#pragma clang fp contract(off)
+  double acc = 0.0, phi_i = 0.11;
+  do {
+    const double h = phi_i * phi_i * (3.0 - 2.0 * phi_i);
+    const double f1 = phi_i * (1.0 - phi_i);
+    const double num = h * (1.0 - h);
+    const double den = ((1.0 - h) * Cl_eq * (1.0 - Cl_eq) + h * Cs_eq * (1.0 - Cs_eq)) * f1;
+    acc = acc + num / den;
+    phi_i = phi_i + 0.01;
+  } while (phi_i <= 0.8999);
+  const double integral = ((double)0.01f / 2.0) * ((double)54.39180136f + 2.0 * acc + 234.073053);
+  const double qsi = R * TP * (1.0 / Vm) * (Cl_eq - Cs_eq) * (Cl_eq - Cs_eq) * integral;
+  const float omk = 1.0f - Ke;
+  const double inv = (E0 * E0 / (double)Sigma) *
+                     (((double)Beta * R * TP * (double)omk / (Vm * Me)) +
+                      (E0 * qsi / (D_liq * sqrt(2.0 * W0))));
+  return (double)ajuste * 1.0 / inv;
+}
+
+PF_HD size_t cidx(int i, int j, int k, int nx, int ny) {
+  return (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+}
+// Cell t (0-based) of the boundary shell of thickness w (2 for the phase
+// decoration, 1 for the solute faces), enumerated plane by plane: the w
+// planes at each z face are whole; every other plane contributes its w rows
+// at each y face (x-contiguous) and the w cells at each x face of its other
+// rows.  Returns false when t is past the shell.
+__host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny, int nz, int w, int &i,
+                                           int &j, int &k) {
+  const long long plane = (long long)nx * ny;
+  const long long full = (long long)w * plane;           // the w planes at one z face
+  const long long per = 2LL * w * nx + 2LL * w * (ny - 2 * w);  // one middle plane
+  const long long mid = per * (nz - 2 * w);
+  if (t < 0 || t >= 2 * full + mid) return false;
+  long long r;
+  if (t < full || t >= full + mid) {
+    const long long u = t < full ? t : t - full - mid;
+    const int kk = (int)(u / plane);
+    k = t < full ? kk + 1 : nz - w + 1 + kk;
+    r = u - (long long)kk * plane;
+    i = (int)(r % nx) + 1;
+    j = (int)(r / nx) + 1;
+    return true;
+  }
+  const long long u = t - full;
+  const int km = (int)(u / per);
+  k = w + 1 + km;
+  r = u - (long long)km * per;
+  const long long rows = 2LL * w * nx;
+  if (r < rows) {
+    const int q = (int)(r / nx);
+    i = (int)(r % nx) + 1;
+    j = q < w ? q + 1 : ny - (2 * w - 1 - q);
+    return true;
+  }
+  const int q = (int)(r - rows);
+  const int col = q % (2 * w);
+  j = q / (2 * w) + w + 1;
+  i = col < w ? col + 1 : nx - (2 * w - 1 - col);
+  return true;
+}
+
+__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w) {
+  return 2LL * w * nx * ny + (2LL * w * nx + 2LL * w * (ny - 2 * w)) * (nz - 2 * w);
+}
+
+// One cell of the phase decoration (equacao_campo_fase.f90:151-167 in
+// closed form): the value of its mapped source cell, from the swept field
+// inside [3,n-2]^3 and from the pre-sweep field elsewhere.
+template <typename R>
+PF_HD void phase_deco_cell(int i, int j, int k, int nx, int ny, int nz, const R *po, R *pn) {
+  const int si = phase_map(i, nx), sj = phase_map(j, ny), sk = phase_map(k, nz);
+  const bool inner = si >= 3 && si <= nx - 2 && sj >= 3 && sj <= ny - 2 && sk >= 3 && sk <= nz - 2;
+  const size_t s = cidx(si, sj, sk, nx, ny);
+  pn[cidx(i, j, k, nx, ny)] = inner ? pn[s] : po[s];
+}
+
+// One cell of the solute face copy (concentracao_soluto.f90:116-124): the
+// inner neighbour's new value, or the snapshot's where the source is itself
+// on the boundary (edges, z > x > y priority).
+template <typename R>
+PF_HD void solute_deco_cell(int i, int j, int k, int nx, int ny, int nz, const R *co, R *cn) {
+  int si = i, sj = j, sk = k;
+  solute_source(si, sj, sk, nx, ny, nz);
+  const bool inner = si > 1 && si < nx && sj > 1 && sj < ny && sk > 1 && sk < nz;
+  const size_t s = cidx(si, sj, sk, nx, ny);
+  cn[cidx(i, j, k, nx, ny)] = inner ? cn[s] : co[s];
+}
+
 // Kernel parameter block, passed by value.  The scalars each launch needs,
 // already combined the way the reference combines them.
 struct KParams {
@@ -106,5 +210,24 @@ struct KParams {
   double Ke;  // (double)(float)Ke
   double Cs_eq, Cl_eq;
 };
+
+// The kernel scalars of a pf_params (Cs_eq/Cl_eq recomputed from TP, as
+// equacao_campo_fase.f90:21-22 does for every sweep).
+inline KParams kparams_from(const pf_params &p) {
+  KParams k;
+  k.nx = p.nx; k.ny = p.ny; k.nz = p.nz; k.it = p.it;
+  k.h_quintico = p.h_quintico; k.ruido_tipo = p.ruido_tipo; k.seed = p.seed;
+  k.dx = p.dx; k.dy = p.dy; k.dz = p.dz;
+  k.dt = p.dt; k.D = p.D; k.delta_H = p.delta_H; k.cp_s = p.cp_s; k.TP = p.TP;
+  k.R = p.R; k.Vm = p.Vm; k.E0 = p.E0; k.W0 = p.W0; k.M0 = p.M0; k.delta_eps = p.delta_eps;
+  k.AAA = p.AAA; k.D_liq = p.D_liq; k.D_sol = p.D_sol;
+  k.Ke = (double)p.Ke;
+  equilibrium(p.TP, p.Tm, p.Ke, p.Me, k.Cs_eq, k.Cl_eq);
+  return k;
+}
+
+// Records the calling thread's last error message (pf_last_error) and
+// returns code.  Defined in pf_runtime.hip.
+int fail_msg(int code, const char *fmt, ...);
 
 }  // namespace pf

@@ -47,32 +47,18 @@ int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E
   return -1;
 }
 
-// source/auxiliar.f90:14-42: trapezoid rule over phi_i = 0.11..0.89 with the
-// hard-coded end values, then 1/M0 from the thin-interface relation.
+// source/auxiliar.f90:14-42 (pf::auxiliar_m0 in pf_common.h).
 double pf_auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm, double E0,
                       double W0, float Sigma, float Beta, float Ke, double Me, double D_liq,
                       float ajuste) {
-  double acc = 0.0, phi_i = 0.11;
-  do {
-    const double h = phi_i * phi_i * (3.0 - 2.0 * phi_i);
-    const double f1 = phi_i * (1.0 - phi_i);
-    const double num = h * (1.0 - h);
-    const double den = ((1.0 - h) * Cl_eq * (1.0 - Cl_eq) + h * Cs_eq * (1.0 - Cs_eq)) * f1;
-    acc = acc + num / den;
-    phi_i = phi_i + 0.01;
-  } while (phi_i <= 0.8999);
-  const double integral = ((double)0.01f / 2.0) * ((double)54.39180136f + 2.0 * acc + 234.073053);
-  const double qsi = R * TP * (1.0 / Vm) * (Cl_eq - Cs_eq) * (Cl_eq - Cs_eq) * integral;
-  const float omk = 1.0f - Ke;
-  const double inv = (E0 * E0 / (double)Sigma) *
-                     (((double)Beta * R * TP * (double)omk / (Vm * Me)) +
-                      (E0 * qsi / (D_liq * sqrt(2.0 * W0))));
-  return (double)ajuste * 1.0 / inv;
+  return pf::auxiliar_m0(TP, Cl_eq, Cs_eq, R, Vm, E0, W0, Sigma, Beta, Ke, Me, D_liq, ajuste);
 }
 
 void pf_equilibrium(double TP, double Tm, float Ke, double Me, double *Cs_eq, double *Cl_eq) {
-  if (Cs_eq) *Cs_eq = ((Tm - TP) * (double)Ke) / Me;
-  if (Cl_eq) *Cl_eq = (Tm - TP) / Me;
+  double cs, cl;
+  pf::equilibrium(TP, Tm, Ke, Me, cs, cl);
+  if (Cs_eq) *Cs_eq = cs;
+  if (Cl_eq) *Cl_eq = cl;
 }
 
 double pf_time_step(double dx, double D_liq) { return (dx * dx) / (7.5 * D_liq); }

@@ -24,9 +24,7 @@ __device__ __forceinline__ R rlog(R x) { return log(x); }
 template <>
 __device__ __forceinline__ float rlog<float>(float x) { return logf(x); }
 
-__device__ __forceinline__ size_t cidx(int i, int j, int k, int nx, int ny) {
-  return (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
-}
+
 
 // ---------------------------------------------------------------------------
 // temperatura.f90:55-70 + temp_body.inc:11-45.  Boundary cells keep TOLD.
@@ -198,50 +196,6 @@ __global__ void __launch_bounds__(256) k_phase_interior(KParams p, const R *__re
   dpp[c] = d;
 }
 
-// Cell t (0-based) of the boundary shell of thickness w (2 for the phase
-// decoration, 1 for the solute faces), enumerated plane by plane: the w
-// planes at each z face are whole; every other plane contributes its w rows
-// at each y face (x-contiguous) and the w cells at each x face of its other
-// rows.  Returns false when t is past the shell.
-__host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny, int nz, int w, int &i,
-                                           int &j, int &k) {
-  const long long plane = (long long)nx * ny;
-  const long long full = (long long)w * plane;           // the w planes at one z face
-  const long long per = 2LL * w * nx + 2LL * w * (ny - 2 * w);  // one middle plane
-  const long long mid = per * (nz - 2 * w);
-  if (t < 0 || t >= 2 * full + mid) return false;
-  long long r;
-  if (t < full || t >= full + mid) {
-    const long long u = t < full ? t : t - full - mid;
-    const int kk = (int)(u / plane);
-    k = t < full ? kk + 1 : nz - w + 1 + kk;
-    r = u - (long long)kk * plane;
-    i = (int)(r % nx) + 1;
-    j = (int)(r / nx) + 1;
-    return true;
-  }
-  const long long u = t - full;
-  const int km = (int)(u / per);
-  k = w + 1 + km;
-  r = u - (long long)km * per;
-  const long long rows = 2LL * w * nx;
-  if (r < rows) {
-    const int q = (int)(r / nx);
-    i = (int)(r % nx) + 1;
-    j = q < w ? q + 1 : ny - (2 * w - 1 - q);
-    return true;
-  }
-  const int q = (int)(r - rows);
-  const int col = q % (2 * w);
-  j = q / (2 * w) + w + 1;
-  i = col < w ? col + 1 : nx - (2 * w - 1 - col);
-  return true;
-}
-
-__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w) {
-  return 2LL * w * nx * ny + (2LL * w * nx + 2LL * w * (ny - 2 * w)) * (nz - 2 * w);
-}
-
 // The 12-plane decoration for the cells outside [3,n-2]^3.  Reads phi_new
 // only inside [3,n-2]^3 (never written here) and phi_old elsewhere.
 template <typename R>
@@ -251,10 +205,7 @@ __global__ void __launch_bounds__(256) k_phase_boundary(KParams p, const R *__re
   int i, j, k;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
        shell_flat(t, nx, ny, nz, 2, i, j, k); t += (long long)gridDim.x * blockDim.x) {
-    const int si = phase_map(i, nx), sj = phase_map(j, ny), sk = phase_map(k, nz);
-    const bool inner = si >= 3 && si <= nx - 2 && sj >= 3 && sj <= ny - 2 && sk >= 3 && sk <= nz - 2;
-    const size_t s = cidx(si, sj, sk, nx, ny);
-    pn[cidx(i, j, k, nx, ny)] = inner ? pn[s] : po[s];
+    phase_deco_cell(i, j, k, nx, ny, nz, po, pn);
   }
 }
 
@@ -326,11 +277,7 @@ __global__ void __launch_bounds__(256) k_solute_boundary(KParams p, const R *__r
   int i, j, k;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
        shell_flat(t, nx, ny, nz, 1, i, j, k); t += (long long)gridDim.x * blockDim.x) {
-    int si = i, sj = j, sk = k;
-    solute_source(si, sj, sk, nx, ny, nz);
-    const bool inner = si > 1 && si < nx && sj > 1 && sj < ny && sk > 1 && sk < nz;
-    const size_t s = cidx(si, sj, sk, nx, ny);
-    cn[cidx(i, j, k, nx, ny)] = inner ? cn[s] : co[s];
+    solute_deco_cell(i, j, k, nx, ny, nz, co, cn);
   }
 }
 

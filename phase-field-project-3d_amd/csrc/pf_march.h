@@ -1,0 +1,441 @@
+// pf_march.h -- the fused z-march sweep body shared by the single-instance
+// kernels (pf_fused.hip) and the batched kernels (pf_batch.hip).
+//
+// A work-group owns a TX x TY column tile (x fastest) and a segment of
+// z-planes and marches up the segment:
+//   - phi planes (tile + 2-cell halo) rotate through 5 LDS slots, loaded once
+//     from HBM with x-contiguous (coalesced) reads;
+//   - the anisotropy quantities of a phase cell's six neighbours
+//     (phase_body.inc:55-129) are the centred-gradient quantities of those
+//     neighbour nodes, so each node's eps^2, exy, eyz, ezx is computed once
+//     (tile + 1-cell ring), shared in-plane through LDS and along z in
+//     registers -- instead of six times per cell;
+//   - sweep B also evaluates the solute (conc_body.inc) with the per-node
+//     1/gamma and mu computed once per node, and the temperature of the next
+//     step (temp_body.inc) from the fresh DPP, which therefore never goes to
+//     HBM except on the last step of a call.
+// Compulsory HBM traffic per cell: sweep A reads phi, c and writes phi;
+// sweep B reads phi, c and writes phi, c, T.
+//
+// Arithmetic follows the reference's formulas with algebraic reuse
+// (reciprocals of dx^2, node reuse, the harmonic mean as 2/(1/g+1/g'), the
+// free-energy ratio with CL = c/den folded in, divisions as a refined
+// reciprocal, the ~1 ulp logarithm of pf_math.h), so results agree with the
+// reference to rounding, not bit for bit; the drop-in kernels keep the
+// bit-faithful form.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pf_common.h"
+#include "pf_math.h"
+
+namespace pf {
+
+constexpr int NSLOT = 5;
+
+// Tile shape: TX x TY cells per work-group (one thread per column).
+template <int TX_, int TY_>
+struct Tile {
+  static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
+  static constexpr int PX = TX + 4, PY = TY + 4, PPL = PX * PY;  // phi plane, halo 2
+  static constexpr int QX = TX + 2, QY = TY + 2, QPL = QX * QY;  // node plane, halo 1
+  static constexpr int NRING = 2 * QX + 2 * TY;                  // ring nodes per plane
+  static constexpr int NLD = (PPL + NT - 1) / NT;                 // phi loads per thread
+  static_assert(NRING <= NT, "one ring node per thread at most");
+  static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
+};
+
+// log and 1/x in the arithmetic type.  FP64: the ~1 ulp forms of
+// pf_math.h.  FP32: the hardware log2 / reciprocal (about 1 ulp).
+template <typename R>
+__device__ __forceinline__ R dlog(R x);
+template <>
+__device__ __forceinline__ double dlog<double>(double x) { return log_fast(x); }
+template <>
+__device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+
+template <typename R>
+__device__ __forceinline__ R rcpn(R x);
+template <>
+__device__ __forceinline__ double rcpn<double>(double x) { return rcp_refined(x); }
+template <>
+__device__ __forceinline__ float rcpn<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Scalars of one sweep, in the arithmetic type.
+template <typename R>
+struct SC {
+  R hx, hy, hz;        // 0.5/dx ...
+  R idx2, idy2, idz2;  // 1/dx^2 ...
+  R cxy, cyz, czx;     // 0.25/(dx dy) ...
+  R E0a, E0b, K;       // eps = E0a + E0b*S4/Q^2, K = 16 E0 delta
+  R small;
+  R Ke, W0, RTV;       // RTV = R*TP/Vm
+  R omCsCl;            // (1-Cs_eq)/(1-Cl_eq)
+  R M0dt, AAA;
+  R Dliq2dt, Dsol2dt;  // 2*D*dt for the solute faces
+  R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
+};
+
+// Computed on the host for a single instance and on the device for batched
+// instances; contraction off so that both give the same bits.
+template <typename R>
+__host__ __device__ inline SC<R> make_sc(const KParams &p) {
+#pragma clang fp contract(off)
+  SC<R> s;
+  s.hx = (R)(0.5 / p.dx); s.hy = (R)(0.5 / p.dy); s.hz = (R)(0.5 / p.dz);
+  s.idx2 = (R)(1.0 / (p.dx * p.dx)); s.idy2 = (R)(1.0 / (p.dy * p.dy));
+  s.idz2 = (R)(1.0 / (p.dz * p.dz));
+  s.cxy = (R)(0.25 / (p.dx * p.dy)); s.cyz = (R)(0.25 / (p.dy * p.dz));
+  s.czx = (R)(0.25 / (p.dz * p.dx));
+  s.E0a = (R)(p.E0 * (1.0 - 3.0 * p.delta_eps));
+  s.E0b = (R)(p.E0 * 4.0 * p.delta_eps);
+  s.K = (R)(16.0 * p.E0 * p.delta_eps);
+  s.small = (R)1.0e-23;
+  s.Ke = (R)p.Ke; s.W0 = (R)p.W0; s.RTV = (R)(p.R * p.TP / p.Vm);
+  s.omCsCl = (R)((1.0 - p.Cs_eq) / (1.0 - p.Cl_eq));
+  s.M0dt = (R)(p.M0 * p.dt); s.AAA = (R)p.AAA;
+  s.Dliq2dt = (R)(2.0 * p.D_liq * p.dt); s.Dsol2dt = (R)(2.0 * p.D_sol * p.dt);
+  s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
+  return s;
+}
+
+template <typename R>
+struct Node {
+  R e2, xy, yz, zx;
+};
+
+// Node quantities from the centred gradient at a node (phase_body.inc:55-129),
+// written with the gradient normalised by Q = |g|^2 + small:
+//   S4/Q^2 = a^2+b^2+c^2 with a = gx^2/Q ...,
+//   gx gy (gx^2-gy^2)/Q^2 = (gx gy/Q)(a-b) ...
+// which is the same quantity but never forms Q^2 (1e-46 for a flat region,
+// below the FP32 range).
+template <typename R>
+__device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
+  const R gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
+  const R rq = rcpn<R>(gx2 + gy2 + gz2 + s.small);
+  const R a = gx2 * rq, b = gy2 * rq, c = gz2 * rq;
+  const R eps = s.E0a + s.E0b * (a * a + b * b + c * c);
+  const R f = s.K * eps;
+  Node<R> v;
+  v.e2 = eps * eps;
+  v.xy = f * (gx * gy * rq) * (a - b);
+  v.yz = f * (gy * gz * rq) * (b - c);
+  v.zx = f * (gz * gx * rq) * (c - a);
+  return v;
+}
+
+template <typename R>
+__device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
+  if (quint) {
+    h = p * p * p * ((R)10 - (R)15 * p + (R)6 * p * p);
+    hl = (R)30 * p * p * ((R)1 - p) * ((R)1 - p);
+  } else {
+    h = p * p * ((R)3 - (R)2 * p);
+    hl = (R)6 * p * ((R)1 - p);
+  }
+}
+
+// Solute node quantities (conc_body.inc:61-118, 139-144): 1/gamma (gamma
+// without the centre's D_C) and q, the node's share of the chemical
+// potential difference taken across each face by dmu():
+//   FP64: q = mu = ln(CL/(1-CL)) = ln(c/(den-c)), dmu = mu_X - mu_P;
+//   FP32: q = CL, dmu = ln((CL_X (1-CL_P)) / (CL_P (1-CL_X)))
+//         = log1p((CL_X - CL_P) / (CL_P (1-CL_X)))   (same quantity).
+// In FP32 the difference of two ~-4 logarithms is noise of about one FP32
+// ulp of mu, which in near-uniform liquid is as large as the true
+// difference and would random-walk c by ~1 ulp per step; the log1p form
+// takes the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
+template <typename R>
+__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &q) {
+  R h, hl;
+  h_of(p, quint, h, hl);
+  const R den = s.Ke * h + ((R)1 - h);
+  const R CL = cc * rcpn<R>(den);
+  const R CS = s.Ke * CL;
+  const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
+  ig = rcpn<R>(g);
+  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc));
+  else q = CL;
+}
+
+template <typename R>
+__device__ __forceinline__ R dmu(R qX, R qP);
+template <>
+__device__ __forceinline__ double dmu<double>(double qX, double qP) { return qX - qP; }
+template <>
+__device__ __forceinline__ float dmu<float>(float qX, float qP) {
+  const float d = (qX - qP) * rcpn<float>(qP * (1.0f - qX));
+  // log1p(d): Taylor series to d^6 for |d| < 1/32 (truncation < 2e-10
+  // relative), else log(1 + d)
+  const float ser =
+      d * (1.0f + d * (-0.5f + d * (1.0f / 3.0f + d * (-0.25f + d * (0.2f + d * (-1.0f / 6.0f))))));
+  return fabsf(d) < 0.03125f ? ser : dlog<float>(1.0f + d);
+}
+
+struct Geo {
+  int nx, ny, nz;
+  int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
+  int ntiles, per_xcd;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// One work-group's tile (bx, by) and z-segment sg of a sweep.
+// SWEEP_B == false: sweep A (phase only).  SWEEP_B == true: phase + solute +
+// (write_T ? next-step T : DPP); if mslot is non-null, the work-group adds
+// the FP64 sum of its cells' solute increments to *mslot (its own slot of
+// the solute-mass ledger, so no atomics and a fixed summation order).
+template <typename R, bool SWEEP_B, typename TL>
+__device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
+                                           const R *__restrict__ po, const R *__restrict__ co,
+                                           R *__restrict__ pn, R *__restrict__ cn,
+                                           const R *__restrict__ Tcur, R *__restrict__ Tnext,
+                                           R *__restrict__ dpp, int write_T,
+                                           double *__restrict__ mslot, int bx, int by, int sg) {
+  // FMAs only within one source expression, decided by the front end, so the
+  // single-instance and batched kernels that inline this body round alike
+#pragma clang fp contract(on)
+  constexpr int TX = TL::TX, NT = TL::NT, PX = TL::PX, PPL = TL::PPL;
+  constexpr int QX = TL::QX, QY = TL::QY, QPL = TL::QPL, NRING = TL::NRING, NLD = TL::NLD;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R *sphi = reinterpret_cast<R *>(smem_raw);  // NSLOT * PPL
+  R *snode = sphi + NSLOT * PPL;              // 4 * QPL
+  R *sgm = snode + 4 * QPL;                   // 2 * QPL (sweep B)
+
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i0 = 2 + bx * TX, j0 = 2 + by * TL::TY;  // first cell of the tile
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = (i <= nx - 1) && (j <= ny - 1);
+  const bool ph_col = own && i >= 3 && i <= nx - 2 && j >= 3 && j <= ny - 2;
+  const int ks = g.k_lo + sg * g.seg;
+  const int ke = min(ks + g.seg - 1, g.k_hi);
+  const size_t sz = (size_t)nx * ny;
+  const int quint = kp.h_quintico;
+  const bool noise = s.AAA != (R)0;
+
+  // ring node handled by this thread (tid < NRING): its node-frame coords
+  int rlx = 0, rly = 0;
+  if (tid < NRING) {
+    if (tid < QX) { rlx = tid; rly = 0; }
+    else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
+    else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
+  }
+  const int olx = tx + 1, oly = ty + 1;  // own node-frame coords
+
+  // In-plane offsets (32 bit; nx*ny < 2^31) of the phi window elements this
+  // thread loads, of its own column and of its ring column; clamped so every
+  // load stays inside the grid (clamped values feed only unused nodes).
+  int loff[NLD];
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int e = tid + u * NT;
+    const int gi = clampi(i0 - 2 + (e % PX), 1, nx), gj = clampi(j0 - 2 + (e / PX), 1, ny);
+    loff[u] = (gi - 1) + (gj - 1) * nx;
+  }
+  const int ooff = (clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx;
+  const int roff = (clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx;
+  auto plane = [&](int q) -> size_t { return (size_t)(clampi(q, 1, nz) - 1) * sz; };
+
+  R pre[NLD];
+  auto gload = [&](int q) {
+    const R *src = po + plane(q);
+#pragma unroll
+    for (int u = 0; u < NLD; ++u)
+      if (tid + u * NT < PPL) pre[u] = src[loff[u]];
+  };
+  auto slot = [&](int q) -> R * { return sphi + (q % NSLOT) * PPL; };  // q >= 0
+  auto lstore = [&](int q) {
+    R *dst = slot(q);
+#pragma unroll
+    for (int u = 0; u < NLD; ++u)
+      if (tid + u * NT < PPL) dst[tid + u * NT] = pre[u];
+  };
+
+  // node at node-frame (lx,ly) of plane q-1 from phi planes q-2, q-1, q
+  auto node_at = [&](int q, int lx, int ly) -> Node<R> {
+    const R *Pm = slot(q - 2), *P0 = slot(q - 1), *Pp = slot(q);
+    const int c = (ly + 1) * PX + (lx + 1);
+    const R gx = (P0[c + 1] - P0[c - 1]) * s.hx;
+    const R gy = (P0[c + PX] - P0[c - PX]) * s.hy;
+    const R gz = (Pp[c] - Pm[c]) * s.hz;
+    return node_from(gx, gy, gz, s);
+  };
+
+  // own-column registers: nodes of planes k-1 (M), k (Z), k+1 (P) and the
+  // solute quantities of planes k-1, k, k+1
+  Node<R> nM{}, nZ{}, nP{};
+  R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
+  R cZ = 0;
+  uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
+  // solute increments of this column for the ledger, summed in the
+  // arithmetic type over the column's z-segment (a few dozen cells) and in
+  // FP64 from there on (FP64 here would cost the FP32 form a wave per SIMD)
+  R macc = (R)0;
+
+  const int q_first = ks - 2, q_last = ke + 2;
+  gload(q_first);
+  for (int q = q_first; q <= q_last; ++q) {
+    lstore(q);
+    if (q < q_last) gload(q + 1);  // prefetch; lands while this plane computes
+    __syncthreads();
+    const int kn = q - 1;  // plane whose node / solute quantities are built now
+    Node<R> nr{};
+    R rig = 0, rmu = 0, cnew_col = 0;
+    if (q >= ks) {
+      if (kn >= 2 && kn <= nz - 1) {  // nodes are only used for planes [2, nz-1]
+        nP = node_at(q, olx, oly);
+        if (tid < NRING) nr = node_at(q, rlx, rly);
+      }
+      if constexpr (SWEEP_B) {
+        if (kn >= 1 && kn <= nz) {
+          const R *P0 = slot(q - 1);
+          const size_t pk = (size_t)(kn - 1) * sz;
+          cnew_col = co[pk + ooff];
+          solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
+          if (tid < NRING)
+            solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu);
+        }
+      }
+    }
+    const int k = q - 2;  // cell plane updated now
+    if (q >= ks + 2 && own) {
+      const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
+      const int c = (ty + 2) * PX + (tx + 2);
+      const R PP = P0[c];
+      const size_t gidx = (size_t)(k - 1) * sz + ooff;
+      R dppv = (R)0, pnew = PP;
+      R cPP;
+      if constexpr (SWEEP_B) cPP = cZ; else cPP = co[gidx];
+      if (ph_col && k >= 3 && k <= nz - 2) {
+        const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
+        const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
+                PSW = P0[c - PX - 1];
+        const R PT = Pp[c], PB = Pm[c];
+        const R PNT = Pp[c + PX], PST = Pp[c - PX], PET = Pp[c + 1], PWT = Pp[c - 1];
+        const R PNB = Pm[c + PX], PSB = Pm[c - PX], PEB = Pm[c + 1], PWB = Pm[c - 1];
+        const int nc = oly * QX + olx;
+        const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
+        const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
+        const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
+        const R yzN = yz[nc + QX], yzS = yz[nc - QX];
+        const R zxE = zx[nc + 1], zxW = zx[nc - 1];
+        const R yzT = nP.yz, yzB = nM.yz, zxT = nP.zx, zxB = nM.zx;
+        R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
+               (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
+               (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
+        const R Cxy = PNE * (xyE - xyN) + PSE * (xyS - xyE) + PNW * (xyN - xyW) +
+                      PSW * (xyW - xyS) + PN * (xyE - xyW) + PS * (xyW - xyE) +
+                      PE * (xyS - xyN) + PW * (xyN - xyS);
+        const R Cyz = PNT * (yzN - yzT) + PNB * (yzB - yzN) + PST * (yzT - yzS) +
+                      PSB * (yzS - yzB) + PT * (yzN - yzS) + PB * (yzS - yzN) +
+                      PN * (yzB - yzT) + PS * (yzT - yzB);
+        const R Czx = PET * (zxT - zxE) + PWT * (zxW - zxT) + PEB * (zxE - zxB) +
+                      PWB * (zxB - zxW) + PE * (zxT - zxB) + PW * (zxB - zxT) +
+                      PT * (zxW - zxE) + PB * (zxE - zxW);
+        AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
+        const R om = (R)1 - PP;
+        const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
+        R h, hl;
+        h_of(PP, quint, h, hl);
+        // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
+        const R den = s.Ke * h + ((R)1 - h);
+        const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
+        const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
+        R ruidos = (R)0;
+        if (noise) {
+          const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
+                                ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                                : (float)hash_unit(hbase);
+          const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
+                                                                : ((R)2 * (R)ran - (R)1) * s.AAA;
+          ruidos = amp * (R)16 * (PP * PP) * (om * om);
+        }
+        const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
+        dppv = pstar - PP;
+        const R cl = pstar > (R)1 ? (R)1 : pstar;
+        pnew = cl < (R)0 ? (R)0 : cl;
+        pn[gidx] = pnew;
+      }
+      if constexpr (SWEEP_B) {
+        // solute at (i,j,k), k in [2,nz-1] (conc_body.inc:92-144): face
+        // mobility 2 D_C/(1/g_P + 1/g_X), flux along mu differences
+        const int nc = oly * QX + olx;
+        const R *ig = sgm, *mu = sgm + QPL;
+        const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
+        const R igc = igZ, muc = muZ;
+        const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
+                     dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
+        const R fy = dmu<R>(mu[nc + QX], muc) * rcpn<R>(ig[nc + QX] + igc) +
+                     dmu<R>(mu[nc - QX], muc) * rcpn<R>(ig[nc - QX] + igc);
+        const R fz = dmu<R>(muP, muc) * rcpn<R>(igP + igc) + dmu<R>(muM, muc) * rcpn<R>(igM + igc);
+        const R inc = D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        cn[gidx] = cPP + inc;
+        macc += inc;
+        if (write_T) {
+          // temp_body.inc:31-41 of the next step: TP + D dt lap(T) + fonte
+          R hh, hl2;
+          h_of(pnew, quint, hh, hl2);
+          R lap = (R)0;
+          if (s.Ddt != (R)0) {
+            const R TP2 = (R)2 * s.TP;
+            lap = (Tcur[gidx + 1] - TP2 + Tcur[gidx - 1]) * s.idx2 +
+                  (Tcur[gidx + nx] - TP2 + Tcur[gidx - nx]) * s.idy2 +
+                  (Tcur[gidx + sz] - TP2 + Tcur[gidx - sz]) * s.idz2;
+          }
+          Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
+        } else if (ph_col && k >= 3 && k <= nz - 2) {
+          dpp[gidx] = dppv;
+        }
+      } else {
+        (void)Tcur; (void)Tnext; (void)dpp; (void)write_T;
+      }
+      hbase += kHashStepK;
+    }
+    __syncthreads();
+    // publish plane kn's node / solute quantities and rotate the column
+    if (q >= ks) {
+      const int oc = oly * QX + olx, rc = rly * QX + rlx;
+      if (kn >= 2 && kn <= nz - 1) {
+        snode[oc] = nP.e2; snode[QPL + oc] = nP.xy; snode[2 * QPL + oc] = nP.yz;
+        snode[3 * QPL + oc] = nP.zx;
+        if (tid < NRING) {
+          snode[rc] = nr.e2; snode[QPL + rc] = nr.xy; snode[2 * QPL + rc] = nr.yz;
+          snode[3 * QPL + rc] = nr.zx;
+        }
+      }
+      if constexpr (SWEEP_B) {
+        sgm[oc] = igP; sgm[QPL + oc] = muP;
+        if (tid < NRING) {
+          sgm[rc] = rig; sgm[QPL + rc] = rmu;
+        }
+      }
+    }
+    nM = nZ; nZ = nP;
+    if constexpr (SWEEP_B) {
+      igM = igZ; muM = muZ; igZ = igP; muZ = muP;
+      cZ = cnew_col;
+    }
+  }
+  if constexpr (SWEEP_B) {
+    if (mslot) {  // uniform over the work-group
+      __shared__ double wsum[NT / 64];
+      double v = (double)macc;
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if ((tid & 63) == 0) wsum[tid >> 6] = v;
+      __syncthreads();
+      if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NT / 64; ++w) t += wsum[w];
+        *mslot += t;
+      }
+    }
+  } else {
+    (void)mslot; (void)macc;
+  }
+}
+
+
+}  // namespace pf

@@ -36,6 +36,16 @@ int fail(int code, const char *fmt, ...) {
 
 }  // namespace
 
+int pf::fail_msg(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
 // One field pair (old/new) with swap-instead-of-copy bookkeeping: `old_idx`
 // names the buffer that holds the reference's *_old array; when `alias` is
 // set the reference's *_new array equals *_old (right after an old<-new copy)
@@ -84,19 +94,7 @@ int check_ctx(const pf_ctx *ctx, const pf_params *p) {
   return PF_OK;
 }
 
-pf::KParams kparams(const pf_params &p) {
-  pf::KParams k;
-  k.nx = p.nx; k.ny = p.ny; k.nz = p.nz; k.it = p.it;
-  k.h_quintico = p.h_quintico; k.ruido_tipo = p.ruido_tipo; k.seed = p.seed;
-  k.dx = p.dx; k.dy = p.dy; k.dz = p.dz;
-  k.dt = p.dt; k.D = p.D; k.delta_H = p.delta_H; k.cp_s = p.cp_s; k.TP = p.TP;
-  k.R = p.R; k.Vm = p.Vm; k.E0 = p.E0; k.W0 = p.W0; k.M0 = p.M0; k.delta_eps = p.delta_eps;
-  k.AAA = p.AAA; k.D_liq = p.D_liq; k.D_sol = p.D_sol;
-  k.Ke = (double)p.Ke;
-  // equacao_campo_fase.f90:21-22: recomputed from TP for every sweep
-  pf_equilibrium(p.TP, p.Tm, p.Ke, p.Me, &k.Cs_eq, &k.Cl_eq);
-  return k;
-}
+pf::KParams kparams(const pf_params &p) { return pf::kparams_from(p); }
 
 // Host FP64 <-> device field (FP64 or FP32).
 int upload(pf_ctx *ctx, void *dst, const double *src) {

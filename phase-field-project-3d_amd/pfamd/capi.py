@@ -73,6 +73,10 @@ EXPORTS = (
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
+    "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
+    "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
+    "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
+    "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_autotune",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass")
@@ -114,6 +118,22 @@ def load(path: str = LIB_PATH):
         "pf_get_tiling": ([vp, vp], C.c_int),
         "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
         "pf_set_mass_correction": ([vp, i32], C.c_int),
+        "pf_batch_create": ([C.c_size_t, C.POINTER(vp)], C.c_int),
+        "pf_batch_destroy": ([vp], C.c_int),
+        "pf_batch_add": ([vp, P, L, i32, vp, vp, vp, C.POINTER(i32)], C.c_int),
+        "pf_batch_remove": ([vp, i32], C.c_int),
+        "pf_batch_run": ([vp, i32], C.c_int),
+        "pf_batch_set_graph_steps": ([vp, i32], C.c_int),
+        "pf_batch_synchronize": ([vp], C.c_int),
+        "pf_batch_status": ([vp, i32, C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "pf_batch_scalars": ([vp, i32, P, L], C.c_int),
+        "pf_batch_download": ([vp, i32, vp, vp, vp], C.c_int),
+        "pf_batch_diagnosticos": ([vp, i32, vp], C.c_int),
+        "pf_batch_bytes": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+        "pf_batch_size_class": ([i32, i32, i32], i32),
+        "pf_batch_set_tiling": ([vp, i32, i32, i32], C.c_int),
+        "pf_batch_get_tiling": ([vp, i32, C.POINTER(i32), C.POINTER(i32)], C.c_int),
+        "pf_batch_autotune": ([vp, i32], C.c_int),
         "pf_mass_ledger": ([vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
@@ -291,6 +311,96 @@ class Context:
         try:
             if getattr(self, "h", None):
                 self.lib.pf_finalize(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent instances advanced together (pf_batch_*)."""
+
+    def __init__(self, hbm_budget: int = 0, lib=None):
+        self.lib = lib or load()
+        h = C.c_void_p()
+        check(self.lib.pf_batch_create(C.c_size_t(hbm_budget), C.byref(h)), self.lib)
+        self.h = h
+
+    def add(self, p, L, nsteps: int, phi, c, T) -> int:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi, c, T)]
+        slot = C.c_int32(-1)
+        check(self.lib.pf_batch_add(self.h, C.byref(p), C.byref(L), int(nsteps),
+                                    *[_ptr(a) for a in arrs], C.byref(slot)), self.lib)
+        return slot.value
+
+    def remove(self, slot: int):
+        check(self.lib.pf_batch_remove(self.h, int(slot)), self.lib)
+
+    def run(self, nsteps: int):
+        check(self.lib.pf_batch_run(self.h, int(nsteps)), self.lib)
+
+    def set_graph_steps(self, k: int):
+        check(self.lib.pf_batch_set_graph_steps(self.h, int(k)), self.lib)
+
+    def synchronize(self):
+        check(self.lib.pf_batch_synchronize(self.h), self.lib)
+
+    def status(self, slot: int):
+        done, left = C.c_int32(0), C.c_int32(0)
+        check(self.lib.pf_batch_status(self.h, int(slot), C.byref(done), C.byref(left)), self.lib)
+        return done.value, left.value
+
+    def scalars(self, slot: int):
+        p, L = PfParams(), PfLoop()
+        check(self.lib.pf_batch_scalars(self.h, int(slot), C.byref(p), C.byref(L)), self.lib)
+        return p, L
+
+    def download(self, slot: int, n: int):
+        phi, c, T = np.empty(n), np.empty(n), np.empty(n)
+        check(self.lib.pf_batch_download(self.h, int(slot), _ptr(phi), _ptr(c), _ptr(T)),
+              self.lib)
+        return phi, c, T
+
+    def diagnosticos(self, slot: int):
+        out = np.zeros(5)
+        check(self.lib.pf_batch_diagnosticos(self.h, int(slot), _ptr(out)), self.lib)
+        return out
+
+    def bytes(self):
+        u, b = C.c_size_t(0), C.c_size_t(0)
+        check(self.lib.pf_batch_bytes(self.h, C.byref(u), C.byref(b)), self.lib)
+        return u.value, b.value
+
+    def size_class(self, precision: int, nx: int, ny: int) -> int:
+        return int(self.lib.pf_batch_size_class(int(precision), int(nx), int(ny)))
+
+    def set_tiling(self, size_class: int, shape: int, target_groups: int):
+        check(self.lib.pf_batch_set_tiling(self.h, int(size_class), int(shape),
+                                           int(target_groups)), self.lib)
+
+    def tiling(self, size_class: int):
+        sh, tg = C.c_int32(0), C.c_int32(0)
+        check(self.lib.pf_batch_get_tiling(self.h, int(size_class), C.byref(sh), C.byref(tg)),
+              self.lib)
+        return sh.value, tg.value
+
+    def autotune(self, reps: int = 2):
+        check(self.lib.pf_batch_autotune(self.h, int(reps)), self.lib)
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.pf_batch_destroy(self.h), self.lib)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.pf_batch_destroy(self.h)
                 self.h = None
         except Exception:
             pass
