@@ -10,8 +10,11 @@ step, cubic h(phi), diagnostics every 1000 steps (inside the timed region).
 One "step" is one reference time step (two phase sweeps, solute,
 temperature, mobility update) over the whole grid with the fields resident
 in HBM.  Prints one JSON line: MLUPS of the FP64 drop-in form (value), the
-FP32-mixed form beside it, the roofline of the dominant kernel from HIP
-events, and the reference's own CPU path timed on this host.
+FP32-mixed form beside it (and its accuracy against FP64 on a small grid),
+the roofline of the dominant kernel from HIP events, the batched ensemble
+(solves/s batched vs one after another), the serving loop (p50/p99 and
+solves/s under a Poisson stream) and the reference's own CPU path timed on
+this host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -191,6 +194,47 @@ def precision_study(checkpoints=(5000, 10000, 20000)):
             "checkpoints": out}
 
 
+def batched_study(steps=5000):
+    """The 64-instance ensemble batched vs one after another (the same
+    problems, single-instance fused loop)."""
+    from pfamd import ensemble
+
+    probs = ensemble.ensemble_problems(64, 7, steps)
+    rb = ensemble.run_batched(probs)
+    log(f"[bench] batched ensemble: {rb.wall_s:.2f} s, {rb.mlups:.0f} MLUPS")
+    rs = ensemble.run_sequential(probs)
+    log(f"[bench] sequential ensemble: {rs.wall_s:.2f} s, {rs.mlups:.0f} MLUPS")
+    return {"config": "64 instances, edges log-uniform 32-160 (z >= x,y), cooling U[5,50] K/s, "
+                      "1-8 bottom seeds r=4, effective noise U[0,0.02], FP32-mixed, "
+                      f"{steps} steps each, seed 7; one hipGraph-replayed launch per sweep",
+            "cells_total": sum(p.cells for p in probs),
+            "solves_per_s": round(len(probs) / rb.wall_s, 3),
+            "sequential_solves_per_s": round(len(probs) / rs.wall_s, 3),
+            "speedup_vs_sequential": round(rs.wall_s / rb.wall_s, 3),
+            "mlups_batched": round(rb.mlups, 1), "mlups_sequential": round(rs.mlups, 1),
+            "wall_s": round(rb.wall_s, 3), "sequential_wall_s": round(rs.wall_s, 3)}
+
+
+def serving_study(n=24, rate=4.0):
+    """Poisson request stream served with continuous batching and served
+    one request at a time (FIFO), same stream."""
+    from pfamd import serve
+
+    reqs = serve.request_stream(n=n, rate=rate, seed=11)
+    rb = serve.serve_batched(reqs)
+    sb = {k: round(v, 3) for k, v in rb.summary().items()}
+    log(f"[bench] serving (batched): {sb}")
+    rs = serve.serve_sequential(reqs)
+    ss = {k: round(v, 3) for k, v in rs.summary().items()}
+    log(f"[bench] serving (sequential): {ss}")
+    return {"config": f"{n} requests, Poisson {rate}/s (seed 11), edges log-uniform 32-256 "
+                      "(z >= x,y), cooling U[5,50] K/s, 1-16 seeds, 1k-20k steps, FP64 or "
+                      "FP32-mixed 50/50; admission by free HBM, 200-step ticks",
+            "cell_steps_total": sum(r.problem.cells * r.problem.steps for r in reqs),
+            "batched": sb, "sequential": ss,
+            "solves_per_s_ratio": round(sb["solves_per_s"] / ss["solves_per_s"], 3)}
+
+
 def cpu_baseline():
     """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
     kernels compiled from source with flang -O2 -fopenmp), timed on this
@@ -262,6 +306,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-precision-study", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling (for profiling runs)")
     a = ap.parse_args(argv)
@@ -326,6 +372,12 @@ def main(argv=None):
     if not a.no_fp32 and not a.no_precision_study:
         log("[bench] precision study (FP32-mixed vs FP64, 64x64x128)")
         out["fp32_mixed"]["precision_study"] = precision_study()
+    if not a.no_batched:
+        log("[bench] batched ensemble (64 instances)")
+        out["batched"] = batched_study()
+    if not a.no_serving:
+        log("[bench] serving (Poisson stream)")
+        out["serving"] = serving_study()
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (reference OpenMP path)")
         cb = cpu_baseline()

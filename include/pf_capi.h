@@ -189,7 +189,7 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps,
 int pf_set_fused(pf_ctx *ctx, int32_t on);
 /* Work-group tiling of the fused sweeps (no counterpart in the reference,
  * whose loops are collapsed over the whole grid, equacao_campo_fase.f90:27).
- * Tile shapes: 0 = 64x8, 1 = 32x8, 2 = 32x16 cells per work-group; the
+ * Tile shapes: 0 = 64x8, 1 = 32x8, 2 = 32x16, 3 = 16x16 cells per work-group; the
  * z-march is split into segments so that a sweep launches about
  * target_groups work-groups.  The tiling never changes results: every cell
  * is computed by the same arithmetic in any tiling. */
@@ -265,15 +265,20 @@ int pf_batch_download(pf_batch *b, int32_t slot, double *phi, double *c, double 
 int pf_batch_diagnosticos(pf_batch *b, int32_t slot, double out[5]);
 int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget);
 /* Size classes: precision x plane-area bucket (<= 48^2, <= 96^2, <= 160^2,
- * larger); each class has a tile shape (0 = 64x8, 1 = 32x8, 2 = 32x16) and a
- * z-split target (work-groups per instance). */
+ * larger); each class has a tile shape (0 = 64x8, 1 = 32x8, 2 = 32x16,
+ * 3 = 16x16; default 16x16). */
 int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny);
-int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape, int32_t target_groups);
-int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape,
-                        int32_t *target_groups);
-/* Times every shape x z-split for each size class present on the batch's own
- * instances (no step is taken) and keeps the fastest.  Synchronous. */
-int pf_batch_autotune(pf_batch *b, int32_t reps);
+int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape);
+int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape);
+/* Work-groups per sweep the z-march split aims for over the whole batch
+ * (default 16384; segments are at least 16 planes): each instance gets its
+ * share by cell count. */
+int pf_batch_set_group_target(pf_batch *b, int32_t target_groups);
+/* Times the whole batch's sweeps with every tile shape for size_class (-1:
+ * each class present in turn) and keeps the fastest (no step is taken; the
+ * T_new reported for the instances is not kept until their next step).
+ * Synchronous. */
+int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
 
 /* ---- measurement ----------------------------------------------------------
  * Per-kernel durations from HIP events recorded around each launch on the

@@ -36,9 +36,10 @@ namespace {
 using Tile64x8 = Tile<64, 8>;
 using Tile32x8 = Tile<32, 8>;
 using Tile32x16 = Tile<32, 16>;
-constexpr int kNShapes = 3;
-constexpr int kShapeX[kNShapes] = {64, 32, 32};
-constexpr int kShapeY[kNShapes] = {8, 8, 16};
+using Tile16x16 = Tile<16, 16>;
+constexpr int kNShapes = 4;
+constexpr int kShapeX[kNShapes] = {64, 32, 32, 16};
+constexpr int kShapeY[kNShapes] = {8, 8, 16, 16};
 
 // Device-resident state of one instance (arithmetic type R).
 template <typename R>
@@ -168,6 +169,7 @@ hipError_t launch_batch_shape(int shape, bool sweep_b, const TileRef *tiles, int
   switch (shape) {
     case 1: return launch_batch_march<R, Tile32x8>(sweep_b, tiles, ntiles, inst, sweep, st);
     case 2: return launch_batch_march<R, Tile32x16>(sweep_b, tiles, ntiles, inst, sweep, st);
+    case 3: return launch_batch_march<R, Tile16x16>(sweep_b, tiles, ntiles, inst, sweep, st);
     default: return launch_batch_march<R, Tile64x8>(sweep_b, tiles, ntiles, inst, sweep, st);
   }
 }
@@ -224,9 +226,11 @@ struct pf_batch {
   void *inst_dev[2] = {nullptr, nullptr};
   int inst_cap[2] = {0, 0};
   std::vector<int> slot_of[2];  // per precision: device index -> slot
-  // tiling per size class: tile shape and segment target
+  // tile shape per size class; work-groups per sweep the z split aims for
+  // over the whole batch (each instance gets its share by cell count)
   int cls_shape[pf::kNClasses];
-  int cls_target[pf::kNClasses];
+  int group_target = 16384;
+  size_t total_cells = 0;
   // tables (rebuilt when instances change)
   bool dirty = true;
   pf::TileRef *tiles_dev = nullptr;
@@ -319,16 +323,23 @@ template <typename R>
 void set_geo(pf::InstDev<R> &I, const pf_batch &b, const pf_batch_slot &s) {
   const int cls = pf::size_class(s.prec, s.nx, s.ny);
   const int shape = b.cls_shape[cls];
-  // z segments: about cls_target work-groups for the instance's plane
+  // z segments: the instance's share (by cells) of group_target work-groups
+  // over the whole batch; a batch of many instances marches whole columns,
+  // which saves the 4 warm-up planes per segment
   const int plane_tiles = ((s.nx - 2 + pf::kShapeX[shape] - 1) / pf::kShapeX[shape]) *
                           ((s.ny - 2 + pf::kShapeY[shape] - 1) / pf::kShapeY[shape]);
-  int nseg = std::max(1, (b.cls_target[cls] + plane_tiles - 1) / plane_tiles);
-  int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
+  const double share = b.total_cells ? (double)s.n / (double)b.total_cells : 1.0;
+  const int want = std::max(1, (int)(share * b.group_target + 0.5));
+  const int nseg = std::max(1, (want + plane_tiles - 1) / plane_tiles);
+  const int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
   I.ga = pf::batch_geo(s.nx, s.ny, s.nz, false, shape, seg);
   I.gb = pf::batch_geo(s.nx, s.ny, s.nz, true, shape, seg);
 }
 
 int rebuild(pf_batch *b) {
+  b->total_cells = 0;
+  for (const auto &s : b->slots)
+    if (s.used) b->total_cells += s.n;
   // tile tables: one group per (precision, shape, sweep)
   std::vector<pf::TileRef> tiles;
   std::vector<pf::Chunk> chunks;
@@ -572,10 +583,7 @@ int pf_batch_create(size_t hbm_budget, pf_batch **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail_msg(PF_E_NODEV, "no HIP device visible");
   pf_batch *b = new pf_batch();
-  for (int c = 0; c < pf::kNClasses; ++c) {
-    b->cls_shape[c] = 1;  // 32x8 until pf_batch_autotune measures
-    b->cls_target[c] = 512;
-  }
+  for (int c = 0; c < pf::kNClasses; ++c) b->cls_shape[c] = 3;  // 16x16 until autotuned
   hipGetDevice(&b->device);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
@@ -890,27 +898,33 @@ int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget) {
   return PF_OK;
 }
 
-int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape, int32_t target_groups) {
+int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
-  if (size_class < 0 || size_class >= pf::kNClasses || shape < 0 || shape >= pf::kNShapes ||
-      target_groups < 1)
-    return fail_msg(PF_E_ARG, "size class in [0,%d), shape in [0,%d), target >= 1",
-                    pf::kNClasses, pf::kNShapes);
+  if (size_class < 0 || size_class >= pf::kNClasses || shape < 0 || shape >= pf::kNShapes)
+    return fail_msg(PF_E_ARG, "size class in [0,%d), shape in [0,%d)", pf::kNClasses,
+                    pf::kNShapes);
   HIP_OK(hipStreamSynchronize(b->stream));
   b->cls_shape[size_class] = shape;
-  b->cls_target[size_class] = target_groups;
   b->dirty = true;
   return PF_OK;
 }
 
-int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape,
-                        int32_t *target_groups) {
+int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
   if (size_class < 0 || size_class >= pf::kNClasses) return fail_msg(PF_E_ARG, "bad size class");
   if (shape) *shape = b->cls_shape[size_class];
-  if (target_groups) *target_groups = b->cls_target[size_class];
+  return PF_OK;
+}
+
+int pf_batch_set_group_target(pf_batch *b, int32_t target_groups) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  if (target_groups < 1) return fail_msg(PF_E_ARG, "target_groups must be >= 1");
+  HIP_OK(hipStreamSynchronize(b->stream));
+  b->group_target = target_groups;
+  b->dirty = true;
   return PF_OK;
 }
 
@@ -918,97 +932,62 @@ int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny) {
   return pf::size_class(precision == PF_FP32_MIXED ? 1 : 0, nx, ny);
 }
 
-// Times each tile shape x segment target for every size class present, on
-// the batch's own instances of that class alone, and keeps the fastest.
-// Both sweeps are timed with the roles of the first phase sweep (read the
-// current phi, c, T; write the step's dead buffers), so the state is
-// unchanged: no step is taken.
-int pf_batch_autotune(pf_batch *b, int32_t reps) {
+// Times each tile shape for the size class (-1: every class present) as
+// the sweeps of the whole batch take with that choice (the other classes
+// keep theirs: how a class's work-groups interleave with the rest matters
+// more than its time alone) and keeps the fastest.  Both sweeps are timed
+// with the roles of the first phase sweep (read the current phi, c, T;
+// write the step's dead buffers), so no step is taken.
+int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
   if (reps < 1) return fail_msg(PF_E_ARG, "reps must be >= 1");
-  static const int targets[] = {128, 512, 2048};
+  if (size_class < -1 || size_class >= pf::kNClasses) return fail_msg(PF_E_ARG, "bad size class");
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  pf::TileRef *tmp_dev = nullptr;
-  size_t tmp_cap = 0;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
   for (int cls = 0; cls < pf::kNClasses && rc == PF_OK; ++cls) {
-    const int prec = cls / 4;
-    std::vector<int> members;  // device indices of this class
-    for (size_t q = 0; q < b->slot_of[prec].size(); ++q) {
-      const pf_batch_slot &s = b->slots[b->slot_of[prec][q]];
-      if (pf::size_class(s.prec, s.nx, s.ny) == cls) members.push_back((int)q);
-    }
-    if (members.empty()) continue;
+    if (size_class >= 0 && cls != size_class) continue;
+    bool present = false;
+    for (const auto &s : b->slots)
+      present = present || (s.used && pf::size_class(s.prec, s.nx, s.ny) == cls);
+    if (!present) continue;
     float best = 1e30f;
-    int bs = b->cls_shape[cls], bt = b->cls_target[cls];
-    for (int shape = 0; shape < pf::kNShapes && rc == PF_OK; ++shape)
-      for (int target : targets) {
-        b->cls_shape[cls] = shape;
-        b->cls_target[cls] = target;
-        if ((rc = rebuild(b)) != PF_OK) break;  // device geometry of the candidate
-        std::vector<pf::TileRef> tl[2];
-        for (int sweep_b = 0; sweep_b < 2; ++sweep_b)
-          for (int q : members) {
-            pf::InstDev<double> g;
-            set_geo(g, *b, b->slots[b->slot_of[prec][q]]);
-            const pf::Geo &gg = sweep_b ? g.gb : g.ga;
-            for (int sg = 0; sg < gg.nseg; ++sg)
-              for (int by = 0; by < gg.nby; ++by)
-                for (int bx = 0; bx < gg.nbx; ++bx) tl[sweep_b].push_back({q, bx, by, sg});
-          }
-        const size_t need = tl[0].size() + tl[1].size();
-        if (need > tmp_cap) {
-          if (tmp_dev) hipFree(tmp_dev);
-          tmp_dev = nullptr;
-          tmp_cap = 0;
-          if (hipMalloc((void **)&tmp_dev, need * sizeof(pf::TileRef)) != hipSuccess) {
-            rc = fail_msg(PF_E_NOMEM, "autotune table");
-            break;
-          }
-          tmp_cap = need;
-        }
-        hipMemcpyAsync(tmp_dev, tl[0].data(), tl[0].size() * sizeof(pf::TileRef),
-                       hipMemcpyHostToDevice, b->stream);
-        hipMemcpyAsync(tmp_dev + tl[0].size(), tl[1].data(), tl[1].size() * sizeof(pf::TileRef),
-                       hipMemcpyHostToDevice, b->stream);
-        hipError_t e = hipSuccess;
-        for (int r = 0; r <= reps && e == hipSuccess; ++r) {
-          if (r == 1) e = hipEventRecord(e0, b->stream);
-          for (int sweep_b = 0; sweep_b < 2 && e == hipSuccess; ++sweep_b) {
-            const pf::TileRef *t = tmp_dev + (sweep_b ? tl[0].size() : 0);
-            const int n = (int)tl[sweep_b].size();
-            if (prec == PF_FP64)
-              e = pf::launch_batch_shape<double>(shape, sweep_b != 0, t, n,
-                                                 (const pf::InstDev<double> *)b->inst_dev[0], 1,
-                                                 b->stream);
-            else
-              e = pf::launch_batch_shape<float>(shape, sweep_b != 0, t, n,
-                                                (const pf::InstDev<float> *)b->inst_dev[1], 1,
-                                                b->stream);
-          }
-        }
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess) {
-          rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(e));
-          break;
-        }
-        ms /= (float)reps;
-        if (ms < best) {
-          best = ms;
-          bs = shape;
-          bt = target;
+    int bs = b->cls_shape[cls];
+    for (int shape = 0; shape < pf::kNShapes && rc == PF_OK; ++shape) {
+      b->cls_shape[cls] = shape;
+      if ((rc = rebuild(b)) != PF_OK) break;
+      hipError_t e = hipSuccess;
+      for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+        if (r == 1) e = hipEventRecord(e0, b->stream);
+        for (const auto &g : b->groups) {
+          if (e != hipSuccess) break;
+          if (g.prec == PF_FP64)
+            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, b->tiles_dev + g.off, g.n,
+                                               (const pf::InstDev<double> *)b->inst_dev[0], 1,
+                                               b->stream);
+          else
+            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, b->tiles_dev + g.off, g.n,
+                                              (const pf::InstDev<float> *)b->inst_dev[1], 1,
+                                              b->stream);
         }
       }
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(e));
+        break;
+      }
+      if (ms < best) {
+        best = ms;
+        bs = shape;
+      }
+    }
     b->cls_shape[cls] = bs;
-    b->cls_target[cls] = bt;
   }
   hipStreamSynchronize(b->stream);
-  if (tmp_dev) hipFree(tmp_dev);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   b->dirty = true;

@@ -18,9 +18,10 @@
 
 namespace pf {
 
-// Tile shapes of the sweep kernels: 0 = 64x8, 1 = 32x8, 2 = 32x16 cells.
-// (Capping registers for a third FP64 wave per SIMD spills and runs slower.)
-constexpr int PF_NVARIANTS = 3;
+// Tile shapes of the sweep kernels: 0 = 64x8, 1 = 32x8, 2 = 32x16, 3 = 16x16
+// cells.  (Capping registers for a third FP64 wave per SIMD spills and runs
+// slower.)
+constexpr int PF_NVARIANTS = 4;
 
 struct FusedState {
   int nx = 0, ny = 0, nz = 0, prec = PF_FP64;

@@ -49,8 +49,9 @@ size_t march_lds(bool sweep_b) {
 using Tile64x8 = Tile<64, 8>;
 using Tile32x8 = Tile<32, 8>;
 using Tile32x16 = Tile<32, 16>;
-constexpr int kTileX[PF_NVARIANTS] = {64, 32, 32};
-constexpr int kTileY[PF_NVARIANTS] = {8, 8, 16};
+using Tile16x16 = Tile<16, 16>;
+constexpr int kTileX[PF_NVARIANTS] = {64, 32, 32, 16};
+constexpr int kTileY[PF_NVARIANTS] = {8, 8, 16, 16};
 
 Geo make_geo(int nx, int ny, int nz, int k_lo, int k_hi, int seg, int tile) {
   Geo g;
@@ -104,6 +105,7 @@ hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp,
   switch (tile) {
     case 1: PF_MARCH(Tile32x8);
     case 2: PF_MARCH(Tile32x16);
+    case 3: PF_MARCH(Tile16x16);
     default: PF_MARCH(Tile64x8);
   }
 #undef PF_MARCH

@@ -76,9 +76,10 @@ EXPORTS = (
     "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
-    "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_autotune",
+    "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_set_group_target",
+    "pf_batch_autotune",
 )
-TILE_SHAPES = ((64, 8), (32, 8), (32, 16))
+TILE_SHAPES = ((64, 8), (32, 8), (32, 16), (16, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass")
 
 
@@ -131,9 +132,10 @@ def load(path: str = LIB_PATH):
         "pf_batch_diagnosticos": ([vp, i32, vp], C.c_int),
         "pf_batch_bytes": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
         "pf_batch_size_class": ([i32, i32, i32], i32),
-        "pf_batch_set_tiling": ([vp, i32, i32, i32], C.c_int),
-        "pf_batch_get_tiling": ([vp, i32, C.POINTER(i32), C.POINTER(i32)], C.c_int),
-        "pf_batch_autotune": ([vp, i32], C.c_int),
+        "pf_batch_set_tiling": ([vp, i32, i32], C.c_int),
+        "pf_batch_get_tiling": ([vp, i32, C.POINTER(i32)], C.c_int),
+        "pf_batch_set_group_target": ([vp, i32], C.c_int),
+        "pf_batch_autotune": ([vp, i32, i32], C.c_int),
         "pf_mass_ledger": ([vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
@@ -373,18 +375,19 @@ class Batch:
     def size_class(self, precision: int, nx: int, ny: int) -> int:
         return int(self.lib.pf_batch_size_class(int(precision), int(nx), int(ny)))
 
-    def set_tiling(self, size_class: int, shape: int, target_groups: int):
-        check(self.lib.pf_batch_set_tiling(self.h, int(size_class), int(shape),
-                                           int(target_groups)), self.lib)
+    def set_tiling(self, size_class: int, shape: int):
+        check(self.lib.pf_batch_set_tiling(self.h, int(size_class), int(shape)), self.lib)
 
-    def tiling(self, size_class: int):
-        sh, tg = C.c_int32(0), C.c_int32(0)
-        check(self.lib.pf_batch_get_tiling(self.h, int(size_class), C.byref(sh), C.byref(tg)),
-              self.lib)
-        return sh.value, tg.value
+    def tiling(self, size_class: int) -> int:
+        sh = C.c_int32(0)
+        check(self.lib.pf_batch_get_tiling(self.h, int(size_class), C.byref(sh)), self.lib)
+        return sh.value
 
-    def autotune(self, reps: int = 2):
-        check(self.lib.pf_batch_autotune(self.h, int(reps)), self.lib)
+    def set_group_target(self, target_groups: int):
+        check(self.lib.pf_batch_set_group_target(self.h, int(target_groups)), self.lib)
+
+    def autotune(self, reps: int = 2, size_class: int = -1):
+        check(self.lib.pf_batch_autotune(self.h, int(size_class), int(reps)), self.lib)
 
     def close(self):
         if getattr(self, "h", None):

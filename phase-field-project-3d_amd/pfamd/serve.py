@@ -1,0 +1,209 @@
+"""Serving: a request loop that admits solidification runs by free HBM and
+advances every admitted run together (continuous batching over pf_batch_*).
+
+A request is a Problem (grid, cooling rate, seeds, step count, precision)
+with an arrival time.  The server loop, one tick at a time:
+  1. admits arrived requests in arrival order while the batch's HBM budget
+     has room for their fields and the cells in flight stay under
+     `max_active_cells` (a request that does not fit waits, and so do the ones
+     behind it: no overtaking).  The cap matters for latency: every admitted
+     run advances by the same number of steps per tick, so beyond the cell
+     count that saturates the GPU more runs in flight only stretch each of
+     them;
+  2. advances every admitted run by up to `tick_steps` steps (one batched
+     launch per sweep for all of them; runs that finish mid-tick stop at their
+     own step count on the device);
+  3. retires the runs that completed: downloads phi/c/T and the diagnostics,
+     frees their memory, and records the completion time.
+Requests that arrive while others are on the GPU are therefore admitted as
+memory frees up and their work overlaps with the runs already in flight.
+The sequential baseline serves the same stream one request at a time
+through the single-instance loop (pf_run_steps).
+
+Time is the host's wall clock; arrivals are replayed in real time (the loop
+sleeps when it has nothing to run and the next arrival is in the future).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+
+from . import capi
+from .ensemble import Problem, draw_problem
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float  # seconds after the stream starts
+    problem: Problem
+
+
+@dataclass
+class Completion:
+    rid: int
+    arrival: float
+    admitted: float
+    done: float
+    fields: Optional[tuple] = None  # (phi, c, T) if kept
+    diag: Optional[np.ndarray] = None
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclass
+class ServeReport:
+    completions: List[Completion]
+    makespan: float
+    busy_ticks: int = 0
+    peak_bytes: int = 0
+    peak_instances: int = 0
+
+    def latencies(self):
+        return np.array([c.latency for c in self.completions])
+
+    def summary(self) -> Dict[str, float]:
+        lat = self.latencies()
+        return {"requests": len(self.completions),
+                "solves_per_s": len(self.completions) / self.makespan,
+                "p50_s": float(np.percentile(lat, 50)),
+                "p99_s": float(np.percentile(lat, 99)),
+                "mean_s": float(lat.mean()),
+                "max_s": float(lat.max()),
+                "makespan_s": self.makespan,
+                "peak_instances": self.peak_instances,
+                "peak_bytes": self.peak_bytes}
+
+
+def request_stream(n: int = 32, rate: float = 2.0, seed: int = 11, steps_lo: int = 1000,
+                   steps_hi: int = 20000, edge_lo: int = 32, edge_hi: int = 256,
+                   max_seeds: int = 16) -> List[Request]:
+    """Poisson arrivals at `rate` per second; grid edges log-uniform in
+    [edge_lo, edge_hi] per axis (z >= x, y), cooling 5-50 K/s, 1-16 seeds,
+    step count uniform in [steps_lo, steps_hi], FP64 or FP32-mixed 50/50."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for q in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        prec = capi.PF_FP64 if rng.random() < 0.5 else capi.PF_FP32_MIXED
+        steps = int(rng.integers(steps_lo, steps_hi + 1))
+        out.append(Request(q, t, draw_problem(rng, edge_lo, edge_hi, max_seeds, prec, steps,
+                                              tag=f"r{q}")))
+    return out
+
+
+def instance_bytes(problem: Problem) -> int:
+    return 6 * problem.cells * (8 if problem.precision == capi.PF_FP64 else 4)
+
+
+def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int = 200,
+                  graph_steps: int = 16, keep: bool = False, autotune: bool = True,
+                  max_active_cells: Optional[int] = None, lib=None,
+                  log: Optional[Callable[[str], None]] = None) -> ServeReport:
+    lib = lib or capi.load()
+    b = capi.Batch(hbm_budget, lib=lib)
+    try:
+        b.set_graph_steps(graph_steps)
+        _, budget = b.bytes()
+        queue = deque(sorted(requests, key=lambda r: r.arrival))
+        active: Dict[int, tuple] = {}  # slot -> (request, admitted time)
+        tuned = set()
+        done: List[Completion] = []
+        rep = ServeReport(done, 0.0)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        while queue or active:
+            # 1. admission by free HBM, in arrival order (inputs are built on
+            # the host at admission, inside the measured latency)
+            while queue and queue[0].arrival <= now():
+                r = queue[0]
+                used, _ = b.bytes()
+                if used + instance_bytes(r.problem) > budget:
+                    if not active:
+                        raise RuntimeError(f"request {r.rid} needs more than the HBM budget")
+                    break
+                in_flight = sum(q.problem.cells for q, _ in active.values())
+                if active and max_active_cells and in_flight + r.problem.cells > max_active_cells:
+                    break
+                s, f = r.problem.setup(lib)
+                slot = b.add(s.p, s.L, r.problem.steps, f[0], f[2], f[4])
+                del f
+                active[slot] = (r, now())
+                queue.popleft()
+                if autotune:
+                    cls = b.size_class(r.problem.precision, r.problem.deck.Nx, r.problem.deck.Ny)
+                    if cls not in tuned:  # first instance of a size class: tune it
+                        b.autotune(reps=1, size_class=cls)
+                        tuned.add(cls)
+            used, _ = b.bytes()
+            rep.peak_bytes = max(rep.peak_bytes, used)
+            rep.peak_instances = max(rep.peak_instances, len(active))
+            if not active:
+                wait = queue[0].arrival - now()
+                if wait > 0:
+                    time.sleep(wait)
+                continue
+            # 2. advance every admitted run; stop the tick at the first completion
+            k = min(tick_steps, min(b.status(sl)[1] for sl in active))
+            b.run(k)
+            b.synchronize()
+            rep.busy_ticks += 1
+            # 3. retire the completed runs
+            for sl in [sl for sl in active if b.status(sl)[1] == 0]:
+                r, adm = active.pop(sl)
+                c = Completion(r.rid, r.arrival, adm, now())
+                if keep:
+                    c.fields = b.download(sl, r.problem.cells)
+                    c.diag = b.diagnosticos(sl)
+                b.remove(sl)
+                done.append(c)
+                if log:
+                    log(f"[serve] request {r.rid} done: latency {c.latency:.2f} s, "
+                        f"{len(active)} running, {len(queue)} waiting")
+        rep.makespan = now()
+        return rep
+    finally:
+        b.close()
+
+
+def serve_sequential(requests: List[Request], keep: bool = False, lib=None,
+                     log: Optional[Callable[[str], None]] = None) -> ServeReport:
+    """The same stream served one request at a time (FIFO) through the
+    single-instance fused loop."""
+    from .run import Simulation
+
+    lib = lib or capi.load()
+    done: List[Completion] = []
+    rep = ServeReport(done, 0.0)
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    for r in sorted(requests, key=lambda r: r.arrival):
+        wait = r.arrival - now()
+        if wait > 0:
+            time.sleep(wait)
+        adm = now()
+        _, f = r.problem.setup(lib)
+        pr = r.problem
+        sim = Simulation(pr.deck, precision=pr.precision, seed=pr.noise_seed, fields=f, lib=lib)
+        if pr.precision == capi.PF_FP32_MIXED:
+            sim.ctx.set_mass_correction(0)  # as in the batched form
+        sim.run(pr.steps)
+        sim.ctx.synchronize()
+        c = Completion(r.rid, r.arrival, adm, now())
+        if keep:
+            g = sim.fields()
+            c.fields = (g["phi_new"], g["c_new"], g["T_new"])
+            c.diag = sim.diagnostics()
+        sim.close()
+        done.append(c)
+        rep.peak_instances = 1
+        if log:
+            log(f"[serve-seq] request {r.rid} done: latency {c.latency:.2f} s")
+    rep.makespan = now()
+    return rep

@@ -110,9 +110,10 @@ def test_batch_tiling_and_budget():
     with capi.Batch(hbm_budget=2 * 48 * n + 100) as b:
         cls = b.size_class(prec, deck.Nx, deck.Ny)
         outs = []
-        for shape, target in ((0, 64), (2, 4096)):
-            b.set_tiling(cls, shape, target)
-            assert b.tiling(cls) == (shape, target)
+        for shape, target in ((0, 64), (3, 4096)):
+            b.set_tiling(cls, shape)
+            b.set_group_target(target)
+            assert b.tiling(cls) == shape
             sl = _add(b, "fp64_seed_noise", 10)
             b.run(10)
             outs.append(_from_batch(b, sl, "fp64_seed_noise"))

@@ -207,7 +207,7 @@ def test_tiling_does_not_change_results(precision):
     deck = _growth_deck(n=(70, 41, 50))
     print(f"[tiling] precision {precision}", flush=True)
     ref = None
-    for tile, target in ((0, 2048), (1, 2048), (2, 256), (1, 1), (2, 4096)):
+    for tile, target in ((0, 2048), (1, 2048), (2, 256), (1, 1), (3, 4096)):
         sim = Simulation(deck, precision=precision)
         sim.ctx.set_tiling(tile, tile, target)
         sim.run(12)
@@ -223,12 +223,12 @@ def test_tiling_does_not_change_results(precision):
     sim.run(5)
     before = sim.fields()
     cands = sim.ctx.autotune(sim.p, reps=2)
-    assert len(cands) == 2 * 3 * 4 and all(c[3] > 0 for c in cands)
+    assert len(cands) == 2 * 4 * 4 and all(c[3] > 0 for c in cands)
     after = sim.fields()
     for nm in FIELDS:
         assert np.array_equal(before[nm], after[nm]), nm
     ta, tb, sa, sb = sim.ctx.tiling()
-    assert 0 <= ta < 3 and 0 <= tb < 3 and sa >= 1 and sb >= 1
+    assert 0 <= ta < 4 and 0 <= tb < 4 and sa >= 1 and sb >= 1
     sim.run(7)
     for nm in ("phi_new", "c_new", "T_new", "dpp"):
         assert np.array_equal(sim.fields()[nm], ref[0][nm]), nm
