@@ -238,12 +238,13 @@ def serving_study(n=24, rate=4.0):
 def cpu_baseline():
     """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
     kernels compiled from source with flang -O2 -fopenmp), timed on this
-    host's cores on 5 steps of the same grid after 1 warm-up step.  Falls
+    host's cores on 20 steps of the same grid after 1 warm-up step (about
+    10 s of CPU work on 16 cores).  Falls
     back to the C oracle (a faithful restatement) when the reference build
     is absent."""
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(16, threads))
-    nwarm, nsteps = 1, 5
+    nwarm, nsteps = 1, 20
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
