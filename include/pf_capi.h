@@ -171,6 +171,16 @@ int pf_download(pf_ctx *ctx, double *phi_new, double *c_new, double *T_new);
 int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old,
                     double *c_new, double *T_old, double *T_new, double *dpp);
 
+/* ---- exact checkpoint ------------------------------------------------------
+ * No counterpart in the reference (its only restart is the 8-digit Tecplot
+ * snapshot read by tipo_inicial = 4, which the driver also supports).  The
+ * file holds the seven fields in their stored precision, the scalars p and
+ * L, the tiling and the solute-mass ledger, so that loading it and running
+ * on gives the same bits as not having stopped.  Both synchronous. */
+int pf_checkpoint_save(pf_ctx *ctx, const pf_params *p, const pf_loop *L, const char *path);
+/* Creates a context on the current device from a checkpoint; fills *p, *L. */
+int pf_checkpoint_load(const char *path, pf_params *p, pf_loop *L, pf_ctx **out);
+
 /* ---- fused fast path ------------------------------------------------------ */
 /* Runs nsteps iterations of the reference's time loop body
  * (principal.f90:96-226 without the output event): temperatura, the TP

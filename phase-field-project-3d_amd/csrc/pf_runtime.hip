@@ -404,6 +404,126 @@ int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old
   return PF_OK;
 }
 
+// ---- binary checkpoint ------------------------------------------------------
+// Layout: CkptHeader, pf_params, pf_loop, then the seven fields in the
+// reference's naming (phi_old, phi_new, c_old, c_new, T_old, T_new, DPP), each
+// n elements of the context's stored type (raw device bytes: exact for FP32).
+namespace {
+struct CkptHeader {
+  char magic[8];  // "PFCKPT01"
+  int32_t version, prec, nx, ny, nz;
+  int32_t params_size, loop_size;
+  int32_t use_fused, tile_a, tile_b, target;
+  int32_t mass_every, ledger_valid;
+  double ledger[3];
+};
+const char kCkptMagic[8] = {'P', 'F', 'C', 'K', 'P', 'T', '0', '1'};
+}  // namespace
+
+int pf_checkpoint_save(pf_ctx *ctx, const pf_params *p, const pf_loop *L, const char *path) {
+  int rc = check_ctx(ctx, p);
+  if (rc == PF_OK && (!p || !L || !path)) rc = fail(PF_E_ARG, "null argument");
+  if (rc != PF_OK) return rc;
+  CkptHeader h;
+  memset(&h, 0, sizeof h);
+  memcpy(h.magic, kCkptMagic, 8);
+  h.version = 1;
+  h.prec = ctx->prec;
+  h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
+  h.params_size = (int32_t)sizeof(pf_params);
+  h.loop_size = (int32_t)sizeof(pf_loop);
+  h.use_fused = ctx->use_fused ? 1 : 0;
+  h.tile_a = ctx->fused.tile_a; h.tile_b = ctx->fused.tile_b; h.target = ctx->fused.target;
+  h.mass_every = ctx->fused.mass_every;
+  h.ledger_valid = ctx->fused.ledger_valid ? 1 : 0;
+  if (h.ledger_valid && pf::fused_mass_read(ctx->fused, h.ledger, ctx->stream) != PF_OK)
+    return fail(PF_E_HIP, "reading the ledger failed");
+  const void *src[7] = {ctx->phi.old_(), ctx->phi.new_(), ctx->c.old_(), ctx->c.new_(),
+                        ctx->T.old_(),   ctx->T.new_(),   ctx->dpp};
+  std::vector<char> buf(ctx->n * ctx->esz);
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(PF_E_ARG, "cannot open %s for writing", path);
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(p, sizeof *p, 1, f) == 1 &&
+            fwrite(L, sizeof *L, 1, f) == 1;
+  for (int q = 0; q < 7 && ok; ++q) {
+    if (hipMemcpyAsync(buf.data(), src[q], buf.size(), hipMemcpyDeviceToHost, ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+      fclose(f);
+      return fail(PF_E_HIP, "checkpoint download failed");
+    }
+    ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  }
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(PF_E_ARG, "writing %s failed", path);
+  return PF_OK;
+}
+
+int pf_checkpoint_load(const char *path, pf_params *p, pf_loop *L, pf_ctx **out) {
+  if (!path || !p || !L || !out) return fail(PF_E_ARG, "null argument");
+  *out = nullptr;
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(PF_E_ARG, "cannot open %s", path);
+  CkptHeader h;
+  pf_params pp;
+  pf_loop ll;
+  bool ok = fread(&h, sizeof h, 1, f) == 1;
+  if (ok && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 1 ||
+             h.params_size != (int32_t)sizeof(pf_params) || h.loop_size != (int32_t)sizeof(pf_loop))) {
+    fclose(f);
+    return fail(PF_E_ARG, "%s is not a checkpoint of this library version", path);
+  }
+  ok = ok && fread(&pp, sizeof pp, 1, f) == 1 && fread(&ll, sizeof ll, 1, f) == 1;
+  if (!ok || pp.nx != h.nx || pp.ny != h.ny || pp.nz != h.nz || pp.precision != h.prec) {
+    fclose(f);
+    return fail(PF_E_ARG, "%s: truncated or inconsistent header", path);
+  }
+  pf_ctx *ctx = nullptr;
+  int rc = pf_init(&pp, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &ctx);
+  if (rc != PF_OK) {
+    fclose(f);
+    return rc;
+  }
+  // pf_init leaves buffer 0 = old, 1 = new of each pair
+  void *dst[7] = {ctx->phi.buf[0], ctx->phi.buf[1], ctx->c.buf[0], ctx->c.buf[1],
+                  ctx->T.buf[0],   ctx->T.buf[1],   ctx->dpp};
+  std::vector<char> buf(ctx->n * ctx->esz);
+  for (int q = 0; q < 7 && rc == PF_OK; ++q) {
+    if (fread(buf.data(), 1, buf.size(), f) != buf.size())
+      rc = fail(PF_E_ARG, "%s: truncated field data", path);
+    else if (hipMemcpyAsync(dst[q], buf.data(), buf.size(), hipMemcpyHostToDevice, ctx->stream) !=
+                 hipSuccess ||
+             hipStreamSynchronize(ctx->stream) != hipSuccess)
+      rc = fail(PF_E_HIP, "checkpoint upload failed");
+  }
+  fclose(f);
+  if (rc == PF_OK) {
+    ctx->use_fused = h.use_fused != 0;
+    if (pf::fused_set_tile(ctx->fused, h.tile_a, h.tile_b, h.target) != PF_OK)
+      rc = fail(PF_E_ARG, "%s: bad tiling", path);
+  }
+  if (rc == PF_OK) {
+    ctx->fused.mass_every = h.mass_every;
+    ctx->fused.ledger_valid = false;
+    if (h.ledger_valid && h.mass_every > 0) {
+      // size the ledger state, then put the saved ledger back
+      if (pf::fused_mass_prepare(ctx->fused, ctx->c.old_(), ctx->stream) != PF_OK ||
+          hipMemcpyAsync(ctx->fused.mass, h.ledger, 3 * sizeof(double), hipMemcpyHostToDevice,
+                         ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(ctx->stream) != hipSuccess)
+        rc = fail(PF_E_HIP, "restoring the solute-mass ledger failed");
+    }
+  }
+  if (rc != PF_OK) {
+    pf_finalize(ctx);
+    return rc;
+  }
+  *p = pp;
+  *L = ll;
+  *out = ctx;
+  return PF_OK;
+}
+
 int pf_synchronize(pf_ctx *ctx) {
   int rc = check_ctx(ctx, nullptr);
   if (rc != PF_OK) return rc;

@@ -73,6 +73,7 @@ EXPORTS = (
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
+    "pf_checkpoint_save", "pf_checkpoint_load",
     "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
@@ -119,6 +120,8 @@ def load(path: str = LIB_PATH):
         "pf_get_tiling": ([vp, vp], C.c_int),
         "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
         "pf_set_mass_correction": ([vp, i32], C.c_int),
+        "pf_checkpoint_save": ([vp, P, L, C.c_char_p], C.c_int),
+        "pf_checkpoint_load": ([C.c_char_p, P, L, C.POINTER(vp)], C.c_int),
         "pf_batch_create": ([C.c_size_t, C.POINTER(vp)], C.c_int),
         "pf_batch_destroy": ([vp], C.c_int),
         "pf_batch_add": ([vp, P, L, i32, vp, vp, vp, C.POINTER(i32)], C.c_int),
@@ -210,6 +213,18 @@ class Context:
         check(self.lib.pf_init(C.byref(p), *[_ptr(a) for a in arrs], C.byref(h)), self.lib)
         self.h = h
 
+    @classmethod
+    def from_checkpoint(cls, path: str, lib=None):
+        """(Context, PfParams, PfLoop) from a pf_checkpoint_save file."""
+        self = cls.__new__(cls)
+        self.lib = lib or load()
+        p, L, h = PfParams(), PfLoop(), C.c_void_p()
+        check(self.lib.pf_checkpoint_load(path.encode(), C.byref(p), C.byref(L), C.byref(h)),
+              self.lib)
+        self.h = h
+        self.n = p.nx * p.ny * p.nz
+        return self, p, L
+
     # -- drop-in calls -----------------------------------------------------
     def temperatura(self, p):
         check(self.lib.pf_temperatura(self.h, C.byref(p)), self.lib)
@@ -278,6 +293,10 @@ class Context:
         check(self.lib.pf_autotune(self.h, C.byref(p), reps, C.byref(n), _ptr(ms)), self.lib)
         return [("AB"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]), float(ms[4 * q + 3]))
                 for q in range(n.value)]
+
+    def checkpoint_save(self, p, L, path: str):
+        check(self.lib.pf_checkpoint_save(self.h, C.byref(p), C.byref(L), path.encode()),
+              self.lib)
 
     def set_mass_correction(self, every: int):
         check(self.lib.pf_set_mass_correction(self.h, int(every)), self.lib)

@@ -113,5 +113,19 @@ class Simulation:
     def fields(self):
         return dict(zip(FIELD_NAMES, self.ctx.download_all()))
 
+    def save(self, path: str):
+        """Exact checkpoint (fields in their stored precision, scalars,
+        tiling, solute-mass ledger)."""
+        self.ctx.checkpoint_save(self.p, self.L, path)
+
+    @classmethod
+    def resume(cls, path: str, deck: Optional[Deck] = None, lib=None) -> "Simulation":
+        """A Simulation continuing from a checkpoint written by save()."""
+        self = cls.__new__(cls)
+        self.lib = lib or capi.load()
+        self.ctx, self.p, self.L = capi.Context.from_checkpoint(path, self.lib)
+        self.deck = deck
+        return self
+
     def close(self):
         self.ctx.close()

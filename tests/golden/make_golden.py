@@ -11,7 +11,8 @@ script drives both on small decks and stores inputs and outputs as .npz:
   unit_*.npz         one call of temperatura / equacao_campo_fase /
                      concentracao_soluto / diagnosticos on random fields
   run_*.npz          the time loop (principal.f90:93-226) for a few steps
-  principal_*.npz    the unmodified principal.f90's Tecplot output event
+  principal_*.npz    the unmodified principal.f90's Tecplot output events
+                     (principal_restart: continued from a restart.dat)
 
 Run here (needs /root/reference and flang):  python tests/golden/make_golden.py
 The fixtures are data only; no reference source is stored.
@@ -247,26 +248,31 @@ PRINCIPAL_DECKS = {
 }
 
 
-def make_principal(name="sphere"):
-    """The unmodified principal.f90 on a small deck: its output events
-    (it = N+1, 2N+1) write Tecplot POINT files with 8 significant digits."""
-    d = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **PRINCIPAL_DECKS[name])
+def _run_principal(d, files=None):
+    """Runs the unmodified principal.f90 on deck d (extra input files in
+    `files`: name -> text) and records its output events (it = N+1, 2N+1,
+    principal.f90:161-187), diagnostics lines and probe prints."""
     wd = tempfile.mkdtemp(prefix="pfprin_")
     try:
         d.write(os.path.join(wd, "entrada.dat"))
+        for fn, text in (files or {}).items():
+            with open(os.path.join(wd, fn), "w") as fh:
+                fh.write(text)
         os.makedirs(os.path.join(wd, "output"))
         env = dict(os.environ, OMP_NUM_THREADS="1")
         r = subprocess.run([PRINCIPAL], cwd=wd, env=env, stdout=subprocess.PIPE,
                            stderr=subprocess.PIPE, text=True)
         if r.returncode != 0:
             raise RuntimeError(r.stderr[-2000:])
-        # output events fire at it = N+1, 2N+1 (principal.f90:161-187)
         files = sorted(os.listdir(os.path.join(wd, "output")))
         assert len(files) == 2, files
         arrays = {"deck": np.array(json.dumps(d.__dict__)), "filenames": np.array(files)}
+        texts = []
         for q, fn in enumerate(files):
             with open(os.path.join(wd, "output", fn)) as fh:
-                lines = fh.read().splitlines()
+                text = fh.read()
+            texts.append(text)
+            lines = text.splitlines()
             arrays[f"e{q}_header"] = np.array(lines[:4])
             arrays[f"e{q}_xyz_phi_tp_c"] = np.array(
                 [[float(x) for x in ln.split()] for ln in lines[4:] if ln.strip()])
@@ -276,9 +282,31 @@ def make_principal(name="sphere"):
         arrays["probe_lines"] = np.array([ln for ln in r.stdout.splitlines()
                                           if len(ln.split()) == 3 and "E" in ln
                                           and "==>" not in ln][:36])
-        save(f"principal_{name}.npz", **arrays)
+        return arrays, texts
     finally:
         shutil.rmtree(wd, ignore_errors=True)
+
+
+def make_principal(name="sphere"):
+    """The unmodified principal.f90 on a small deck: its output events
+    write Tecplot POINT files with 8 significant digits."""
+    d = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **PRINCIPAL_DECKS[name])
+    arrays, _ = _run_principal(d)
+    save(f"principal_{name}.npz", **arrays)
+
+
+def make_restart():
+    """principal.f90 continued from restart.dat (tipo_inicial = 4,
+    entrada.f90:272-321): the planar deck's second output event is the
+    restart file, shifted down by k_desl = R_semente = 2 layers."""
+    base = dict(PRINCIPAL_DECKS["planar"])
+    d0 = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **base)
+    _, texts = _run_principal(d0)
+    base.update(tipo_inicial=4, R_semente=2.0)
+    d1 = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **base)
+    arrays, _ = _run_principal(d1, {"restart.dat": texts[1]})
+    arrays["restart_dat"] = np.array(texts[1])
+    save("principal_restart.npz", **arrays)
 
 
 def main():
@@ -289,6 +317,7 @@ def main():
     make_runs()
     for name in PRINCIPAL_DECKS:
         make_principal(name)
+    make_restart()
 
 
 if __name__ == "__main__":

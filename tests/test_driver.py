@@ -71,3 +71,28 @@ def test_driver_matches_reference_output(tmp_path, fused):
               if len(ln.split()) == 3 and "E" in ln and "==>" not in ln]
     assert probes[:len(z["probe_lines"])] == [str(p) for p in z["probe_lines"]]
     assert os.path.exists(tmp_path / "performance_log.txt")
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not gpu_available(), reason="needs a gfx950 GPU")
+def test_driver_restart_matches_reference(tmp_path):
+    """tipo_inicial = 4 (entrada.f90:272-321): continue from restart.dat, the
+    reference's own output event, with the bottom 2 layers dropped (moving
+    frame); the reference's output events after the restart are the target."""
+    z = load_golden("principal_restart.npz")
+    deck = deck_from_json(z["deck"])
+    print("[driver] restart", flush=True)
+    (tmp_path / "restart.dat").write_text(str(z["restart_dat"]))
+    r = _run(tmp_path, deck)
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "shift = 2 layers" in r.stdout
+    files = sorted(os.listdir(tmp_path / "output"))
+    assert [f.strip() for f in files] == [str(n).strip() for n in z["filenames"]]
+    for q, fn in enumerate(files):
+        lines = (tmp_path / "output" / fn).read_text().splitlines()
+        data = np.array([[float(x) for x in ln.split()] for ln in lines[4:] if ln.strip()])
+        np.testing.assert_allclose(data, z[f"e{q}_xyz_phi_tp_c"], rtol=2e-7, atol=1e-300)
+    diag = [ln for ln in r.stdout.splitlines() if ln.startswith("L2(phi)")]
+    got = np.array([[_parse_fortran_e(t) for t in ln.replace("==>", " ").split()[1::2]]
+                    for ln in diag])
+    np.testing.assert_allclose(got, z["diag"], rtol=2e-6)

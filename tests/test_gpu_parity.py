@@ -331,3 +331,33 @@ def test_api_state_errors():
         ctx.concentracao_soluto(q)
     assert ctx.device_bytes() >= 7 * 8 ** 3 * 8
     ctx.close()
+
+
+@pytest.mark.parametrize("precision", [capi.PF_FP64, capi.PF_FP32_MIXED])
+def test_checkpoint_kill_and_resume(tmp_path, precision):
+    """Stop after 117 steps, write the exact checkpoint, drop the context,
+    resume from the file in a new context and run 183 more: the same bits as
+    the run that did not stop (FP32 crosses mass corrections at 200 and 300,
+    so the ledger is part of the state)."""
+    deck = _growth_deck(n=(40, 36, 52))
+    print(f"[checkpoint] precision {precision}", flush=True)
+    a = Simulation(deck, precision=precision, seed=9)
+    a.run(117)
+    path = str(tmp_path / "state.pfckpt")
+    a.save(path)
+    a.run(183)
+    fa, da, la = a.fields(), a.diagnostics(), a.ctx.mass_ledger()
+    sa = (a.p.it, a.p.TP, a.p.M0, float(a.L.time))
+    a.close()
+    b = Simulation.resume(path)
+    assert b.p.it == 117
+    b.run(183)
+    fb, db, lb = b.fields(), b.diagnostics(), b.ctx.mass_ledger()
+    assert (b.p.it, b.p.TP, b.p.M0, float(b.L.time)) == sa
+    for nm in FIELDS:
+        assert np.array_equal(fa[nm], fb[nm]), nm
+    assert np.array_equal(da, db) and la == lb
+    b.close()
+    with pytest.raises(capi.PfError):
+        (tmp_path / "bad").write_bytes(b"not a checkpoint at all")
+        Simulation.resume(str(tmp_path / "bad"))
