@@ -46,8 +46,9 @@ struct Tile {
   static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
 };
 
-// log and 1/x in the arithmetic type.  FP64: the ~1 ulp forms of
-// pf_math.h.  FP32: the hardware log2 / reciprocal (about 1 ulp).
+// log and 1/x in the arithmetic type.  FP64: the ~1 ulp log and the
+// one-Newton-step reciprocal (<= 11 ulp) of pf_math.h.  FP32: the hardware
+// log2 / reciprocal (about 1 ulp).
 template <typename R>
 __device__ __forceinline__ R dlog(R x);
 template <>
@@ -58,7 +59,7 @@ __device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
 template <typename R>
 __device__ __forceinline__ R rcpn(R x);
 template <>
-__device__ __forceinline__ double rcpn<double>(double x) { return rcp_refined(x); }
+__device__ __forceinline__ double rcpn<double>(double x) { return rcp_fast(x); }
 template <>
 __device__ __forceinline__ float rcpn<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -274,6 +275,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // arithmetic type over the column's z-segment (a few dozen cells) and in
   // FP64 from there on (FP64 here would cost the FP32 form a wave per SIMD)
   R macc = (R)0;
+  R fzu_prev = (R)0;  // solute flux through the up face of the previous cell
 
   const int q_first = ks - 2, q_last = ke + 2;
   gload(q_first);
@@ -346,6 +348,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
         R ruidos = (R)0;
         if (noise) {
+          // the draw depends on the step, not the sweep; passing it from
+          // the first sweep to the last through HBM measured slower than
+          // hashing again (an unprefetched load inside the march)
           const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
                                 ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
                                 : (float)hash_unit(hbase);
@@ -370,7 +375,13 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
                      dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
         const R fy = dmu<R>(mu[nc + QX], muc) * rcpn<R>(ig[nc + QX] + igc) +
                      dmu<R>(mu[nc - QX], muc) * rcpn<R>(ig[nc - QX] + igc);
-        const R fz = dmu<R>(muP, muc) * rcpn<R>(igP + igc) + dmu<R>(muM, muc) * rcpn<R>(igM + igc);
+        // the flux through the face below is the one through the face above
+        // the cell beneath, negated (bit-exact in FP64: x - y = -(y - x) and
+        // a + b = b + a), except for the first cell of the segment
+        const R fzu = dmu<R>(muP, muc) * rcpn<R>(igP + igc);
+        const R fzd = q == ks + 2 ? dmu<R>(muM, muc) * rcpn<R>(igM + igc) : -fzu_prev;
+        fzu_prev = fzu;
+        const R fz = fzu + fzd;
         const R inc = D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
         cn[gidx] = cPP + inc;
         macc += inc;

@@ -30,6 +30,20 @@ __host__ __device__ __forceinline__ double rcp_refined(double x) {
   return r != r ? r0 : r;
 }
 
+// 1/x with one Newton step: measured on gfx950 the estimate is good to about
+// 2^-24 and one step leaves at most 11 ulp (2.4e-15 relative; two steps are
+// correctly rounded).  For the stencil's divisions, whose operands already
+// carry rounding noise far above that.
+__host__ __device__ __forceinline__ double rcp_fast(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r0 = __builtin_amdgcn_rcp(x);
+#else
+  const double r0 = 1.0 / x;
+#endif
+  const double r = fma(r0, fma(-x, r0, 1.0), r0);
+  return r != r ? r0 : r;
+}
+
 // ln 2 split so that e * kLn2Hi is exact for |e| < 2^21 (kLn2Hi keeps 32
 // significant bits; kLn2Lo = ln 2 - kLn2Hi rounded).
 constexpr double kLn2Hi = 0x1.62e42fee00000p-1;
