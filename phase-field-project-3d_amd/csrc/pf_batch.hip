@@ -120,8 +120,7 @@ __global__ void __launch_bounds__(TL::NT)
                             I.T[I.to], I.T[I.to ^ 1], nullptr, 1, nullptr, d.bx, d.by, d.sg);
   else
     march_tile<R, false, TL>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], nullptr,
-                             nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr, d.bx, d.by,
-                             d.sg);
+                             nullptr, nullptr, nullptr, 0, nullptr, d.bx, d.by, d.sg);
 }
 
 // which = 0: phase decoration of sweep `sweep` (shell of width 2);
