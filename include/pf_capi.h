@@ -204,14 +204,21 @@ int pf_set_fused(pf_ctx *ctx, int32_t on);
  * target_groups work-groups.  The tiling never changes results: every cell
  * is computed by the same arithmetic in any tiling. */
 int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_groups);
+/* Sweep B as one fused kernel (on = 0) or split into phase + T and a
+ * solute-only kernel (on = 1, the default until pf_autotune measures) whose
+ * tile shape is tile_s.  Both forms give the same bits. */
+int pf_set_sweep_split(pf_ctx *ctx, int32_t on, int32_t tile_s);
 /* out = {tile of sweep A, tile of sweep B, z-planes per segment A, B}. */
 int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
-/* Times each tile shape x segment split of both sweeps on the context's
- * current fields (reps launches each after one warm-up, writing into scratch
- * buffers it allocates and frees: the state is untouched) and keeps the
- * fastest per sweep.  If ms is non-NULL it receives 4 doubles per candidate
- * {sweep (0 = A, 1 = B), tile, target_groups, average ms}, at most 64
- * candidates, their number in *ncand.  Synchronous. */
+/* Times each tile shape x segment split of sweep A, of the fused sweep B and
+ * of the two kernels of its split form (phase + T, then the solute alone) on
+ * the context's current fields (reps launches each after one warm-up,
+ * writing into scratch buffers it allocates and frees: the state is
+ * untouched), keeps the fastest per kernel and uses the split form of sweep B
+ * when its two kernels together are faster.  If ms is non-NULL it receives 4
+ * doubles per candidate {kind (0 = A, 1 = fused B, 2 = phase + T, 3 =
+ * solute), tile, target_groups, average ms}, at most 64 candidates, their
+ * number in *ncand.  Synchronous. */
 int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms);
 /* Solute-mass correction of the fused path (no counterpart in the
  * reference, which is FP64 throughout).  The context keeps an FP64 ledger of
@@ -295,12 +302,13 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
  * context's stream (no counterpart in the reference, whose only timer is the
  * host wall clock at output events, principal.f90:308-408). */
 #define PF_KT_SWEEP_A 0     /* fused phase sweep (all but the last sweep)   */
-#define PF_KT_SWEEP_B 1     /* fused last sweep + solute + next temperature */
+#define PF_KT_SWEEP_B 1     /* fused last sweep + (solute +) next temperature */
 #define PF_KT_BOUNDARY 2    /* boundary decorations                         */
 #define PF_KT_TEMPERATURA 3 /* standalone temperatura                       */
 #define PF_KT_DROPIN 4      /* drop-in phase / solute kernels               */
 #define PF_KT_MASS 5        /* solute-mass correction (FP32-mixed form)     */
-#define PF_KT_COUNT 6
+#define PF_KT_SOLUTE 6      /* solute kernel of the split form of sweep B   */
+#define PF_KT_COUNT 7
 /* STREAM-style copy of `bytes` bytes (FP64 elements, coalesced, one read
  * and one write per element) repeated `iters` times on the current device;
  * reports the copy bandwidth (read + write bytes / average kernel time).

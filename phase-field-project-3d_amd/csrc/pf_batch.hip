@@ -116,10 +116,10 @@ __global__ void __launch_bounds__(TL::NT)
   if (!I.active) return;
   const int pin = I.po ^ ((sweep - 1) & 1), pout = pin ^ 1;
   if constexpr (SWEEP_B)
-    march_tile<R, true, TL>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], I.c[I.co ^ 1],
+    march_tile<R, kSweepB, TL>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], I.c[I.co ^ 1],
                             I.T[I.to], I.T[I.to ^ 1], nullptr, 1, nullptr, d.bx, d.by, d.sg);
   else
-    march_tile<R, false, TL>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], nullptr,
+    march_tile<R, kSweepA, TL>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout], nullptr,
                              nullptr, nullptr, nullptr, 0, nullptr, d.bx, d.by, d.sg);
 }
 

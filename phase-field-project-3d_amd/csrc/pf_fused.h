@@ -28,6 +28,9 @@ struct FusedState {
   int tile_a = 0, tile_b = 0;  // tile shape of sweep A / sweep B
   int target = 2048;           // work-groups per sweep the z split aims for
   int seg_a = 0, seg_b = 0;    // z-planes per work-group segment
+  // split form of sweep B: phase + T, then the solute in its own kernel
+  bool split_b = false;
+  int tile_s = 0, seg_s = 0;
   bool enabled = false;
   // solute-mass ledger (pf_fused.hip): period of the correction in steps
   // (0 = off), device scalars {ledger, interior sum, factor}, one FP64 slot
@@ -74,13 +77,17 @@ struct FusedStep {
 int fused_init(FusedState &s, int nx, int ny, int nz, int prec);
 // Tile shape per sweep and the work-group count the z split aims for.
 int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups);
-// Times every tile shape x z-split candidate of both sweeps on the given
-// fields (writing into scratch buffers of its own) and keeps the fastest.
+// Split form of sweep B on/off and the solute kernel's tile shape.
+int fused_set_split(FusedState &s, bool on, int tile_s);
+// Times every tile shape x z-split candidate of sweep A, the fused sweep B
+// and the two kernels of its split form on the given fields (writing into
+// scratch buffers of its own) and keeps the fastest; the split form is
+// chosen when its two kernels together beat the fused sweep.
 constexpr int PF_AUTOTUNE_MAX = 64;
 struct AutotuneResult {
   int n = 0;
   struct { int sweep, tile, target; float ms; } cand[PF_AUTOTUNE_MAX];
-  float best_ms[2] = {0, 0};
+  float best_ms[4] = {0, 0, 0, 0};  // sweep A, fused B, phase + T, solute
 };
 int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, void *T_old,
                    int reps, hipStream_t stream, AutotuneResult *res);

@@ -29,7 +29,7 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
 
 // FP32 sweeps fit four waves per SIMD (<= 128 VGPRs); the hint keeps the
 // compiler there.  FP64 sweeps need more registers than three waves allow.
-template <typename R, bool SWEEP_B, typename TL>
+template <typename R, int MODE, typename TL>
 __global__ void __launch_bounds__(TL::NT)
     __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
@@ -37,13 +37,18 @@ __global__ void __launch_bounds__(TL::NT)
             R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot) {
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
-  march_tile<R, SWEEP_B, TL>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
-                             mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
+  march_tile<R, MODE, TL>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
+                          mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
 }
 
+// The solute update alone (split form of sweep B).
 template <typename R, typename TL>
-size_t march_lds(bool sweep_b) {
-  return sizeof(R) * (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
+__global__ void __launch_bounds__(TL::NT)
+    k_solute(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
+             R *__restrict__ cn, double *__restrict__ mslot) {
+  int bx, by, sg;
+  if (!tile_of(g, bx, by, sg)) return;
+  solute_tile<R, TL>(g, kp, s, po, co, cn, mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
 }
 
 using Tile64x8 = Tile<64, 8>;
@@ -80,28 +85,30 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
 }
 
 template <typename R, typename TL>
-hipError_t launch_march_t(bool sweep_b, const Geo &g, const KParams &kp, const SC<R> &s,
+hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
                           int write_T, double *mslot, hipStream_t st) {
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
-  const size_t lds = march_lds<R, TL>(sweep_b);
-  if (sweep_b)
-    hipLaunchKernelGGL((k_march<R, true, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
-                       Tnext, dpp, write_T, mslot);
+  const size_t lds = march_lds_bytes<R, TL>(mode);
+  if (mode == kSweepB)
+    hipLaunchKernelGGL((k_march<R, kSweepB, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
+                       Tcur, Tnext, dpp, write_T, mslot);
+  else if (mode == kSweepPT)
+    hipLaunchKernelGGL((k_march<R, kSweepPT, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
+                       Tcur, Tnext, dpp, write_T, nullptr);
   else
-    hipLaunchKernelGGL((k_march<R, false, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn, Tcur,
-                       Tnext, dpp, write_T, nullptr);
+    hipLaunchKernelGGL((k_march<R, kSweepA, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
+                       Tcur, Tnext, dpp, write_T, nullptr);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp, const R *po,
+hipError_t launch_march(int mode, int tile, const Geo &g, const KParams &kp, const R *po,
                         const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
                         double *mslot, hipStream_t st) {
   const SC<R> s = make_sc<R>(kp);
 #define PF_MARCH(TL) \
-  return launch_march_t<R, TL>(sweep_b, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, \
-                               mslot, st)
+  return launch_march_t<R, TL>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, st)
   switch (tile) {
     case 1: PF_MARCH(Tile32x8);
     case 2: PF_MARCH(Tile32x16);
@@ -111,10 +118,39 @@ hipError_t launch_march(bool sweep_b, int tile, const Geo &g, const KParams &kp,
 #undef PF_MARCH
 }
 
-// The ledger is kept while it is valid and has a slot for every sweep-B
-// work-group (fused_mass_prepare sizes the slots before a run).
-bool mass_on(const FusedState &S, const Geo &gb) {
-  return S.mass_every > 0 && S.ledger_valid && S.mslot && 8 * gb.per_xcd <= S.nslot;
+template <typename R, typename TL>
+hipError_t launch_solute_t(const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
+                           const R *co, R *cn, double *mslot, hipStream_t st) {
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
+  const size_t lds = solute_lds_bytes<TL, R>();
+  hipLaunchKernelGGL((k_solute<R, TL>), grd, blk, lds, st, g, kp, s, po, co, cn, mslot);
+  return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_solute_split(int tile, const Geo &g, const KParams &kp, const R *po, const R *co,
+                               R *cn, double *mslot, hipStream_t st) {
+  const SC<R> s = make_sc<R>(kp);
+  switch (tile) {
+    case 1: return launch_solute_t<R, Tile32x8>(g, kp, s, po, co, cn, mslot, st);
+    case 2: return launch_solute_t<R, Tile32x16>(g, kp, s, po, co, cn, mslot, st);
+    case 3: return launch_solute_t<R, Tile16x16>(g, kp, s, po, co, cn, mslot, st);
+    default: return launch_solute_t<R, Tile64x8>(g, kp, s, po, co, cn, mslot, st);
+  }
+}
+
+// The ledger is kept while it is valid and has a slot for every work-group
+// of the launch that updates the solute (fused_mass_prepare sizes the slots
+// before a run).
+bool mass_on(const FusedState &S, const Geo &g) {
+  return S.mass_every > 0 && S.ledger_valid && S.mslot && 8 * g.per_xcd <= S.nslot;
+}
+
+// Geometry of the launch that updates the solute: sweep B, or the solute
+// kernel of the split form.
+Geo solute_geo(const FusedState &S) {
+  return S.split_b ? make_geo(S.nx, S.ny, S.nz, 2, S.nz - 1, S.seg_s, S.tile_s)
+                   : make_geo(S.nx, S.ny, S.nz, 2, S.nz - 1, S.seg_b, S.tile_b);
 }
 
 template <typename R>
@@ -148,8 +184,8 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
     R *pn = (R *)phi[pnw];
     if (sw < fs.nv) {
       tm.begin(PF_KT_SWEEP_A, st);
-      e = launch_march<R>(false, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr, nullptr,
-                          nullptr, nullptr, 0, nullptr, st);
+      e = launch_march<R>(kSweepA, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr,
+                          nullptr, nullptr, nullptr, 0, nullptr, st);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
       tm.begin(PF_KT_BOUNDARY, st);
@@ -161,12 +197,28 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
       // T of the next step goes into the buffer of this step's TOLD (dead now)
       const int t_cur = *T_alias ? *T_old : 1 - *T_old;
       const int t_nxt = 1 - t_cur;
-      tm.begin(PF_KT_SWEEP_B, st);
-      e = launch_march<R>(true, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
-                          (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1,
-                          mass_on(S, gb) ? S.mslot : nullptr, st);
-      tm.end(st);
-      if (e != hipSuccess) return PF_E_HIP;
+      if (S.split_b) {
+        // phase + T, then the solute alone (both read the same phi and c)
+        tm.begin(PF_KT_SWEEP_B, st);
+        e = launch_march<R>(kSweepPT, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, nullptr,
+                            (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1,
+                            nullptr, st);
+        tm.end(st);
+        if (e != hipSuccess) return PF_E_HIP;
+        const Geo gs = solute_geo(S);
+        tm.begin(PF_KT_SOLUTE, st);
+        e = launch_solute_split<R>(S.tile_s, gs, k, po, (const R *)c[*c_old], (R *)c[cnw],
+                                   mass_on(S, gs) ? S.mslot : nullptr, st);
+        tm.end(st);
+        if (e != hipSuccess) return PF_E_HIP;
+      } else {
+        tm.begin(PF_KT_SWEEP_B, st);
+        e = launch_march<R>(kSweepB, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
+                            (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1,
+                            mass_on(S, gb) ? S.mslot : nullptr, st);
+        tm.end(st);
+        if (e != hipSuccess) return PF_E_HIP;
+      }
       tm.begin(PF_KT_BOUNDARY, st);
       e = launch_phase_boundary<R>(k, po, pn, st);
       if (e == hipSuccess) e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st);
@@ -201,6 +253,7 @@ int fused_init(FusedState &s, int nx, int ny, int nz, int prec) {
   // both precisions on 256^2 x 528 (32x8 for the phase-only sweep, 32x16
   // for the phase + solute sweep, about 1024 work-groups)
   fused_set_tile(s, 1, 2, 1024);
+  fused_set_split(s, true, 0);
   s.mass_every = prec == PF_FP32_MIXED ? 100 : 0;
   return PF_OK;
 }
@@ -214,6 +267,15 @@ int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups) {
   s.target = target_groups;
   s.seg_a = pick_seg(s.nx, s.ny, s.nz - 4, tile_a, target_groups);
   s.seg_b = pick_seg(s.nx, s.ny, s.nz - 2, tile_b, target_groups);
+  s.seg_s = pick_seg(s.nx, s.ny, s.nz - 2, s.tile_s, target_groups);
+  return PF_OK;
+}
+
+int fused_set_split(FusedState &s, bool on, int tile_s) {
+  if (tile_s < 0 || tile_s >= PF_NVARIANTS) return PF_E_ARG;
+  s.split_b = on;
+  s.tile_s = tile_s;
+  s.seg_s = pick_seg(s.nx, s.ny, s.nz - 2, tile_s, s.target);
   return PF_OK;
 }
 
@@ -231,19 +293,26 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PF_E_HIP;
   static const int targets[] = {1024, 2048, 4096, 8192};
   const R *po = (const R *)phi_old, *co = (const R *)c_old, *Tc = (const R *)T_old;
-  float best[2] = {1e30f, 1e30f};
+  // kinds: 0 sweep A, 1 fused sweep B, 2 phase + T of the split form,
+  // 3 solute kernel of the split form
+  float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+  int btile[4] = {S.tile_a, S.tile_b, S.tile_b, S.tile_s}, bseg[4] = {S.seg_a, S.seg_b, S.seg_b,
+                                                                         S.seg_s};
   int cand = 0;
-  for (int sweep_b = 0; sweep_b < 2 && rc == PF_OK; ++sweep_b) {
+  for (int kind = 0; kind < 4 && rc == PF_OK; ++kind) {
     for (int tile = 0; tile < PF_NVARIANTS && rc == PF_OK; ++tile) {
       for (int target : targets) {
-        const int seg = pick_seg(nx, ny, sweep_b ? nz - 2 : nz - 4, tile, target);
-        const Geo g = sweep_b ? make_geo(nx, ny, nz, 2, nz - 1, seg, tile)
-                              : make_geo(nx, ny, nz, 3, nz - 2, seg, tile);
+        const int seg = pick_seg(nx, ny, kind ? nz - 2 : nz - 4, tile, target);
+        const Geo g = kind ? make_geo(nx, ny, nz, 2, nz - 1, seg, tile)
+                           : make_geo(nx, ny, nz, 3, nz - 2, seg, tile);
         hipError_t e = hipSuccess;
         for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is the warm-up
           if (r == 1) hipEventRecord(e0, st);
-          e = launch_march<R>(sweep_b != 0, tile, g, k, po, co, out, out + n, Tc, out + 2 * n,
-                              nullptr, 1, nullptr, st);
+          if (kind == 3)
+            e = launch_solute_split<R>(tile, g, k, po, co, out + n, nullptr, st);
+          else
+            e = launch_march<R>(kind == 0 ? kSweepA : kind == 1 ? kSweepB : kSweepPT, tile, g, k,
+                                po, co, out, out + n, Tc, out + 2 * n, nullptr, 1, nullptr, st);
         }
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventRecord(e1, st);
@@ -252,24 +321,32 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
         if (e != hipSuccess) { rc = PF_E_HIP; break; }
         ms /= (float)reps;
         if (res && cand < PF_AUTOTUNE_MAX) {
-          res->cand[cand].sweep = sweep_b;
+          res->cand[cand].sweep = kind;
           res->cand[cand].tile = tile;
           res->cand[cand].target = target;
           res->cand[cand].ms = ms;
         }
         ++cand;
-        if (ms < best[sweep_b]) {
-          best[sweep_b] = ms;
-          if (sweep_b) { S.tile_b = tile; S.seg_b = seg; }
-          else { S.tile_a = tile; S.seg_a = seg; }
+        if (ms < best[kind]) {
+          best[kind] = ms;
+          btile[kind] = tile;
+          bseg[kind] = seg;
         }
       }
     }
   }
+  if (rc == PF_OK) {
+    S.tile_a = btile[0];
+    S.seg_a = bseg[0];
+    S.split_b = best[2] + best[3] < best[1];
+    S.tile_b = S.split_b ? btile[2] : btile[1];
+    S.seg_b = S.split_b ? bseg[2] : bseg[1];
+    S.tile_s = btile[3];
+    S.seg_s = bseg[3];
+  }
   if (res) {
     res->n = cand < PF_AUTOTUNE_MAX ? cand : PF_AUTOTUNE_MAX;
-    res->best_ms[0] = best[0];
-    res->best_ms[1] = best[1];
+    for (int q = 0; q < 4; ++q) res->best_ms[q] = best[q];
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
@@ -410,8 +487,8 @@ hipError_t mass_interior(const FusedState &S, const void *c, hipStream_t st) {
 
 int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
   if (S.mass_every <= 0 || !S.enabled) return PF_OK;
-  const Geo gb = make_geo(S.nx, S.ny, S.nz, 2, S.nz - 1, S.seg_b, S.tile_b);
-  const int need = 8 * gb.per_xcd;
+  const Geo gs = solute_geo(S);
+  const int need = 8 * gs.per_xcd;
   if (!S.mass) {
     if (hipMalloc((void **)&S.mass, 4 * sizeof(double)) != hipSuccess) return PF_E_NOMEM;
     if (hipMalloc((void **)&S.mpart, kMassBlocks * sizeof(double)) != hipSuccess)

@@ -142,14 +142,15 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
 // without the centre's D_C) and q, the node's share of the chemical
 // potential difference taken across each face by dmu():
 //   FP64: q = mu = ln(CL/(1-CL)) = ln(c/(den-c)), dmu = mu_X - mu_P;
-//   FP32: q = CL, dmu = ln((CL_X (1-CL_P)) / (CL_P (1-CL_X)))
-//         = log1p((CL_X - CL_P) / (CL_P (1-CL_X)))   (same quantity).
+//   FP32: q = CL, dmu = ln((CL_X (1-CL_P)) / (CL_P (1-CL_X))) (same quantity)
+//         from the exact difference CL_X - CL_P (see dmu<float>).
 // In FP32 the difference of two ~-4 logarithms is noise of about one FP32
 // ulp of mu, which in near-uniform liquid is as large as the true
-// difference and would random-walk c by ~1 ulp per step; the log1p form
-// takes the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
+// difference and would random-walk c by ~1 ulp per step; the FP32 form takes
+// the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
 template <typename R>
 __device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &q) {
+#pragma clang fp contract(off)
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
@@ -167,12 +168,49 @@ template <>
 __device__ __forceinline__ double dmu<double>(double qX, double qP) { return qX - qP; }
 template <>
 __device__ __forceinline__ float dmu<float>(float qX, float qP) {
-  const float d = (qX - qP) * rcpn<float>(qP * (1.0f - qX));
-  // log1p(d): Taylor series to d^6 for |d| < 1/32 (truncation < 2e-10
-  // relative), else log(1 + d)
+#pragma clang fp contract(off)
+  // ln r, r = CL_X (1-CL_P) / (CL_P (1-CL_X)), as 2 atanh(t) with
+  // t = (r-1)/(r+1) = (CL_X - CL_P) / ((CL_X + CL_P) - 2 CL_X CL_P).  The
+  // denominator is symmetric as computed and the numerator flips sign
+  // exactly, so t(X,P) = -t(P,X); the odd series and the log difference
+  // below keep dmu(X,P) = -dmu(P,X) bit for bit: a face's flux is the same
+  // from both sides (and carrying it down the column changes nothing).
+  const float t = (qX - qP) * rcpn<float>((qX + qP) - 2.0f * (qX * qP));
+  const float a = fabsf(t), t2 = t * t;
+  // 2 atanh(t) = 2t (1 + t^2/3 + t^4/5 + ...): truncation < 1e-10 relative for |t| < 1/8
   const float ser =
-      d * (1.0f + d * (-0.5f + d * (1.0f / 3.0f + d * (-0.25f + d * (0.2f + d * (-1.0f / 6.0f))))));
-  return fabsf(d) < 0.03125f ? ser : dlog<float>(1.0f + d);
+      2.0f * t * (1.0f + t2 * (1.0f / 3.0f + t2 * (0.2f + t2 * (1.0f / 7.0f + t2 * (1.0f / 9.0f)))));
+  // large jumps of CL: ln 2 (log2(CL_X (1-CL_P)) - log2(CL_P (1-CL_X))), the
+  // same two products in swapped roles, so also exactly odd (and ln r >= 0.25
+  // here, where the difference of logarithms loses nothing that matters).
+  // The base-2 logarithms are subtracted before the one scaling by ln 2: with
+  // two natural logarithms the compiler may fuse one scaling into the
+  // subtraction (an FMA), which breaks the exact sign flip.
+  const float lg = (__builtin_amdgcn_logf(qX * (1.0f - qP)) -
+                    __builtin_amdgcn_logf(qP * (1.0f - qX))) * 0.693147180559945309f;
+  return a < 0.125f ? ser : lg;
+}
+
+// Solute increment of the cell at LDS index nc (conc_body.inc:92-144): face
+// mobility 2 D_C/(1/g_P + 1/g_X), flux along the q differences.  The flux
+// through the face below is the one through the face above the cell beneath,
+// negated (exact: dmu is odd and a + b = b + a), except for the first cell
+// of a z-segment (fresh).  Contraction is off so that the kernels that inline
+// this in different surroundings round it the same way.
+template <typename R>
+__device__ __forceinline__ R solute_inc(const R *ig, const R *mu, int nc, int qx, R igc, R muc,
+                                        R igP, R muP, R igM, R muM, bool fresh, R &fzu_prev,
+                                        R D2dt, const SC<R> &s) {
+#pragma clang fp contract(off)
+  const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
+               dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
+  const R fy = dmu<R>(mu[nc + qx], muc) * rcpn<R>(ig[nc + qx] + igc) +
+               dmu<R>(mu[nc - qx], muc) * rcpn<R>(ig[nc - qx] + igc);
+  const R fzu = dmu<R>(muP, muc) * rcpn<R>(igP + igc);
+  const R fzd = fresh ? dmu<R>(muM, muc) * rcpn<R>(igM + igc) : -fzu_prev;
+  fzu_prev = fzu;
+  const R fz = fzu + fzd;
+  return D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
 }
 
 struct Geo {
@@ -183,12 +221,18 @@ struct Geo {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// One work-group's tile (bx, by) and z-segment sg of a sweep.
-// SWEEP_B == false: sweep A (phase only).  SWEEP_B == true: phase + solute +
-// (write_T ? next-step T : DPP); if mslot is non-null, the work-group adds
-// the FP64 sum of its cells' solute increments to *mslot (its own slot of
-// the solute-mass ledger, so no atomics and a fixed summation order).
-template <typename R, bool SWEEP_B, typename TL>
+// Sweep kinds of march_tile.
+enum : int {
+  kSweepA = 0,   // phase sweep only
+  kSweepB = 1,   // phase + solute + (write_T ? next-step T : DPP)
+  kSweepPT = 2,  // phase + (write_T ? next-step T : DPP); the solute runs in solute_tile
+};
+
+// One work-group's tile (bx, by) and z-segment sg of a sweep of kind MODE.
+// With the solute (kSweepB) and mslot non-null, the work-group adds the FP64
+// sum of its cells' solute increments to *mslot (its own slot of the
+// solute-mass ledger, so no atomics and a fixed summation order).
+template <typename R, int MODE, typename TL>
 __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
                                            const R *__restrict__ po, const R *__restrict__ co,
                                            R *__restrict__ pn, R *__restrict__ cn,
@@ -198,6 +242,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // FMAs only within one source expression, decided by the front end, so the
   // single-instance and batched kernels that inline this body round alike
 #pragma clang fp contract(on)
+  constexpr bool SWEEP_B = MODE == kSweepB;  // with the solute
+  constexpr bool T_OUT = MODE != kSweepA;     // writes T (or DPP)
   constexpr int TX = TL::TX, NT = TL::NT, PX = TL::PX, PPL = TL::PPL;
   constexpr int QX = TL::QX, QY = TL::QY, QPL = TL::QPL, NRING = TL::NRING, NLD = TL::NLD;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -365,26 +411,17 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         pn[gidx] = pnew;
       }
       if constexpr (SWEEP_B) {
-        // solute at (i,j,k), k in [2,nz-1] (conc_body.inc:92-144): face
-        // mobility 2 D_C/(1/g_P + 1/g_X), flux along mu differences
+        // solute at (i,j,k), k in [2,nz-1]
         const int nc = oly * QX + olx;
         const R *ig = sgm, *mu = sgm + QPL;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R igc = igZ, muc = muZ;
-        const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
-                     dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
-        const R fy = dmu<R>(mu[nc + QX], muc) * rcpn<R>(ig[nc + QX] + igc) +
-                     dmu<R>(mu[nc - QX], muc) * rcpn<R>(ig[nc - QX] + igc);
-        // the flux through the face below is the one through the face above
-        // the cell beneath, negated (bit-exact in FP64: x - y = -(y - x) and
-        // a + b = b + a), except for the first cell of the segment
-        const R fzu = dmu<R>(muP, muc) * rcpn<R>(igP + igc);
-        const R fzd = q == ks + 2 ? dmu<R>(muM, muc) * rcpn<R>(igM + igc) : -fzu_prev;
-        fzu_prev = fzu;
-        const R fz = fzu + fzd;
-        const R inc = D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+        const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 2,
+                                    fzu_prev, D2dt, s);
         cn[gidx] = cPP + inc;
         macc += inc;
+      }
+      if constexpr (T_OUT) {
         if (write_T) {
           // temp_body.inc:31-41 of the next step: TP + D dt lap(T) + fonte
           R hh, hl2;
@@ -444,8 +481,120 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       }
     }
   } else {
-    (void)mslot; (void)macc;
+    (void)mslot; (void)macc; (void)cn;
   }
+}
+
+// LDS bytes of a march_tile work-group.
+template <typename R, typename TL>
+constexpr size_t march_lds_bytes(int mode) {
+  return sizeof(R) *
+         (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (mode == kSweepB ? 2 * TL::QPL : 0));
+}
+
+// One work-group's tile and z-segment of the solute update alone
+// (conc_body.inc:61-144), for the split form of sweep B: the same per-node
+// 1/gamma and mu and the same face fluxes as march_tile's kSweepB, in the
+// same expressions (same bits), but it needs only phi and c at the nodes --
+// no phi planes, gradients or anisotropy -- so it runs with few registers
+// and many waves.  Node planes go through three LDS buffers (plane q is
+// published before the barrier of iteration q and read after the barrier of
+// iteration q+1; its buffer is rewritten in iteration q+3, after a barrier
+// all its readers have passed), so one barrier per plane suffices.
+template <typename R, typename TL>
+__device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, const SC<R> &s,
+                                            const R *__restrict__ po, const R *__restrict__ co,
+                                            R *__restrict__ cn, double *__restrict__ mslot,
+                                            int bx, int by, int sg) {
+#pragma clang fp contract(on)
+  constexpr int TX = TL::TX, NT = TL::NT, QX = TL::QX, QY = TL::QY, QPL = TL::QPL;
+  constexpr int NRING = TL::NRING;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  R *sgm = reinterpret_cast<R *>(smem_raw);  // 3 x {1/gamma, mu} x QPL
+
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
+  const int i0 = 2 + bx * TX, j0 = 2 + by * TL::TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool own = (i <= nx - 1) && (j <= ny - 1);
+  const int ks = g.k_lo + sg * g.seg;
+  const int ke = min(ks + g.seg - 1, g.k_hi);
+  const size_t sz = (size_t)nx * ny;
+  const int quint = kp.h_quintico;
+  int rlx = 0, rly = 0;
+  if (tid < NRING) {
+    if (tid < QX) { rlx = tid; rly = 0; }
+    else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
+    else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
+  }
+  const int olx = tx + 1, oly = ty + 1;
+  const int ooff = (clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx;
+  const int roff = (clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx;
+  const int oc = oly * QX + olx, rc = rly * QX + rlx;
+
+  R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
+  R pZ = 0, cZ = 0;  // phi and c of the own cell of plane k
+  R macc = (R)0, fzu_prev = (R)0;
+  // node plane q (all in [1, nz]); prefetched one plane ahead
+  const int q_first = ks - 1, q_last = ke + 1;
+  R pq = 0, cq = 0, rp = 0, rcv = 0;
+  auto fetch = [&](int q, R &p, R &c, R &p2, R &c2) {
+    const size_t pk = (size_t)(q - 1) * sz;
+    p = po[pk + ooff];
+    c = co[pk + ooff];
+    if (tid < NRING) {
+      p2 = po[pk + roff];
+      c2 = co[pk + roff];
+    }
+  };
+  fetch(q_first, pq, cq, rp, rcv);
+  for (int q = q_first; q <= q_last; ++q) {
+    const R pcur = pq, ccur = cq, rpc = rp, rcc = rcv;
+    if (q < q_last) fetch(q + 1, pq, cq, rp, rcv);
+    R *gw = sgm + (q % 3) * 2 * QPL;
+    solute_node(pcur, ccur, quint, s, igP, muP);
+    gw[oc] = igP;
+    gw[QPL + oc] = muP;
+    if (tid < NRING) {
+      R rig, rmu;
+      solute_node(rpc, rcc, quint, s, rig, rmu);
+      gw[rc] = rig;
+      gw[QPL + rc] = rmu;
+    }
+    __syncthreads();
+    const int k = q - 1;  // cell plane updated now
+    if (q >= ks + 1 && own) {
+      const size_t gidx = (size_t)(k - 1) * sz + ooff;
+      const R PP = pZ, cPP = cZ;
+      const int nc = oc;
+      const R *ig = sgm + (k % 3) * 2 * QPL, *mu = ig + QPL;
+      const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
+      const R igc = igZ, muc = muZ;
+      const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
+                                  fzu_prev, D2dt, s);
+      cn[gidx] = cPP + inc;
+      macc += inc;
+    }
+    igM = igZ; muM = muZ; igZ = igP; muZ = muP;
+    pZ = pcur; cZ = ccur;
+  }
+  if (mslot) {  // uniform over the work-group
+    __shared__ double wsum[NT / 64];
+    double v = (double)macc;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((tid & 63) == 0) wsum[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int w = 0; w < NT / 64; ++w) t += wsum[w];
+      *mslot += t;
+    }
+  }
+}
+
+template <typename TL, typename R>
+constexpr size_t solute_lds_bytes() {
+  return sizeof(R) * (size_t)(6 * TL::QPL);
 }
 
 

@@ -555,6 +555,14 @@ int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_gr
   return PF_OK;
 }
 
+int pf_set_sweep_split(pf_ctx *ctx, int32_t on, int32_t tile_s) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (pf::fused_set_split(ctx->fused, on != 0, tile_s) != PF_OK)
+    return fail(PF_E_ARG, "tile shape must be in [0,%d)", pf::PF_NVARIANTS);
+  return PF_OK;
+}
+
 int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]) {
   int rc = check_ctx(ctx, nullptr);
   if (rc != PF_OK) return rc;

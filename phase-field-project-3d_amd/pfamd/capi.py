@@ -73,7 +73,7 @@ EXPORTS = (
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
-    "pf_checkpoint_save", "pf_checkpoint_load",
+    "pf_checkpoint_save", "pf_checkpoint_load", "pf_set_sweep_split",
     "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
@@ -81,7 +81,7 @@ EXPORTS = (
     "pf_batch_autotune",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16), (16, 16))
-PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass")
+PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass", "solute")
 
 
 def load(path: str = LIB_PATH):
@@ -120,6 +120,7 @@ def load(path: str = LIB_PATH):
         "pf_get_tiling": ([vp, vp], C.c_int),
         "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
         "pf_set_mass_correction": ([vp, i32], C.c_int),
+        "pf_set_sweep_split": ([vp, i32, i32], C.c_int),
         "pf_checkpoint_save": ([vp, P, L, C.c_char_p], C.c_int),
         "pf_checkpoint_load": ([C.c_char_p, P, L, C.POINTER(vp)], C.c_int),
         "pf_batch_create": ([C.c_size_t, C.POINTER(vp)], C.c_int),
@@ -279,6 +280,9 @@ class Context:
     def set_tiling(self, tile_a: int, tile_b: int, target_groups: int = 1024):
         check(self.lib.pf_set_tiling(self.h, tile_a, tile_b, target_groups), self.lib)
 
+    def set_sweep_split(self, on: bool, tile_s: int = 0):
+        check(self.lib.pf_set_sweep_split(self.h, 1 if on else 0, int(tile_s)), self.lib)
+
     def tiling(self):
         """(tile_a, tile_b, z-planes per segment A, B)."""
         out = np.zeros(4, dtype=np.int32)
@@ -287,12 +291,13 @@ class Context:
 
     def autotune(self, p, reps: int = 3):
         """Times every tiling candidate; keeps the fastest.  Returns a list of
-        (sweep, tile, target_groups, ms)."""
+        (kind, tile, target_groups, ms) with kind A (sweep A), B (fused sweep
+        B), P (phase + T of the split form), S (its solute kernel)."""
         n = C.c_int32(0)
         ms = np.zeros(4 * 64)
         check(self.lib.pf_autotune(self.h, C.byref(p), reps, C.byref(n), _ptr(ms)), self.lib)
-        return [("AB"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]), float(ms[4 * q + 3]))
-                for q in range(n.value)]
+        return [("ABPS"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]),
+                 float(ms[4 * q + 3])) for q in range(n.value)]
 
     def checkpoint_save(self, p, L, path: str):
         check(self.lib.pf_checkpoint_save(self.h, C.byref(p), C.byref(L), path.encode()),
