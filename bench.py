@@ -42,7 +42,11 @@ HBM_PEAK_GBS = 8000.0
 #   sweep A reads phi, c and writes phi                       -> 3 fields
 #   sweep B reads phi, c and writes phi, c, T (T is not read:
 #   the temperature update's Laplacian has D = 0)              -> 5 fields
-FIELDS_PER_CELL = {"sweep_A": 3, "sweep_B": 5}
+# fields moved per cell by each sweep kernel (compulsory: every field read or
+# written once); sweep B reads phi, c and writes phi, c, T when fused and
+# phi, T when split, with the solute kernel reading phi, c and writing c
+FIELDS_PER_CELL = {"sweep_A": 3, "sweep_B": 5, "solute": 3}
+FIELDS_PER_CELL_SPLIT = {"sweep_A": 3, "sweep_B": 4, "solute": 3}
 
 
 def workload_deck():
@@ -64,7 +68,17 @@ def _sync(sim):
         pass
 
 
-def run_form(precision, steps, warmup, timing=True, autotune=True):
+def parse_tiling(spec):
+    """'ta,tb,target,split,tile_s' -> tuple of ints (a fixed tiling in place
+    of the autotuner, so that a profiling run launches the same kernels as a
+    timed one)."""
+    v = tuple(int(x) for x in spec.split(","))
+    if len(v) != 5:
+        raise SystemExit("--tiling wants ta,tb,target,split,tile_s")
+    return v
+
+
+def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     """Warm-up, then exactly `steps` timed steps; returns timings."""
     from pfamd.run import Simulation
 
@@ -77,7 +91,13 @@ def run_form(precision, steps, warmup, timing=True, autotune=True):
         sim.run(warmup)
     # pick the sweep tiling on this grid (times every candidate once, before
     # the timed region; results do not depend on it)
-    cands = sim.ctx.autotune(sim.p, reps=3) if autotune else []
+    cands = []
+    if tiling is not None:
+        sim.ctx.set_tiling(tiling[0], tiling[1], tiling[2])
+        sim.ctx.set_sweep_split(bool(tiling[3]), tiling[4])
+    elif autotune:
+        cands = sim.ctx.autotune(sim.p, reps=3)
+    split = sim.ctx.sweep_split()
     tiling = sim.ctx.tiling()
     _sync(sim)
     sim.ctx.timing(timing)
@@ -97,7 +117,7 @@ def run_form(precision, steps, warmup, timing=True, autotune=True):
     nbytes = sim.ctx.device_bytes()
     sim.close()
     return {"wall_s": wall, "kernels": kt, "diag": diag.tolist(), "init_s": t_init,
-            "device_bytes": nbytes, "tiling": tiling, "autotune": cands}
+            "device_bytes": nbytes, "tiling": tiling, "split": split, "autotune": cands}
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_fp64_256x256x528_pmc.md")
@@ -120,7 +140,8 @@ def pmc_traffic(kernel="sweep_B_fp64"):
 def roofline(res, precision_bytes):
     ncell = NX * NY * NZ
     best = None
-    for nm, nf in FIELDS_PER_CELL.items():
+    fields = FIELDS_PER_CELL_SPLIT if res["split"][0] else FIELDS_PER_CELL
+    for nm, nf in fields.items():
         ms, cnt = res["kernels"].get(nm, (0.0, 0))
         if cnt == 0:
             continue
@@ -296,6 +317,10 @@ def tiling_summary(res):
     for nm, t, seg in (("sweep_A", res["tiling"][0], res["tiling"][2]),
                        ("sweep_B", res["tiling"][1], res["tiling"][3])):
         out[nm] = {"tile": "x".join(map(str, TILE_SHAPES[t])), "z_planes_per_group": seg}
+    on, ts, segs = res["split"]
+    out["sweep_B"]["form"] = "split: phase + T kernel, then the solute kernel" if on else "fused"
+    if on:
+        out["solute"] = {"tile": "x".join(map(str, TILE_SHAPES[ts])), "z_planes_per_group": segs}
     return out
 
 
@@ -310,7 +335,12 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the library's default tiling (for profiling runs)")
+                    help="keep the library's default tiling")
+    ap.add_argument("--tiling", type=parse_tiling, default=None,
+                    help="FP64 tiling ta,tb,target,split,tile_s instead of the autotuner "
+                         "(for profiling runs)")
+    ap.add_argument("--tiling32", type=parse_tiling, default=None,
+                    help="the same for the FP32-mixed form")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (instances of this "
@@ -322,11 +352,11 @@ def main(argv=None):
         print("no GPU visible", file=sys.stderr)
         return 1
     log(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps")
-    r64 = run_form(capi.PF_FP64, a.steps, a.warmup, autotune=not a.no_autotune)
+    r64 = run_form(capi.PF_FP64, a.steps, a.warmup, autotune=not a.no_autotune, tiling=a.tiling)
     ncell = NX * NY * NZ
     mlups = ncell * a.steps / r64["wall_s"] / 1e6
     log(f"[bench] FP64 autotune {r64['autotune']}")
-    log(f"[bench] FP64 {mlups:.1f} MLUPS, tiling {r64['tiling']}, kernels {r64['kernels']}")
+    log(f"[bench] FP64 {mlups:.1f} MLUPS, tiling {r64['tiling']} split {r64['split']}, kernels {r64['kernels']}")
     rl = roofline(r64, 8)
     out = {
         "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
@@ -358,10 +388,11 @@ def main(argv=None):
     log(f"[bench] STREAM copy {gbs_copy:.0f} GB/s")
     if not a.no_fp32:
         log("[bench] FP32-mixed")
-        r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup, autotune=not a.no_autotune)
+        r32 = run_form(capi.PF_FP32_MIXED, a.steps, a.warmup, autotune=not a.no_autotune,
+                       tiling=a.tiling32)
         m32 = ncell * a.steps / r32["wall_s"] / 1e6
         log(f"[bench] FP32 autotune {r32['autotune']}")
-        log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['tiling']}, kernels {r32['kernels']}")
+        log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['tiling']} split {r32['split']}, kernels {r32['kernels']}")
         rl32 = roofline(r32, 4)
         out["fp32_mixed"] = {"value": round(m32, 2), "unit": "MLUPS",
                              "ms_per_step": r32["wall_s"] / a.steps * 1e3,

@@ -208,6 +208,8 @@ int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_gr
  * solute-only kernel (on = 1, the default until pf_autotune measures) whose
  * tile shape is tile_s.  Both forms give the same bits. */
 int pf_set_sweep_split(pf_ctx *ctx, int32_t on, int32_t tile_s);
+/* out = {split on, tile shape of the solute kernel, its z-planes per segment}. */
+int pf_get_sweep_split(const pf_ctx *ctx, int32_t out[3]);
 /* out = {tile of sweep A, tile of sweep B, z-planes per segment A, B}. */
 int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
 /* Times each tile shape x segment split of sweep A, of the fused sweep B and

@@ -33,6 +33,13 @@ COMMON = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
     # keep hipcc from forming packed FP32 ops (v_pk_*_f32)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # a*b + c becomes an FMA only within one source expression (decided by
+    # the front end), never across statements by the code generator: how it
+    # fuses across statements depends on the surrounding code, which differs
+    # between kernels that inline the same arithmetic (own vs halo node,
+    # tile shapes, the single-instance and batched kernels), and results
+    # must not depend on which kernel computed a value
+    "-ffp-contract=on",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas",
 ]

@@ -27,6 +27,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pf_common.h"
 #include "pf_math.h"
 
@@ -324,102 +326,108 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   R fzu_prev = (R)0;  // solute flux through the up face of the previous cell
 
   const int q_first = ks - 2, q_last = ke + 2;
-  gload(q_first);
-  for (int q = q_first; q <= q_last; ++q) {
+  // One z-plane of the march.  The first four planes (ST false) only load
+  // phi and build nodes.  From plane ks + 2 on (ST true) each iteration builds
+  // the nodes of plane q-1 and updates cell plane q-2 with no branch between
+  // the two: cells outside the update range (edge columns, planes 2 and nz-1,
+  // columns beyond the grid) are computed on clamped, in-bounds data and only
+  // their stores are masked, so the long dependency chain of a node (the
+  // reciprocal) and the independent arithmetic of the update are one
+  // scheduling region.
+  auto plane_step = [&](int q, auto steady) __attribute__((always_inline)) {
+    constexpr bool ST = decltype(steady)::value;
     lstore(q);
     if (q < q_last) gload(q + 1);  // prefetch; lands while this plane computes
     __syncthreads();
     const int kn = q - 1;  // plane whose node / solute quantities are built now
+    // (steady planes: kn in [ks+1, ke+1], inside [3, nz]; a node of plane nz
+    // is built from clamped planes and never used)
+    const bool build = ST || (q >= ks && kn >= 2 && kn <= nz - 1);
     Node<R> nr{};
     R rig = 0, rmu = 0, cnew_col = 0;
-    if (q >= ks) {
-      if (kn >= 2 && kn <= nz - 1) {  // nodes are only used for planes [2, nz-1]
-        nP = node_at(q, olx, oly);
-        if (tid < NRING) nr = node_at(q, rlx, rly);
-      }
-      if constexpr (SWEEP_B) {
-        if (kn >= 1 && kn <= nz) {
-          const R *P0 = slot(q - 1);
-          const size_t pk = (size_t)(kn - 1) * sz;
-          cnew_col = co[pk + ooff];
-          solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
-          if (tid < NRING)
-            solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu);
-        }
+    if (build) {
+      nP = node_at(q, olx, oly);
+      if (tid < NRING) nr = node_at(q, rlx, rly);
+    }
+    if constexpr (SWEEP_B) {
+      if (ST || (q >= ks && kn >= 1 && kn <= nz)) {
+        const R *P0 = slot(q - 1);
+        const size_t pk = (size_t)(kn - 1) * sz;
+        cnew_col = co[pk + ooff];
+        solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
+        if (tid < NRING)
+          solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu);
       }
     }
-    const int k = q - 2;  // cell plane updated now
-    if (q >= ks + 2 && own) {
+    if constexpr (ST) {
+      const int k = q - 2;  // cell plane updated now
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = P0[c];
       const size_t gidx = (size_t)(k - 1) * sz + ooff;
-      R dppv = (R)0, pnew = PP;
+      const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
       if constexpr (SWEEP_B) cPP = cZ; else cPP = co[gidx];
-      if (ph_col && k >= 3 && k <= nz - 2) {
-        const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
-        const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
-                PSW = P0[c - PX - 1];
-        const R PT = Pp[c], PB = Pm[c];
-        const R PNT = Pp[c + PX], PST = Pp[c - PX], PET = Pp[c + 1], PWT = Pp[c - 1];
-        const R PNB = Pm[c + PX], PSB = Pm[c - PX], PEB = Pm[c + 1], PWB = Pm[c - 1];
-        const int nc = oly * QX + olx;
-        const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
-        const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
-        const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
-        const R yzN = yz[nc + QX], yzS = yz[nc - QX];
-        const R zxE = zx[nc + 1], zxW = zx[nc - 1];
-        const R yzT = nP.yz, yzB = nM.yz, zxT = nP.zx, zxB = nM.zx;
-        R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
-               (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
-               (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
-        const R Cxy = PNE * (xyE - xyN) + PSE * (xyS - xyE) + PNW * (xyN - xyW) +
-                      PSW * (xyW - xyS) + PN * (xyE - xyW) + PS * (xyW - xyE) +
-                      PE * (xyS - xyN) + PW * (xyN - xyS);
-        const R Cyz = PNT * (yzN - yzT) + PNB * (yzB - yzN) + PST * (yzT - yzS) +
-                      PSB * (yzS - yzB) + PT * (yzN - yzS) + PB * (yzS - yzN) +
-                      PN * (yzB - yzT) + PS * (yzT - yzB);
-        const R Czx = PET * (zxT - zxE) + PWT * (zxW - zxT) + PEB * (zxE - zxB) +
-                      PWB * (zxB - zxW) + PE * (zxT - zxB) + PW * (zxB - zxT) +
-                      PT * (zxW - zxE) + PB * (zxE - zxW);
-        AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
-        const R om = (R)1 - PP;
-        const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
-        R h, hl;
-        h_of(PP, quint, h, hl);
-        // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
-        const R den = s.Ke * h + ((R)1 - h);
-        const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
-        const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
-        R ruidos = (R)0;
-        if (noise) {
-          // the draw depends on the step, not the sweep; passing it from
-          // the first sweep to the last through HBM measured slower than
-          // hashing again (an unprefetched load inside the march)
-          const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH
-                                ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
-                                : (float)hash_unit(hbase);
-          const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
-                                                                : ((R)2 * (R)ran - (R)1) * s.AAA;
-          ruidos = amp * (R)16 * (PP * PP) * (om * om);
-        }
-        const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
-        dppv = pstar - PP;
-        const R cl = pstar > (R)1 ? (R)1 : pstar;
-        pnew = cl < (R)0 ? (R)0 : cl;
-        pn[gidx] = pnew;
+      const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
+      const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
+              PSW = P0[c - PX - 1];
+      const R PT = Pp[c], PB = Pm[c];
+      const R PNT = Pp[c + PX], PST = Pp[c - PX], PET = Pp[c + 1], PWT = Pp[c - 1];
+      const R PNB = Pm[c + PX], PSB = Pm[c - PX], PEB = Pm[c + 1], PWB = Pm[c - 1];
+      const int nc = oly * QX + olx;
+      const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
+      const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
+      const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
+      const R yzN = yz[nc + QX], yzS = yz[nc - QX];
+      const R zxE = zx[nc + 1], zxW = zx[nc - 1];
+      const R yzT = nP.yz, yzB = nM.yz, zxT = nP.zx, zxB = nM.zx;
+      R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
+             (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
+             (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
+      const R Cxy = PNE * (xyE - xyN) + PSE * (xyS - xyE) + PNW * (xyN - xyW) +
+                    PSW * (xyW - xyS) + PN * (xyE - xyW) + PS * (xyW - xyE) +
+                    PE * (xyS - xyN) + PW * (xyN - xyS);
+      const R Cyz = PNT * (yzN - yzT) + PNB * (yzB - yzN) + PST * (yzT - yzS) +
+                    PSB * (yzS - yzB) + PT * (yzN - yzS) + PB * (yzS - yzN) +
+                    PN * (yzB - yzT) + PS * (yzT - yzB);
+      const R Czx = PET * (zxT - zxE) + PWT * (zxW - zxT) + PEB * (zxE - zxB) +
+                    PWB * (zxB - zxW) + PE * (zxT - zxB) + PW * (zxB - zxT) +
+                    PT * (zxW - zxE) + PB * (zxE - zxW);
+      AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
+      const R om = (R)1 - PP;
+      const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
+      R h, hl;
+      h_of(PP, quint, h, hl);
+      // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
+      const R den = s.Ke * h + ((R)1 - h);
+      const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
+      const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
+      R ruidos = (R)0;
+      if (noise) {
+        // the draw depends on the step, not the sweep; passing it from the
+        // first sweep to the last through HBM measured slower than hashing
+        // again (an unprefetched load inside the march)
+        const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                                                           : (float)hash_unit(hbase);
+        const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
+                                                              : ((R)2 * (R)ran - (R)1) * s.AAA;
+        ruidos = amp * (R)16 * (PP * PP) * (om * om);
       }
+      const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
+      const R cl = pstar > (R)1 ? (R)1 : pstar;
+      const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
+      const R dppv = upd ? pstar - PP : (R)0;
+      if (upd) pn[gidx] = pnew;
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
-        const int nc = oly * QX + olx;
         const R *ig = sgm, *mu = sgm + QPL;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
-        const R igc = igZ, muc = muZ;
-        const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 2,
+        const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
-        cn[gidx] = cPP + inc;
-        macc += inc;
+        if (own) {
+          cn[gidx] = cPP + inc;
+          macc += inc;
+        }
       }
       if constexpr (T_OUT) {
         if (write_T) {
@@ -427,34 +435,34 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
           R hh, hl2;
           h_of(pnew, quint, hh, hl2);
           R lap = (R)0;
-          if (s.Ddt != (R)0) {
+          if (s.Ddt != (R)0 && own) {
             const R TP2 = (R)2 * s.TP;
             lap = (Tcur[gidx + 1] - TP2 + Tcur[gidx - 1]) * s.idx2 +
                   (Tcur[gidx + nx] - TP2 + Tcur[gidx - nx]) * s.idy2 +
                   (Tcur[gidx + sz] - TP2 + Tcur[gidx - sz]) * s.idz2;
           }
-          Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
-        } else if (ph_col && k >= 3 && k <= nz - 2) {
+          if (own) Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
+        } else if (upd) {
           dpp[gidx] = dppv;
         }
       } else {
-        (void)Tcur; (void)Tnext; (void)dpp; (void)write_T;
+        (void)Tcur; (void)Tnext; (void)dpp; (void)write_T; (void)dppv;
       }
       hbase += kHashStepK;
     }
     __syncthreads();
     // publish plane kn's node / solute quantities and rotate the column
-    if (q >= ks) {
-      const int oc = oly * QX + olx, rc = rly * QX + rlx;
-      if (kn >= 2 && kn <= nz - 1) {
-        snode[oc] = nP.e2; snode[QPL + oc] = nP.xy; snode[2 * QPL + oc] = nP.yz;
-        snode[3 * QPL + oc] = nP.zx;
-        if (tid < NRING) {
-          snode[rc] = nr.e2; snode[QPL + rc] = nr.xy; snode[2 * QPL + rc] = nr.yz;
-          snode[3 * QPL + rc] = nr.zx;
-        }
+    const int oc = oly * QX + olx, rc = rly * QX + rlx;
+    if (build) {
+      snode[oc] = nP.e2; snode[QPL + oc] = nP.xy; snode[2 * QPL + oc] = nP.yz;
+      snode[3 * QPL + oc] = nP.zx;
+      if (tid < NRING) {
+        snode[rc] = nr.e2; snode[QPL + rc] = nr.xy; snode[2 * QPL + rc] = nr.yz;
+        snode[3 * QPL + rc] = nr.zx;
       }
-      if constexpr (SWEEP_B) {
+    }
+    if constexpr (SWEEP_B) {
+      if (ST || q >= ks) {
         sgm[oc] = igP; sgm[QPL + oc] = muP;
         if (tid < NRING) {
           sgm[rc] = rig; sgm[QPL + rc] = rmu;
@@ -466,7 +474,10 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       igM = igZ; muM = muZ; igZ = igP; muZ = muP;
       cZ = cnew_col;
     }
-  }
+  };
+  gload(q_first);
+  for (int q = q_first; q < ks + 2; ++q) plane_step(q, std::false_type{});
+  for (int q = ks + 2; q <= q_last; ++q) plane_step(q, std::true_type{});
   if constexpr (SWEEP_B) {
     if (mslot) {  // uniform over the work-group
       __shared__ double wsum[NT / 64];

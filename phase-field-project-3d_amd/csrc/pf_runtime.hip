@@ -574,6 +574,16 @@ int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]) {
   return PF_OK;
 }
 
+int pf_get_sweep_split(const pf_ctx *ctx, int32_t out[3]) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail(PF_E_ARG, "null output");
+  out[0] = ctx->fused.split_b ? 1 : 0;
+  out[1] = ctx->fused.tile_s;
+  out[2] = ctx->fused.seg_s;
+  return PF_OK;
+}
+
 int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms) {
   int rc = check_ctx(ctx, p);
   if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");

@@ -74,6 +74,7 @@ EXPORTS = (
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
     "pf_checkpoint_save", "pf_checkpoint_load", "pf_set_sweep_split",
+    "pf_get_sweep_split",
     "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
@@ -121,6 +122,7 @@ def load(path: str = LIB_PATH):
         "pf_autotune": ([vp, P, i32, vp, vp], C.c_int),
         "pf_set_mass_correction": ([vp, i32], C.c_int),
         "pf_set_sweep_split": ([vp, i32, i32], C.c_int),
+        "pf_get_sweep_split": ([vp, vp], C.c_int),
         "pf_checkpoint_save": ([vp, P, L, C.c_char_p], C.c_int),
         "pf_checkpoint_load": ([C.c_char_p, P, L, C.POINTER(vp)], C.c_int),
         "pf_batch_create": ([C.c_size_t, C.POINTER(vp)], C.c_int),
@@ -282,6 +284,12 @@ class Context:
 
     def set_sweep_split(self, on: bool, tile_s: int = 0):
         check(self.lib.pf_set_sweep_split(self.h, 1 if on else 0, int(tile_s)), self.lib)
+
+    def sweep_split(self):
+        """(split on, solute kernel's tile shape, its z-planes per segment)."""
+        out = np.zeros(3, dtype=np.int32)
+        check(self.lib.pf_get_sweep_split(self.h, out.ctypes.data_as(C.c_void_p)), self.lib)
+        return (bool(out[0]), int(out[1]), int(out[2]))
 
     def tiling(self):
         """(tile_a, tile_b, z-planes per segment A, B)."""
