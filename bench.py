@@ -69,13 +69,25 @@ def _sync(sim):
 
 
 def parse_tiling(spec):
-    """'ta,tb,target,split,tile_s' -> tuple of ints (a fixed tiling in place
-    of the autotuner, so that a profiling run launches the same kernels as a
-    timed one)."""
-    v = tuple(int(x) for x in spec.split(","))
-    if len(v) != 5:
-        raise SystemExit("--tiling wants ta,tb,target,split,tile_s")
-    return v
+    """'tA:gA,tB:gB,tS:gS,split' (tile shape : work-group target of sweep A,
+    sweep B and the split form's solute kernel, then 0/1 for the split form)
+    -> ((tA, gA), (tB, gB), (tS, gS), split): a fixed tiling in place of the
+    autotuner, so that a profiling run launches the same kernels as a timed
+    one (the bench line's tiling.spec gives the autotuner's pick in this
+    form)."""
+    try:
+        parts = spec.split(",")
+        kinds = tuple(tuple(int(x) for x in p.split(":")) for p in parts[:3])
+        split = int(parts[3])
+        assert len(parts) == 4 and all(len(k) == 2 for k in kinds)
+    except (ValueError, AssertionError, IndexError):
+        raise SystemExit("--tiling wants tA:gA,tB:gB,tS:gS,split")
+    return kinds + (split,)
+
+
+def tiling_spec(sim):
+    k = [sim.ctx.sweep_tiling(q) for q in range(3)]
+    return ",".join(f"{t}:{g}" for t, g, _ in k) + f",{int(sim.ctx.sweep_split()[0])}"
 
 
 def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
@@ -93,12 +105,14 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     # the timed region; results do not depend on it)
     cands = []
     if tiling is not None:
-        sim.ctx.set_tiling(tiling[0], tiling[1], tiling[2])
-        sim.ctx.set_sweep_split(bool(tiling[3]), tiling[4])
+        for q in range(3):
+            sim.ctx.set_sweep_tiling(q, *tiling[q])
+        sim.ctx.set_sweep_split(bool(tiling[3]), tiling[2][0])
     elif autotune:
         cands = sim.ctx.autotune(sim.p, reps=3)
     split = sim.ctx.sweep_split()
     tiling = sim.ctx.tiling()
+    spec = tiling_spec(sim)
     _sync(sim)
     sim.ctx.timing(timing)
     done = 0
@@ -117,7 +131,8 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     nbytes = sim.ctx.device_bytes()
     sim.close()
     return {"wall_s": wall, "kernels": kt, "diag": diag.tolist(), "init_s": t_init,
-            "device_bytes": nbytes, "tiling": tiling, "split": split, "autotune": cands}
+            "device_bytes": nbytes, "tiling": tiling, "split": split, "spec": spec,
+            "autotune": cands}
 
 
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_fp64_256x256x528_pmc.md")
@@ -317,6 +332,7 @@ def tiling_summary(res):
     for nm, t, seg in (("sweep_A", res["tiling"][0], res["tiling"][2]),
                        ("sweep_B", res["tiling"][1], res["tiling"][3])):
         out[nm] = {"tile": "x".join(map(str, TILE_SHAPES[t])), "z_planes_per_group": seg}
+    out["spec"] = res["spec"]
     on, ts, segs = res["split"]
     out["sweep_B"]["form"] = "split: phase + T kernel, then the solute kernel" if on else "fused"
     if on:
@@ -337,7 +353,7 @@ def main(argv=None):
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling")
     ap.add_argument("--tiling", type=parse_tiling, default=None,
-                    help="FP64 tiling ta,tb,target,split,tile_s instead of the autotuner "
+                    help="FP64 tiling tA:gA,tB:gB,tS:gS,split instead of the autotuner "
                          "(for profiling runs)")
     ap.add_argument("--tiling32", type=parse_tiling, default=None,
                     help="the same for the FP32-mixed form")
@@ -356,7 +372,7 @@ def main(argv=None):
     ncell = NX * NY * NZ
     mlups = ncell * a.steps / r64["wall_s"] / 1e6
     log(f"[bench] FP64 autotune {r64['autotune']}")
-    log(f"[bench] FP64 {mlups:.1f} MLUPS, tiling {r64['tiling']} split {r64['split']}, kernels {r64['kernels']}")
+    log(f"[bench] FP64 {mlups:.1f} MLUPS, tiling {r64['spec']}, kernels {r64['kernels']}")
     rl = roofline(r64, 8)
     out = {
         "metric": "MLUPS", "value": round(mlups, 2), "unit": "MLUPS", "n_gpus": 1,
@@ -392,7 +408,7 @@ def main(argv=None):
                        tiling=a.tiling32)
         m32 = ncell * a.steps / r32["wall_s"] / 1e6
         log(f"[bench] FP32 autotune {r32['autotune']}")
-        log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['tiling']} split {r32['split']}, kernels {r32['kernels']}")
+        log(f"[bench] FP32 {m32:.1f} MLUPS, tiling {r32['spec']}, kernels {r32['kernels']}")
         rl32 = roofline(r32, 4)
         out["fp32_mixed"] = {"value": round(m32, 2), "unit": "MLUPS",
                              "ms_per_step": r32["wall_s"] / a.steps * 1e3,

@@ -210,6 +210,12 @@ int pf_set_tiling(pf_ctx *ctx, int32_t tile_a, int32_t tile_b, int32_t target_gr
 int pf_set_sweep_split(pf_ctx *ctx, int32_t on, int32_t tile_s);
 /* out = {split on, tile shape of the solute kernel, its z-planes per segment}. */
 int pf_get_sweep_split(const pf_ctx *ctx, int32_t out[3]);
+/* Tile shape and work-group target of one kernel kind (0 = sweep A, 1 =
+ * sweep B or the phase + T kernel of its split form, 2 = the solute kernel
+ * of the split form), as pf_autotune sets them per kind; get returns
+ * {tile, target_groups, z-planes per segment}. */
+int pf_set_sweep_tiling(pf_ctx *ctx, int32_t kind, int32_t tile, int32_t target_groups);
+int pf_get_sweep_tiling(const pf_ctx *ctx, int32_t kind, int32_t out[3]);
 /* out = {tile of sweep A, tile of sweep B, z-planes per segment A, B}. */
 int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
 /* Times each tile shape x segment split of sweep A, of the fused sweep B and

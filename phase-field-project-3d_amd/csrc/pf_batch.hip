@@ -624,7 +624,8 @@ int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nste
   if (p->nx < 6 || p->ny < 6 || p->nz < 6)
     return fail_msg(PF_E_ARG, "batched grids need every extent >= 6 (got %dx%dx%d)", p->nx, p->ny,
                     p->nz);
-  if ((long long)p->nx * p->ny > (1LL << 30)) return fail_msg(PF_E_ARG, "plane too large");
+  if ((long long)p->nx * p->ny * (p->precision == PF_FP64 ? 8 : 4) >= (1LL << 32))
+    return fail_msg(PF_E_ARG, "plane too large (a plane of a field must be < 4 GiB)");
   if (p->precision != PF_FP64 && p->precision != PF_FP32_MIXED)
     return fail_msg(PF_E_ARG, "unknown precision %d", p->precision);
   if (nsteps < 0) return fail_msg(PF_E_ARG, "negative step count");

@@ -28,6 +28,7 @@ struct FusedState {
   int tile_a = 0, tile_b = 0;  // tile shape of sweep A / sweep B
   int target = 2048;           // work-groups per sweep the z split aims for
   int seg_a = 0, seg_b = 0;    // z-planes per work-group segment
+  int tgt_a = 2048, tgt_b = 2048, tgt_s = 2048;  // the targets the segments come from
   // split form of sweep B: phase + T, then the solute in its own kernel
   bool split_b = false;
   int tile_s = 0, seg_s = 0;
@@ -79,6 +80,11 @@ int fused_init(FusedState &s, int nx, int ny, int nz, int prec);
 int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups);
 // Split form of sweep B on/off and the solute kernel's tile shape.
 int fused_set_split(FusedState &s, bool on, int tile_s);
+// Tile shape and work-group target of one kernel kind: 0 = sweep A, 1 =
+// sweep B (fused, or the phase + T kernel of the split form), 2 = the
+// solute kernel of the split form.
+int fused_set_kind(FusedState &s, int kind, int tile, int target_groups);
+int fused_get_kind(const FusedState &s, int kind, int out[3]);  // {tile, target, seg}
 // Times every tile shape x z-split candidate of sweep A, the fused sweep B
 // and the two kernels of its split form on the given fields (writing into
 // scratch buffers of its own) and keeps the fastest; the split form is

@@ -262,20 +262,34 @@ int fused_set_tile(FusedState &s, int tile_a, int tile_b, int target_groups) {
   if (tile_a < 0 || tile_a >= PF_NVARIANTS || tile_b < 0 || tile_b >= PF_NVARIANTS ||
       target_groups < 1)
     return PF_E_ARG;
-  s.tile_a = tile_a;
-  s.tile_b = tile_b;
   s.target = target_groups;
-  s.seg_a = pick_seg(s.nx, s.ny, s.nz - 4, tile_a, target_groups);
-  s.seg_b = pick_seg(s.nx, s.ny, s.nz - 2, tile_b, target_groups);
-  s.seg_s = pick_seg(s.nx, s.ny, s.nz - 2, s.tile_s, target_groups);
+  fused_set_kind(s, 0, tile_a, target_groups);
+  fused_set_kind(s, 1, tile_b, target_groups);
+  fused_set_kind(s, 2, s.tile_s, target_groups);
   return PF_OK;
 }
 
 int fused_set_split(FusedState &s, bool on, int tile_s) {
   if (tile_s < 0 || tile_s >= PF_NVARIANTS) return PF_E_ARG;
   s.split_b = on;
-  s.tile_s = tile_s;
-  s.seg_s = pick_seg(s.nx, s.ny, s.nz - 2, tile_s, s.target);
+  return fused_set_kind(s, 2, tile_s, s.tgt_s);
+}
+
+int fused_set_kind(FusedState &s, int kind, int tile, int target_groups) {
+  if (kind < 0 || kind > 2 || tile < 0 || tile >= PF_NVARIANTS || target_groups < 1)
+    return PF_E_ARG;
+  const int seg = pick_seg(s.nx, s.ny, kind == 0 ? s.nz - 4 : s.nz - 2, tile, target_groups);
+  if (kind == 0) { s.tile_a = tile; s.tgt_a = target_groups; s.seg_a = seg; }
+  if (kind == 1) { s.tile_b = tile; s.tgt_b = target_groups; s.seg_b = seg; }
+  if (kind == 2) { s.tile_s = tile; s.tgt_s = target_groups; s.seg_s = seg; }
+  return PF_OK;
+}
+
+int fused_get_kind(const FusedState &s, int kind, int out[3]) {
+  if (kind < 0 || kind > 2) return PF_E_ARG;
+  out[0] = kind == 0 ? s.tile_a : kind == 1 ? s.tile_b : s.tile_s;
+  out[1] = kind == 0 ? s.tgt_a : kind == 1 ? s.tgt_b : s.tgt_s;
+  out[2] = kind == 0 ? s.seg_a : kind == 1 ? s.seg_b : s.seg_s;
   return PF_OK;
 }
 
@@ -296,8 +310,8 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
   // kinds: 0 sweep A, 1 fused sweep B, 2 phase + T of the split form,
   // 3 solute kernel of the split form
   float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
-  int btile[4] = {S.tile_a, S.tile_b, S.tile_b, S.tile_s}, bseg[4] = {S.seg_a, S.seg_b, S.seg_b,
-                                                                         S.seg_s};
+  int btile[4] = {S.tile_a, S.tile_b, S.tile_b, S.tile_s};
+  int btgt[4] = {S.tgt_a, S.tgt_b, S.tgt_b, S.tgt_s};
   int cand = 0;
   for (int kind = 0; kind < 4 && rc == PF_OK; ++kind) {
     for (int tile = 0; tile < PF_NVARIANTS && rc == PF_OK; ++tile) {
@@ -330,19 +344,17 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
         if (ms < best[kind]) {
           best[kind] = ms;
           btile[kind] = tile;
-          bseg[kind] = seg;
+          btgt[kind] = target;
         }
       }
     }
   }
   if (rc == PF_OK) {
-    S.tile_a = btile[0];
-    S.seg_a = bseg[0];
     S.split_b = best[2] + best[3] < best[1];
-    S.tile_b = S.split_b ? btile[2] : btile[1];
-    S.seg_b = S.split_b ? bseg[2] : bseg[1];
-    S.tile_s = btile[3];
-    S.seg_s = bseg[3];
+    const int kb = S.split_b ? 2 : 1;
+    fused_set_kind(S, 0, btile[0], btgt[0]);
+    fused_set_kind(S, 1, btile[kb], btgt[kb]);
+    fused_set_kind(S, 2, btile[3], btgt[3]);
   }
   if (res) {
     res->n = cand < PF_AUTOTUNE_MAX ? cand : PF_AUTOTUNE_MAX;

@@ -215,6 +215,20 @@ __device__ __forceinline__ R solute_inc(const R *ig, const R *mu, int nc, int qx
   return D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
 }
 
+// The element at byte offset boff from base.  With a uniform base (a field
+// pointer plus a whole number of planes, kept in scalar registers) and a
+// 32-bit per-lane byte offset, the code generator addresses global memory
+// with an SGPR base and a VGPR offset instead of 64-bit vector arithmetic per
+// access.  A plane of a field must be under 4 GiB (checked at launch).
+template <typename T>
+__device__ __forceinline__ T &at(T *base, uint32_t boff) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + boff);
+}
+template <typename T>
+__device__ __forceinline__ const T &at(const T *base, uint32_t boff) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
+}
+
 struct Geo {
   int nx, ny, nz;
   int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
@@ -277,15 +291,18 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // In-plane offsets (32 bit; nx*ny < 2^31) of the phi window elements this
   // thread loads, of its own column and of its ring column; clamped so every
   // load stays inside the grid (clamped values feed only unused nodes).
-  int loff[NLD];
+  uint32_t loff[NLD];  // byte offsets
 #pragma unroll
   for (int u = 0; u < NLD; ++u) {
     const int e = tid + u * NT;
     const int gi = clampi(i0 - 2 + (e % PX), 1, nx), gj = clampi(j0 - 2 + (e / PX), 1, ny);
-    loff[u] = (gi - 1) + (gj - 1) * nx;
+    loff[u] = (uint32_t)((gi - 1) + (gj - 1) * nx) * (uint32_t)sizeof(R);
   }
-  const int ooff = (clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx;
-  const int roff = (clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx;
+  const uint32_t ooff =
+      (uint32_t)((clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx) * (uint32_t)sizeof(R);
+  const uint32_t roff =
+      (uint32_t)((clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx) *
+      (uint32_t)sizeof(R);
   auto plane = [&](int q) -> size_t { return (size_t)(clampi(q, 1, nz) - 1) * sz; };
 
   R pre[NLD];
@@ -293,7 +310,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     const R *src = po + plane(q);
 #pragma unroll
     for (int u = 0; u < NLD; ++u)
-      if (tid + u * NT < PPL) pre[u] = src[loff[u]];
+      if (tid + u * NT < PPL) pre[u] = at(src, loff[u]);
   };
   auto slot = [&](int q) -> R * { return sphi + (q % NSLOT) * PPL; };  // q >= 0
   auto lstore = [&](int q) {
@@ -353,10 +370,10 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       if (ST || (q >= ks && kn >= 1 && kn <= nz)) {
         const R *P0 = slot(q - 1);
         const size_t pk = (size_t)(kn - 1) * sz;
-        cnew_col = co[pk + ooff];
+        cnew_col = at(co + pk, ooff);
         solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
         if (tid < NRING)
-          solute_node(P0[(rly + 1) * PX + rlx + 1], co[pk + roff], quint, s, rig, rmu);
+          solute_node(P0[(rly + 1) * PX + rlx + 1], at(co + pk, roff), quint, s, rig, rmu);
       }
     }
     if constexpr (ST) {
@@ -364,10 +381,10 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = P0[c];
-      const size_t gidx = (size_t)(k - 1) * sz + ooff;
+      const size_t pk = (size_t)(k - 1) * sz;  // plane k of a field
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
-      if constexpr (SWEEP_B) cPP = cZ; else cPP = co[gidx];
+      if constexpr (SWEEP_B) cPP = cZ; else cPP = at(co + pk, ooff);
       const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
       const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
               PSW = P0[c - PX - 1];
@@ -384,15 +401,17 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
              (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
              (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
-      const R Cxy = PNE * (xyE - xyN) + PSE * (xyS - xyE) + PNW * (xyN - xyW) +
-                    PSW * (xyW - xyS) + PN * (xyE - xyW) + PS * (xyW - xyE) +
-                    PE * (xyS - xyN) + PW * (xyN - xyS);
-      const R Cyz = PNT * (yzN - yzT) + PNB * (yzB - yzN) + PST * (yzT - yzS) +
-                    PSB * (yzS - yzB) + PT * (yzN - yzS) + PB * (yzS - yzN) +
-                    PN * (yzB - yzT) + PS * (yzT - yzB);
-      const R Czx = PET * (zxT - zxE) + PWT * (zxW - zxT) + PEB * (zxE - zxB) +
-                    PWB * (zxB - zxW) + PE * (zxT - zxB) + PW * (zxB - zxT) +
-                    PT * (zxW - zxE) + PB * (zxE - zxW);
+      // the cross-derivative terms (phase_body.inc:137-149), each a sum of
+      // eight phi x node-difference products, collected by node quantity:
+      // every node value multiplies the sum of two phi differences across it
+      // (the same sum; 39 operations instead of 48)
+      const R dNS0 = PN - PS, dEW0 = PE - PW, dTB0 = PT - PB;
+      const R Cxy = xyE * ((PNE - PSE) + dNS0) - xyW * ((PNW - PSW) + dNS0) -
+                    xyN * ((PNE - PNW) + dEW0) + xyS * ((PSE - PSW) + dEW0);
+      const R Cyz = yzN * ((PNT - PNB) + dTB0) - yzS * ((PST - PSB) + dTB0) -
+                    yzT * ((PNT - PST) + dNS0) + yzB * ((PNB - PSB) + dNS0);
+      const R Czx = zxT * ((PET - PWT) + dEW0) - zxE * ((PET - PEB) + dTB0) +
+                    zxW * ((PWT - PWB) + dTB0) - zxB * ((PEB - PWB) + dEW0);
       AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
       const R om = (R)1 - PP;
       const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
@@ -417,7 +436,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R cl = pstar > (R)1 ? (R)1 : pstar;
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;
-      if (upd) pn[gidx] = pnew;
+      if (upd) at(pn + pk, ooff) = pnew;
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
         const R *ig = sgm, *mu = sgm + QPL;
@@ -425,7 +444,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
         if (own) {
-          cn[gidx] = cPP + inc;
+          at(cn + pk, ooff) = cPP + inc;
           macc += inc;
         }
       }
@@ -437,13 +456,13 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
           R lap = (R)0;
           if (s.Ddt != (R)0 && own) {
             const R TP2 = (R)2 * s.TP;
-            lap = (Tcur[gidx + 1] - TP2 + Tcur[gidx - 1]) * s.idx2 +
-                  (Tcur[gidx + nx] - TP2 + Tcur[gidx - nx]) * s.idy2 +
-                  (Tcur[gidx + sz] - TP2 + Tcur[gidx - sz]) * s.idz2;
+            const R *tc = &at(Tcur + pk, ooff);
+            lap = (tc[1] - TP2 + tc[-1]) * s.idx2 + (tc[nx] - TP2 + tc[-nx]) * s.idy2 +
+                  (tc[sz] - TP2 + tc[-(ptrdiff_t)sz]) * s.idz2;
           }
-          if (own) Tnext[gidx] = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
+          if (own) at(Tnext + pk, ooff) = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
         } else if (upd) {
-          dpp[gidx] = dppv;
+          at(dpp + pk, ooff) = dppv;
         }
       } else {
         (void)Tcur; (void)Tnext; (void)dpp; (void)write_T; (void)dppv;
@@ -539,8 +558,11 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
     else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
   }
   const int olx = tx + 1, oly = ty + 1;
-  const int ooff = (clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx;
-  const int roff = (clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx;
+  const uint32_t ooff =
+      (uint32_t)((clampi(i, 1, nx) - 1) + (clampi(j, 1, ny) - 1) * nx) * (uint32_t)sizeof(R);
+  const uint32_t roff =
+      (uint32_t)((clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx) *
+      (uint32_t)sizeof(R);
   const int oc = oly * QX + olx, rc = rly * QX + rlx;
 
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
@@ -551,11 +573,11 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   R pq = 0, cq = 0, rp = 0, rcv = 0;
   auto fetch = [&](int q, R &p, R &c, R &p2, R &c2) {
     const size_t pk = (size_t)(q - 1) * sz;
-    p = po[pk + ooff];
-    c = co[pk + ooff];
+    p = at(po + pk, ooff);
+    c = at(co + pk, ooff);
     if (tid < NRING) {
-      p2 = po[pk + roff];
-      c2 = co[pk + roff];
+      p2 = at(po + pk, roff);
+      c2 = at(co + pk, roff);
     }
   };
   fetch(q_first, pq, cq, rp, rcv);
@@ -575,7 +597,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
     __syncthreads();
     const int k = q - 1;  // cell plane updated now
     if (q >= ks + 1 && own) {
-      const size_t gidx = (size_t)(k - 1) * sz + ooff;
+      const size_t pk = (size_t)(k - 1) * sz;
       const R PP = pZ, cPP = cZ;
       const int nc = oc;
       const R *ig = sgm + (k % 3) * 2 * QPL, *mu = ig + QPL;
@@ -583,7 +605,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const R igc = igZ, muc = muZ;
       const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
-      cn[gidx] = cPP + inc;
+      at(cn + pk, ooff) = cPP + inc;
       macc += inc;
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;

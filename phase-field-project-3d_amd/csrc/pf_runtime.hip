@@ -257,8 +257,9 @@ int pf_init(const pf_params *p, const double *phi_old, const double *phi_new, co
     return fail(PF_E_ARG, "grid %dx%dx%d: each extent must be >= 5", p->nx, p->ny, p->nz);
   if (p->precision != PF_FP64 && p->precision != PF_FP32_MIXED)
     return fail(PF_E_ARG, "unknown precision %d", p->precision);
-  if ((long long)p->nx * p->ny > (1LL << 30))
-    return fail(PF_E_ARG, "plane %dx%d too large", p->nx, p->ny);
+  // the sweeps address a plane of a field with 32-bit byte offsets
+  if ((long long)p->nx * p->ny * (p->precision == PF_FP64 ? 8 : 4) >= (1LL << 32))
+    return fail(PF_E_ARG, "plane %dx%d too large (a plane of a field must be < 4 GiB)", p->nx, p->ny);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(PF_E_NODEV, "no HIP device visible");
@@ -571,6 +572,25 @@ int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]) {
   out[1] = ctx->fused.tile_b;
   out[2] = ctx->fused.seg_a;
   out[3] = ctx->fused.seg_b;
+  return PF_OK;
+}
+
+int pf_set_sweep_tiling(pf_ctx *ctx, int32_t kind, int32_t tile, int32_t target_groups) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (pf::fused_set_kind(ctx->fused, kind, tile, target_groups) != PF_OK)
+    return fail(PF_E_ARG, "kind must be 0..2, tile shape in [0,%d), target_groups >= 1",
+                pf::PF_NVARIANTS);
+  return PF_OK;
+}
+
+int pf_get_sweep_tiling(const pf_ctx *ctx, int32_t kind, int32_t out[3]) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail(PF_E_ARG, "null output");
+  int v[3];
+  if (pf::fused_get_kind(ctx->fused, kind, v) != PF_OK) return fail(PF_E_ARG, "kind must be 0..2");
+  for (int q = 0; q < 3; ++q) out[q] = v[q];
   return PF_OK;
 }
 

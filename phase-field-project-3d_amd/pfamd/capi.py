@@ -74,7 +74,7 @@ EXPORTS = (
     "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
     "pf_checkpoint_save", "pf_checkpoint_load", "pf_set_sweep_split",
-    "pf_get_sweep_split",
+    "pf_get_sweep_split", "pf_set_sweep_tiling", "pf_get_sweep_tiling",
     "pf_batch_create", "pf_batch_destroy", "pf_batch_add", "pf_batch_remove", "pf_batch_run",
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
@@ -123,6 +123,8 @@ def load(path: str = LIB_PATH):
         "pf_set_mass_correction": ([vp, i32], C.c_int),
         "pf_set_sweep_split": ([vp, i32, i32], C.c_int),
         "pf_get_sweep_split": ([vp, vp], C.c_int),
+        "pf_set_sweep_tiling": ([vp, i32, i32, i32], C.c_int),
+        "pf_get_sweep_tiling": ([vp, i32, vp], C.c_int),
         "pf_checkpoint_save": ([vp, P, L, C.c_char_p], C.c_int),
         "pf_checkpoint_load": ([C.c_char_p, P, L, C.POINTER(vp)], C.c_int),
         "pf_batch_create": ([C.c_size_t, C.POINTER(vp)], C.c_int),
@@ -290,6 +292,18 @@ class Context:
         out = np.zeros(3, dtype=np.int32)
         check(self.lib.pf_get_sweep_split(self.h, out.ctypes.data_as(C.c_void_p)), self.lib)
         return (bool(out[0]), int(out[1]), int(out[2]))
+
+    def set_sweep_tiling(self, kind: int, tile: int, target: int):
+        """Tile shape and work-group target of kernel kind 0 (sweep A), 1
+        (sweep B / its phase + T kernel) or 2 (the split form's solute kernel)."""
+        check(self.lib.pf_set_sweep_tiling(self.h, int(kind), int(tile), int(target)), self.lib)
+
+    def sweep_tiling(self, kind: int):
+        """(tile, target, z-planes per segment) of kernel kind 0, 1 or 2."""
+        out = np.zeros(3, dtype=np.int32)
+        check(self.lib.pf_get_sweep_tiling(self.h, int(kind), out.ctypes.data_as(C.c_void_p)),
+              self.lib)
+        return tuple(int(v) for v in out)
 
     def tiling(self):
         """(tile_a, tile_b, z-planes per segment A, B)."""
