@@ -135,18 +135,21 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
             "autotune": cands}
 
 
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_fp64_256x256x528_pmc.md")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_256x256x528_pmc.md")
 
 
-def pmc_traffic(kernel="sweep_B_fp64"):
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy kernel),
-    or None when the summary is absent."""
+def pmc_traffic(kernel):
+    """HBM bytes per launch of a kernel ('sweep_B_fp64', ...; the fused
+    sweeps the summary was taken on) from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy kernel), or None when
+    the summary does not cover it (the split form's kernels, say)."""
     import re
 
     try:
         txt = open(PMC_SUMMARY).read()
     except OSError:
+        return None
+    if kernel is None:
         return None
     m = re.search(rf"^traffic_bytes_per_launch: {kernel} (\d+)", txt, re.M)
     return int(m.group(1)) if m else None
@@ -384,11 +387,13 @@ def main(argv=None):
                    "grid": [NX, NY, NZ], "cells": ncell, "form": "FP64 drop-in (fused sweeps)"},
         "roofline": {"bound": "hbm", "achieved": round(rl["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(),
+                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(
+                         None if rl["kernel"] == "sweep_B" and r64["split"][0]
+                         else rl["kernel"] + "_fp64"),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
                                      "1.0, factors from a copy of known size)",
-                     "limiter": "VALU issue: 536 VALU instructions per cell at ~49% issue "
-                                "utilisation with 2 waves/SIMD (FP64 register demand)",
+                     "limiter": "VALU issue: 483 VALU instructions per cell (sweep B) at ~53% "
+                                "issue utilisation with 2 waves/SIMD (FP64 register demand)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,
