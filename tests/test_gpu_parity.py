@@ -235,6 +235,47 @@ def test_tiling_does_not_change_results(precision):
     sim.close()
 
 
+def test_per_kind_tiling():
+    """Tile shape and work-group target set per kernel kind (sweep A, sweep
+    B, the split form's solute kernel) are read back as set, with the z
+    segment derived from the target, and change no result; bad kinds and
+    shapes are rejected."""
+    deck = _growth_deck(n=(50, 37, 44))
+    print("[tiling] per kind", flush=True)
+    ref = None
+    for kinds, split in ((((1, 2048), (2, 1024), (0, 2048)), False),
+                         (((3, 256), (0, 8192), (2, 64)), True),
+                         (((0, 1), (3, 4096), (3, 4096)), True)):
+        sim = Simulation(deck, precision=capi.PF_FP64)
+        for q, (tile, target) in enumerate(kinds):
+            sim.ctx.set_sweep_tiling(q, tile, target)
+        sim.ctx.set_sweep_split(split, kinds[2][0])
+        for q, (tile, target) in enumerate(kinds):
+            t, g, seg = sim.ctx.sweep_tiling(q)
+            assert (t, g) == (tile, target) and seg >= 1
+        assert sim.ctx.sweep_split()[:2] == (split, kinds[2][0])
+        ta, tb, sa, sb = sim.ctx.tiling()
+        assert (ta, tb) == (kinds[0][0], kinds[1][0])
+        assert (sa, sb) == (sim.ctx.sweep_tiling(0)[2], sim.ctx.sweep_tiling(1)[2])
+        sim.run(9)
+        out = sim.fields()
+        sim.close()
+        if ref is None:
+            ref = out
+            continue
+        for nm in ("phi_new", "c_new", "T_new", "dpp"):
+            assert np.array_equal(out[nm], ref[nm]), (kinds, nm)
+    sim = Simulation(deck, precision=capi.PF_FP64)
+    for bad in ((3, 0, 1024), (0, 4, 1024), (1, 0, 0), (-1, 0, 1024)):
+        with pytest.raises(capi.PfError):
+            sim.ctx.set_sweep_tiling(*bad)
+    with pytest.raises(capi.PfError):
+        sim.ctx.sweep_tiling(3)
+    with pytest.raises(capi.PfError):
+        sim.ctx.set_sweep_split(True, 4)
+    sim.close()
+
+
 @pytest.mark.parametrize("precision", [capi.PF_FP64, capi.PF_FP32_MIXED])
 def test_split_and_fused_sweep_b_agree(precision):
     """Sweep B as one kernel and split into a phase + T kernel and a
