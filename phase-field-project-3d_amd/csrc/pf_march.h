@@ -42,7 +42,9 @@ struct Tile {
   static constexpr int TX = TX_, TY = TY_, NT = TX * TY;
   static constexpr int PX = TX + 4, PY = TY + 4, PPL = PX * PY;  // phi plane, halo 2
   static constexpr int QX = TX + 2, QY = TY + 2, QPL = QX * QY;  // node plane, halo 1
-  static constexpr int NRING = 2 * QX + 2 * TY;                  // ring nodes per plane
+  // ring nodes per plane: the halo nodes east/west/north/south of the tile's
+  // edge cells (the four corner nodes of the halo are no cell's neighbour)
+  static constexpr int NRING = 2 * TX + 2 * TY;
   static constexpr int NLD = (PPL + NT - 1) / NT;                 // phi loads per thread
   static_assert(NRING <= NT, "one ring node per thread at most");
   static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
@@ -282,9 +284,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // ring node handled by this thread (tid < NRING): its node-frame coords
   int rlx = 0, rly = 0;
   if (tid < NRING) {
-    if (tid < QX) { rlx = tid; rly = 0; }
-    else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
-    else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
+    if (tid < TX) { rlx = tid + 1; rly = 0; }
+    else if (tid < 2 * TX) { rlx = tid - TX + 1; rly = QY - 1; }
+    else { const int r = tid - 2 * TX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
   }
   const int olx = tx + 1, oly = ty + 1;  // own node-frame coords
 
@@ -553,9 +555,9 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   const int quint = kp.h_quintico;
   int rlx = 0, rly = 0;
   if (tid < NRING) {
-    if (tid < QX) { rlx = tid; rly = 0; }
-    else if (tid < 2 * QX) { rlx = tid - QX; rly = QY - 1; }
-    else { const int r = tid - 2 * QX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
+    if (tid < TX) { rlx = tid + 1; rly = 0; }
+    else if (tid < 2 * TX) { rlx = tid - TX + 1; rly = QY - 1; }
+    else { const int r = tid - 2 * TX; rly = 1 + (r >> 1); rlx = (r & 1) ? QX - 1 : 0; }
   }
   const int olx = tx + 1, oly = ty + 1;
   const uint32_t ooff =
