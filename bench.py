@@ -138,11 +138,22 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "fused_sweeps_256x256x528_pmc.md")
 
 
+def traffic_key(res, kernel, prec):
+    """The summary's key of a timed kernel: its kind (the split form's
+    phase + T kernel is `sweep_PT`), precision and tile shape."""
+    from pfamd.capi import TILE_SHAPES
+
+    ta, tb = res["tiling"][0], res["tiling"][1]
+    split, ts = res["split"][0], res["split"][1]
+    kind, tile = {"sweep_A": ("sweep_A", ta), "solute": ("solute", ts),
+                  "sweep_B": ("sweep_PT" if split else "sweep_B", tb)}[kernel]
+    return f"{kind}_{prec}_" + "x".join(map(str, TILE_SHAPES[tile]))
+
+
 def pmc_traffic(kernel):
-    """HBM bytes per launch of a kernel ('sweep_B_fp64', ...; the fused
-    sweeps the summary was taken on) from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy kernel), or None when
-    the summary does not cover it (the split form's kernels, say)."""
+    """HBM bytes per launch of a kernel (a traffic_key) from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated on a copy kernel), or
+    None when the summary does not cover that kernel and tile shape."""
     import re
 
     try:
@@ -387,13 +398,13 @@ def main(argv=None):
                    "grid": [NX, NY, NZ], "cells": ncell, "form": "FP64 drop-in (fused sweeps)"},
         "roofline": {"bound": "hbm", "achieved": round(rl["achieved_gbs"], 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(
-                         None if rl["kernel"] == "sweep_B" and r64["split"][0]
-                         else rl["kernel"] + "_fp64"),
+                     "frac": round(rl["achieved_gbs"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(traffic_key(r64, rl["kernel"], "fp64")),
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
                                      "1.0, factors from a copy of known size)",
-                     "limiter": "VALU issue: 483 VALU instructions per cell (sweep B) at ~53% "
-                                "issue utilisation with 2 waves/SIMD (FP64 register demand)",
+                     "traffic_kernel": traffic_key(r64, rl["kernel"], "fp64"),
+                     "limiter": "VALU issue: 260-480 VALU instructions per cell per sweep at "
+                                "46-53% issue utilisation with 2 waves/SIMD (FP64 register "
+                                "demand)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,
@@ -420,6 +431,8 @@ def main(argv=None):
                              "speedup_vs_fp64": round(m32 / mlups, 3),
                              "roofline_kernel": rl32["kernel"],
                              "roofline_achieved_gbs": round(rl32["achieved_gbs"], 1),
+                             "roofline_traffic": pmc_traffic(traffic_key(r32, rl32["kernel"],
+                                                                         "fp32")),
                              "tiling": tiling_summary(r32),
                              "diag_final": r32["diag"]}
     if not a.no_fp32 and not a.no_precision_study:
