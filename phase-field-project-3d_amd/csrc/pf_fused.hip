@@ -27,18 +27,23 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
   return true;
 }
 
-// FP32 sweeps fit four waves per SIMD (<= 128 VGPRs); the hint keeps the
-// compiler there.  FP64 sweeps need more registers than three waves allow.
-template <typename R, int MODE, typename TL>
+// Occupancy hints: FP32 sweeps fit four waves per SIMD (<= 128 VGPRs); FP64
+// sweep A and the phase + T kernel fit three (<= 168) without the smooth
+// noise; the others take what they need (two).
+template <typename R, int MODE, typename TL, bool SMOOTH>
+constexpr int march_waves() {
+  return sizeof(R) == 4 ? 4 : (MODE != kSweepB && !SMOOTH ? 3 : 1);
+}
+template <typename R, int MODE, typename TL, bool SMOOTH>
 __global__ void __launch_bounds__(TL::NT)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
+    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, TL, SMOOTH>())))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
             R *__restrict__ pn, R *__restrict__ cn, const R *__restrict__ Tcur,
             R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot) {
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
-  march_tile<R, MODE, TL>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
-                          mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
+  march_tile<R, MODE, TL, SMOOTH>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
+                                  mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
 }
 
 // The solute update alone (split form of sweep B).
@@ -84,21 +89,34 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
   return seg;
 }
 
+template <typename R, typename TL, bool SMOOTH>
+void launch_march_v(int mode, const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
+                    const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
+                    double *mslot, hipStream_t st) {
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
+  const size_t lds = march_lds_bytes<R, TL>(mode);
+  if (mode == kSweepB)
+    hipLaunchKernelGGL((k_march<R, kSweepB, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+                       pn, cn, Tcur, Tnext, dpp, write_T, mslot);
+  else if (mode == kSweepPT)
+    hipLaunchKernelGGL((k_march<R, kSweepPT, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+                       pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
+  else
+    hipLaunchKernelGGL((k_march<R, kSweepA, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+                       pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
+}
+
 template <typename R, typename TL>
 hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
                           int write_T, double *mslot, hipStream_t st) {
-  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
-  const size_t lds = march_lds_bytes<R, TL>(mode);
-  if (mode == kSweepB)
-    hipLaunchKernelGGL((k_march<R, kSweepB, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
-                       Tcur, Tnext, dpp, write_T, mslot);
-  else if (mode == kSweepPT)
-    hipLaunchKernelGGL((k_march<R, kSweepPT, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
-                       Tcur, Tnext, dpp, write_T, nullptr);
+  // the smooth noise is compiled only into the kernels that need it
+  if (kp.AAA != 0.0 && kp.ruido_tipo == PF_RUIDO_SMOOTH)
+    launch_march_v<R, TL, true>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot,
+                                st);
   else
-    hipLaunchKernelGGL((k_march<R, kSweepA, TL>), grd, blk, lds, st, g, kp, s, po, co, pn, cn,
-                       Tcur, Tnext, dpp, write_T, nullptr);
+    launch_march_v<R, TL, false>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot,
+                                 st);
   return hipGetLastError();
 }
 

@@ -231,6 +231,13 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t boff) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
 
+// A scheduling fence in FP64 code: no instruction moves across it, which
+// bounds the registers live at once (phases of the sweep update).
+template <typename R>
+__device__ __forceinline__ void sched_fence() {
+  if constexpr (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
+}
+
 struct Geo {
   int nx, ny, nz;
   int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
@@ -250,7 +257,11 @@ enum : int {
 // With the solute (kSweepB) and mslot non-null, the work-group adds the FP64
 // sum of its cells' solute increments to *mslot (its own slot of the
 // solute-mass ledger, so no atomics and a fixed summation order).
-template <typename R, int MODE, typename TL>
+// SMOOTH: the kernel carries the smooth noise (ruido_tipo 2), whose eight
+// hashes and interpolation raise the register demand of every other path;
+// launches pick the variant from ruido_tipo (a SMOOTH = false kernel must not
+// be launched with the smooth noise on).
+template <typename R, int MODE, typename TL, bool SMOOTH = true>
 __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
                                            const R *__restrict__ po, const R *__restrict__ co,
                                            R *__restrict__ pn, R *__restrict__ cn,
@@ -380,6 +391,16 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     }
     if constexpr (ST) {
       const int k = q - 2;  // cell plane updated now
+      // the noise draw first, before the update's values are loaded: the
+      // smooth noise's eight hashes and interpolation then need no
+      // registers beside them (phase_body.inc:181-189; the draw depends on
+      // the step, not the sweep -- passing it from the first sweep to the
+      // last through HBM measured slower than hashing again)
+      float ran = 0.f;
+      if (noise)
+        ran = SMOOTH && kp.ruido_tipo == PF_RUIDO_SMOOTH ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                                                         : (float)hash_unit(hbase);
+      sched_fence<R>();
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = P0[c];
@@ -387,19 +408,15 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
       if constexpr (SWEEP_B) cPP = cZ; else cPP = at(co + pk, ooff);
-      const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
-      const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
-              PSW = P0[c - PX - 1];
-      const R PT = Pp[c], PB = Pm[c];
-      const R PNT = Pp[c + PX], PST = Pp[c - PX], PET = Pp[c + 1], PWT = Pp[c - 1];
-      const R PNB = Pm[c + PX], PSB = Pm[c - PX], PEB = Pm[c + 1], PWB = Pm[c - 1];
+      // In FP64 the update runs in four phases, each reading its own phi and
+      // node values from LDS after a scheduling fence: the 33 values are
+      // never all live at once, which keeps the kernel within the registers
+      // of three waves per SIMD.
       const int nc = oly * QX + olx;
       const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
+      const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
+      const R PT = Pp[c], PB = Pm[c];
       const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
-      const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
-      const R yzN = yz[nc + QX], yzS = yz[nc - QX];
-      const R zxE = zx[nc + 1], zxW = zx[nc - 1];
-      const R yzT = nP.yz, yzB = nM.yz, zxT = nP.zx, zxB = nM.zx;
       R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
              (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
              (nP.e2 * (PT - PP) + nM.e2 * (PB - PP)) * s.idz2;
@@ -408,12 +425,23 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // every node value multiplies the sum of two phi differences across it
       // (the same sum; 39 operations instead of 48)
       const R dNS0 = PN - PS, dEW0 = PE - PW, dTB0 = PT - PB;
+      sched_fence<R>();
+      const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
+              PSW = P0[c - PX - 1];
+      const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
       const R Cxy = xyE * ((PNE - PSE) + dNS0) - xyW * ((PNW - PSW) + dNS0) -
                     xyN * ((PNE - PNW) + dEW0) + xyS * ((PSE - PSW) + dEW0);
+      sched_fence<R>();
+      const R PNT = Pp[c + PX], PST = Pp[c - PX], PNB = Pm[c + PX], PSB = Pm[c - PX];
+      const R yzN = yz[nc + QX], yzS = yz[nc - QX], yzT = nP.yz, yzB = nM.yz;
       const R Cyz = yzN * ((PNT - PNB) + dTB0) - yzS * ((PST - PSB) + dTB0) -
                     yzT * ((PNT - PST) + dNS0) + yzB * ((PNB - PSB) + dNS0);
+      sched_fence<R>();
+      const R PET = Pp[c + 1], PWT = Pp[c - 1], PEB = Pm[c + 1], PWB = Pm[c - 1];
+      const R zxE = zx[nc + 1], zxW = zx[nc - 1], zxT = nP.zx, zxB = nM.zx;
       const R Czx = zxT * ((PET - PWT) + dEW0) - zxE * ((PET - PEB) + dTB0) +
                     zxW * ((PWT - PWB) + dTB0) - zxB * ((PEB - PWB) + dEW0);
+      sched_fence<R>();
       AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
       const R om = (R)1 - PP;
       const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
@@ -425,11 +453,6 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
       R ruidos = (R)0;
       if (noise) {
-        // the draw depends on the step, not the sweep; passing it from the
-        // first sweep to the last through HBM measured slower than hashing
-        // again (an unprefetched load inside the march)
-        const float ran = kp.ruido_tipo == PF_RUIDO_SMOOTH ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
-                                                           : (float)hash_unit(hbase);
         const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
                                                               : ((R)2 * (R)ran - (R)1) * s.AAA;
         ruidos = amp * (R)16 * (PP * PP) * (om * om);
