@@ -27,12 +27,14 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
   return true;
 }
 
-// Occupancy hints: FP32 sweeps fit four waves per SIMD (<= 128 VGPRs); FP64
-// sweep A and the phase + T kernel fit three (<= 168) without the smooth
-// noise; the others take what they need (two).
+// Occupancy hints (waves per SIMD the register allocation must allow):
+// without the smooth noise, FP32 sweep A and phase + T fit six (<= 80
+// VGPRs), the fused FP32 sweep B five (<= 96), FP64 sweep A and phase + T
+// three (<= 168); with it, FP32 fits four; FP64 sweep B takes what it needs.
 template <typename R, int MODE, typename TL, bool SMOOTH>
 constexpr int march_waves() {
-  return sizeof(R) == 4 ? 4 : (MODE != kSweepB && !SMOOTH ? 3 : 1);
+  if (sizeof(R) == 4) return SMOOTH ? 4 : (MODE == kSweepB ? 5 : 6);
+  return MODE != kSweepB && !SMOOTH ? 3 : 1;
 }
 template <typename R, int MODE, typename TL, bool SMOOTH>
 __global__ void __launch_bounds__(TL::NT)

@@ -172,27 +172,36 @@ template <>
 __device__ __forceinline__ double dmu<double>(double qX, double qP) { return qX - qP; }
 template <>
 __device__ __forceinline__ float dmu<float>(float qX, float qP) {
-#pragma clang fp contract(off)
   // ln r, r = CL_X (1-CL_P) / (CL_P (1-CL_X)), as 2 atanh(t) with
   // t = (r-1)/(r+1) = (CL_X - CL_P) / ((CL_X + CL_P) - 2 CL_X CL_P).  The
   // denominator is symmetric as computed and the numerator flips sign
   // exactly, so t(X,P) = -t(P,X); the odd series and the log difference
   // below keep dmu(X,P) = -dmu(P,X) bit for bit: a face's flux is the same
   // from both sides (and carrying it down the column changes nothing).
-  const float t = (qX - qP) * rcpn<float>((qX + qP) - 2.0f * (qX * qP));
+  const float sum = qX + qP, prod = qX * qP;  // each symmetric
+  const float t = (qX - qP) * rcpn<float>(sum - 2.0f * prod);
   const float a = fabsf(t), t2 = t * t;
-  // 2 atanh(t) = 2t (1 + t^2/3 + t^4/5 + ...): truncation < 1e-10 relative for |t| < 1/8
+  // 2 atanh(t) = 2t (1 + t^2/3 + t^4/5 + ...): truncation < 1e-10 relative
+  // for |t| < 1/8; a polynomial in t^2 (even) times 2t (odd), with explicit
+  // FMAs so that every kernel rounds it alike
   const float ser =
-      2.0f * t * (1.0f + t2 * (1.0f / 3.0f + t2 * (0.2f + t2 * (1.0f / 7.0f + t2 * (1.0f / 9.0f)))));
+      (2.0f * t) *
+      fmaf(t2, fmaf(t2, fmaf(t2, fmaf(t2, 1.0f / 9.0f, 1.0f / 7.0f), 0.2f), 1.0f / 3.0f), 1.0f);
   // large jumps of CL: ln 2 (log2(CL_X (1-CL_P)) - log2(CL_P (1-CL_X))), the
   // same two products in swapped roles, so also exactly odd (and ln r >= 0.25
   // here, where the difference of logarithms loses nothing that matters).
   // The base-2 logarithms are subtracted before the one scaling by ln 2: with
   // two natural logarithms the compiler may fuse one scaling into the
   // subtraction (an FMA), which breaks the exact sign flip.
-  const float lg = (__builtin_amdgcn_logf(qX * (1.0f - qP)) -
-                    __builtin_amdgcn_logf(qP * (1.0f - qX))) * 0.693147180559945309f;
-  return a < 0.125f ? ser : lg;
+  // Only waves with such a face evaluate it (near-uniform regions, most of
+  // the grid, have none); which form a lane uses depends on |t| alone.
+  float r = ser;
+  if (__builtin_amdgcn_ballot_w64(a >= 0.125f) != 0) {
+    const float lg = (__builtin_amdgcn_logf(qX * (1.0f - qP)) -
+                      __builtin_amdgcn_logf(qP * (1.0f - qX))) * 0.693147180559945309f;
+    r = a < 0.125f ? ser : lg;
+  }
+  return r;
 }
 
 // Solute increment of the cell at LDS index nc (conc_body.inc:92-144): face
