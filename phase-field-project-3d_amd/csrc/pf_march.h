@@ -154,7 +154,6 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
 // the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
 template <typename R>
 __device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &q) {
-#pragma clang fp contract(off)
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
@@ -208,13 +207,12 @@ __device__ __forceinline__ float dmu<float>(float qX, float qP) {
 // mobility 2 D_C/(1/g_P + 1/g_X), flux along the q differences.  The flux
 // through the face below is the one through the face above the cell beneath,
 // negated (exact: dmu is odd and a + b = b + a), except for the first cell
-// of a z-segment (fresh).  Contraction is off so that the kernels that inline
-// this in different surroundings round it the same way.
+// of a z-segment (fresh): the carried flux is a product rounded on its own
+// (fzu, a statement of its own, is never fused into a sum).
 template <typename R>
 __device__ __forceinline__ R solute_inc(const R *ig, const R *mu, int nc, int qx, R igc, R muc,
                                         R igP, R muP, R igM, R muM, bool fresh, R &fzu_prev,
                                         R D2dt, const SC<R> &s) {
-#pragma clang fp contract(off)
   const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
                dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
   const R fy = dmu<R>(mu[nc + qx], muc) * rcpn<R>(ig[nc + qx] + igc) +
