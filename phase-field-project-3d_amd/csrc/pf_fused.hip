@@ -27,18 +27,9 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
   return true;
 }
 
-// Occupancy hints (waves per SIMD the register allocation must allow):
-// without the smooth noise, FP32 sweep A and phase + T fit six (<= 80
-// VGPRs), the fused FP32 sweep B five (<= 96), FP64 sweep A and phase + T
-// three (<= 168); with it, FP32 fits four; FP64 sweep B takes what it needs.
-template <typename R, int MODE, typename TL, bool SMOOTH>
-constexpr int march_waves() {
-  if (sizeof(R) == 4) return SMOOTH ? 4 : (MODE == kSweepB ? 5 : 6);
-  return MODE != kSweepB && !SMOOTH ? 3 : 1;
-}
 template <typename R, int MODE, typename TL, bool SMOOTH>
 __global__ void __launch_bounds__(TL::NT)
-    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, TL, SMOOTH>())))
+    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, SMOOTH>())))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
             R *__restrict__ pn, R *__restrict__ cn, const R *__restrict__ Tcur,
             R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot) {

@@ -264,6 +264,17 @@ enum : int {
 // With the solute (kSweepB) and mslot non-null, the work-group adds the FP64
 // sum of its cells' solute increments to *mslot (its own slot of the
 // solute-mass ledger, so no atomics and a fixed summation order).
+// Occupancy hints for the kernels around march_tile (waves per SIMD the
+// register allocation must allow): without the smooth noise, FP32 sweep A
+// and phase + T fit six (<= 80 VGPRs), the fused FP32 sweep B five (<= 96),
+// FP64 sweep A and phase + T three (<= 168); with it, FP32 fits four; FP64
+// sweep B takes what it needs.
+template <typename R, int MODE, bool SMOOTH>
+constexpr int march_waves() {
+  if (sizeof(R) == 4) return SMOOTH ? 4 : (MODE == kSweepB ? 5 : 6);
+  return MODE != kSweepB && !SMOOTH ? 3 : 1;
+}
+
 // SMOOTH: the kernel carries the smooth noise (ruido_tipo 2), whose eight
 // hashes and interpolation raise the register demand of every other path;
 // launches pick the variant from ruido_tipo (a SMOOTH = false kernel must not
