@@ -103,9 +103,9 @@ __global__ void k_batch_end(InstDev<R> *__restrict__ inst, int n) {
 
 // ---- batched sweeps and decorations ----------------------------------------
 
-template <typename R, bool SWEEP_B, typename TL, bool SMOOTH>
+template <typename R, bool SWEEP_B, typename TL, int VAR>
 __global__ void __launch_bounds__(TL::NT)
-    __attribute__((amdgpu_waves_per_eu(march_waves<R, SWEEP_B ? kSweepB : kSweepA, SMOOTH>())))
+    __attribute__((amdgpu_waves_per_eu(march_waves<R, SWEEP_B ? kSweepB : kSweepA, VAR>())))
     k_march_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
                   const InstDev<R> *__restrict__ inst, int sweep) {
   const int b = blockIdx.x;
@@ -116,11 +116,11 @@ __global__ void __launch_bounds__(TL::NT)
   if (!I.active) return;
   const int pin = I.po ^ ((sweep - 1) & 1), pout = pin ^ 1;
   if constexpr (SWEEP_B)
-    march_tile<R, kSweepB, TL, SMOOTH>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
+    march_tile<R, kSweepB, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        I.c[I.co ^ 1], I.T[I.to], I.T[I.to ^ 1], nullptr, 1,
                                        nullptr, d.bx, d.by, d.sg);
   else
-    march_tile<R, kSweepA, TL, SMOOTH>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
+    march_tile<R, kSweepA, TL, VAR>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, d.bx,
                                        d.by, d.sg);
 }
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ ch
   }
 }
 
-template <typename R, typename TL, bool SMOOTH>
+template <typename R, typename TL, int VAR>
 void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, const InstDev<R> *inst,
                           int sweep, hipStream_t st) {
   const int per_xcd = (ntiles + 7) / 8;
@@ -156,34 +156,40 @@ void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, const 
       sizeof(R) * (size_t)(5 * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
   if (sweep_b)
-    hipLaunchKernelGGL((k_march_batch<R, true, TL, SMOOTH>), grd, blk, lds, st, tiles, ntiles,
+    hipLaunchKernelGGL((k_march_batch<R, true, TL, VAR>), grd, blk, lds, st, tiles, ntiles,
                        per_xcd, inst, sweep);
   else
-    hipLaunchKernelGGL((k_march_batch<R, false, TL, SMOOTH>), grd, blk, lds, st, tiles, ntiles,
-                       per_xcd, inst, sweep);
+    hipLaunchKernelGGL((k_march_batch<R, false, TL, VAR & kVarSmooth>), grd, blk, lds, st, tiles,
+                       ntiles, per_xcd, inst, sweep);
 }
 
-// smooth: some instance of the group runs the smooth noise (the kernels
-// without it must not see such an instance)
+// var: the optional paths (kVar*) some instance of the group needs (the
+// kernels without a path must not see an instance that needs it)
 template <typename R, typename TL>
-hipError_t launch_batch_march(bool sweep_b, bool smooth, const TileRef *tiles, int ntiles,
+hipError_t launch_batch_march(bool sweep_b, int var, const TileRef *tiles, int ntiles,
                               const InstDev<R> *inst, int sweep, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
-  if (smooth)
-    launch_batch_march_v<R, TL, true>(sweep_b, tiles, ntiles, inst, sweep, st);
-  else
-    launch_batch_march_v<R, TL, false>(sweep_b, tiles, ntiles, inst, sweep, st);
+  switch (var) {
+    case 0: launch_batch_march_v<R, TL, 0>(sweep_b, tiles, ntiles, inst, sweep, st); break;
+    case kVarSmooth:
+      launch_batch_march_v<R, TL, kVarSmooth>(sweep_b, tiles, ntiles, inst, sweep, st);
+      break;
+    case kVarTDiff:
+      launch_batch_march_v<R, TL, kVarTDiff>(sweep_b, tiles, ntiles, inst, sweep, st);
+      break;
+    default: launch_batch_march_v<R, TL, kVarAll>(sweep_b, tiles, ntiles, inst, sweep, st);
+  }
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_batch_shape(int shape, bool sweep_b, bool smooth, const TileRef *tiles,
-                              int ntiles, const InstDev<R> *inst, int sweep, hipStream_t st) {
+hipError_t launch_batch_shape(int shape, bool sweep_b, int var, const TileRef *tiles, int ntiles,
+                              const InstDev<R> *inst, int sweep, hipStream_t st) {
   switch (shape) {
-    case 1: return launch_batch_march<R, Tile32x8>(sweep_b, smooth, tiles, ntiles, inst, sweep, st);
-    case 2: return launch_batch_march<R, Tile32x16>(sweep_b, smooth, tiles, ntiles, inst, sweep, st);
-    case 3: return launch_batch_march<R, Tile16x16>(sweep_b, smooth, tiles, ntiles, inst, sweep, st);
-    default: return launch_batch_march<R, Tile64x8>(sweep_b, smooth, tiles, ntiles, inst, sweep, st);
+    case 1: return launch_batch_march<R, Tile32x8>(sweep_b, var, tiles, ntiles, inst, sweep, st);
+    case 2: return launch_batch_march<R, Tile32x16>(sweep_b, var, tiles, ntiles, inst, sweep, st);
+    case 3: return launch_batch_march<R, Tile16x16>(sweep_b, var, tiles, ntiles, inst, sweep, st);
+    default: return launch_batch_march<R, Tile64x8>(sweep_b, var, tiles, ntiles, inst, sweep, st);
   }
 }
 
@@ -252,7 +258,8 @@ struct pf_batch {
   size_t chunks_cap = 0;
   struct Group {
     int prec, shape;
-    bool sweep_b, smooth;  // smooth: an instance of the group runs the smooth noise
+    bool sweep_b;
+    int var;  // kernel variant: the optional paths an instance of the group needs
     size_t off;
     int n;
   };
@@ -365,12 +372,13 @@ int rebuild(pf_batch *b) {
     for (int sweep_b = 0; sweep_b < 2; ++sweep_b)
       for (int shape = 0; shape < pf::kNShapes; ++shape) {
         const size_t off = tiles.size();
-        bool smooth = false;
+        int var = 0;
         for (size_t q = 0; q < ids.size(); ++q) {
           const pf_batch_slot &s = b->slots[ids[q]];
           const int cls = pf::size_class(s.prec, s.nx, s.ny);
           if (b->cls_shape[cls] != shape) continue;
-          smooth = smooth || (s.p.AAA != 0.0 && s.p.ruido_tipo == PF_RUIDO_SMOOTH);
+          if (s.p.AAA != 0.0 && s.p.ruido_tipo == PF_RUIDO_SMOOTH) var |= pf::kVarSmooth;
+          if (s.p.D != 0.0) var |= pf::kVarTDiff;
           pf::InstDev<double> tmp;  // geometry only
           set_geo(tmp, *b, s);
           const pf::Geo &g = sweep_b ? tmp.gb : tmp.ga;
@@ -379,7 +387,7 @@ int rebuild(pf_batch *b) {
               for (int bx = 0; bx < g.nbx; ++bx) tiles.push_back({(int)q, bx, by, sg});
         }
         const int n = (int)(tiles.size() - off);
-        if (n) b->groups.push_back({prec, shape, sweep_b != 0, smooth, off, n});
+        if (n) b->groups.push_back({prec, shape, sweep_b != 0, var, off, n});
       }
     for (int which = 0; which < 2; ++which) {
       const size_t off = chunks.size();
@@ -459,10 +467,10 @@ int enqueue_step(pf_batch *b) {
       if (g.sweep_b != sweep_b) continue;
       hipError_t e;
       if (g.prec == PF_FP64)
-        e = pf::launch_batch_shape<double>(g.shape, sweep_b, g.smooth, b->tiles_dev + g.off, g.n,
+        e = pf::launch_batch_shape<double>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
                                            (const pf::InstDev<double> *)b->inst_dev[0], sweep, st);
       else
-        e = pf::launch_batch_shape<float>(g.shape, sweep_b, g.smooth, b->tiles_dev + g.off, g.n,
+        e = pf::launch_batch_shape<float>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
                                           (const pf::InstDev<float> *)b->inst_dev[1], sweep, st);
       if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
     }
@@ -979,11 +987,11 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
         for (const auto &g : b->groups) {
           if (e != hipSuccess) break;
           if (g.prec == PF_FP64)
-            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, g.smooth, b->tiles_dev + g.off, g.n,
+            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
                                                (const pf::InstDev<double> *)b->inst_dev[0], 1,
                                                b->stream);
           else
-            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, g.smooth, b->tiles_dev + g.off, g.n,
+            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
                                               (const pf::InstDev<float> *)b->inst_dev[1], 1,
                                               b->stream);
         }

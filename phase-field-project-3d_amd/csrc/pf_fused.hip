@@ -27,15 +27,15 @@ __device__ __forceinline__ bool tile_of(const Geo &g, int &bx, int &by, int &sg)
   return true;
 }
 
-template <typename R, int MODE, typename TL, bool SMOOTH>
+template <typename R, int MODE, typename TL, int VAR>
 __global__ void __launch_bounds__(TL::NT)
-    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, SMOOTH>())))
+    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, VAR>())))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
             R *__restrict__ pn, R *__restrict__ cn, const R *__restrict__ Tcur,
             R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot) {
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
-  march_tile<R, MODE, TL, SMOOTH>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
+  march_tile<R, MODE, TL, VAR>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
                                   mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
 }
 
@@ -82,20 +82,21 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
   return seg;
 }
 
-template <typename R, typename TL, bool SMOOTH>
+template <typename R, typename TL, int VAR>
 void launch_march_v(int mode, const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
                     const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
                     double *mslot, hipStream_t st) {
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
   const size_t lds = march_lds_bytes<R, TL>(mode);
   if (mode == kSweepB)
-    hipLaunchKernelGGL((k_march<R, kSweepB, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+    hipLaunchKernelGGL((k_march<R, kSweepB, TL, VAR>), grd, blk, lds, st, g, kp, s, po, co,
                        pn, cn, Tcur, Tnext, dpp, write_T, mslot);
   else if (mode == kSweepPT)
-    hipLaunchKernelGGL((k_march<R, kSweepPT, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+    hipLaunchKernelGGL((k_march<R, kSweepPT, TL, VAR>), grd, blk, lds, st, g, kp, s, po, co,
                        pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
   else
-    hipLaunchKernelGGL((k_march<R, kSweepA, TL, SMOOTH>), grd, blk, lds, st, g, kp, s, po, co,
+    hipLaunchKernelGGL((k_march<R, kSweepA, TL, VAR & kVarSmooth>), grd, blk, lds, st, g, kp, s,
+                       po, co,
                        pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
 }
 
@@ -103,13 +104,18 @@ template <typename R, typename TL>
 hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
                           int write_T, double *mslot, hipStream_t st) {
-  // the smooth noise is compiled only into the kernels that need it
-  if (kp.AAA != 0.0 && kp.ruido_tipo == PF_RUIDO_SMOOTH)
-    launch_march_v<R, TL, true>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot,
-                                st);
-  else
-    launch_march_v<R, TL, false>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot,
-                                 st);
+  // the optional paths are compiled only into the kernels that need them
+  const int var = (kp.AAA != 0.0 && kp.ruido_tipo == PF_RUIDO_SMOOTH ? kVarSmooth : 0) |
+                  (kp.D != 0.0 ? kVarTDiff : 0);
+#define PF_VAR(V) \
+  launch_march_v<R, TL, V>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, st)
+  switch (var) {
+    case 0: PF_VAR(0); break;
+    case kVarSmooth: PF_VAR(kVarSmooth); break;
+    case kVarTDiff: PF_VAR(kVarTDiff); break;
+    default: PF_VAR(kVarAll); break;
+  }
+#undef PF_VAR
   return hipGetLastError();
 }
 

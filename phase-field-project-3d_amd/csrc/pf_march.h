@@ -264,22 +264,28 @@ enum : int {
 // With the solute (kSweepB) and mslot non-null, the work-group adds the FP64
 // sum of its cells' solute increments to *mslot (its own slot of the
 // solute-mass ledger, so no atomics and a fixed summation order).
+// Kernel variants (bits of VAR): paths that few decks take but whose
+// register demand would burden every other path are compiled only into the
+// kernels that need them; launches pick the variant from the deck (a kernel
+// must not be launched for a deck that needs a path it lacks).
+//   kVarSmooth: the smooth noise (ruido_tipo 2: eight hashes and an
+//               interpolation per cell);
+//   kVarTDiff:  the thermal Laplacian of the next step's T (D != 0; the
+//               reference's decks have D = 0).
+enum : int { kVarSmooth = 1, kVarTDiff = 2, kVarAll = 3 };
+
 // Occupancy hints for the kernels around march_tile (waves per SIMD the
-// register allocation must allow): without the smooth noise, FP32 sweep A
+// register allocation must allow): without the optional paths, FP32 sweep A
 // and phase + T fit six (<= 80 VGPRs), the fused FP32 sweep B five (<= 96),
-// FP64 sweep A and phase + T three (<= 168); with it, FP32 fits four; FP64
+// FP64 sweep A and phase + T three (<= 168); with them, FP32 fits four; FP64
 // sweep B takes what it needs.
-template <typename R, int MODE, bool SMOOTH>
+template <typename R, int MODE, int VAR>
 constexpr int march_waves() {
-  if (sizeof(R) == 4) return SMOOTH ? 4 : (MODE == kSweepB ? 5 : 6);
-  return MODE != kSweepB && !SMOOTH ? 3 : 1;
+  if (sizeof(R) == 4) return VAR != 0 ? 4 : (MODE == kSweepB ? 5 : 6);
+  return MODE != kSweepB && VAR == 0 ? 3 : 1;
 }
 
-// SMOOTH: the kernel carries the smooth noise (ruido_tipo 2), whose eight
-// hashes and interpolation raise the register demand of every other path;
-// launches pick the variant from ruido_tipo (a SMOOTH = false kernel must not
-// be launched with the smooth noise on).
-template <typename R, int MODE, typename TL, bool SMOOTH = true>
+template <typename R, int MODE, typename TL, int VAR = kVarAll>
 __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
                                            const R *__restrict__ po, const R *__restrict__ co,
                                            R *__restrict__ pn, R *__restrict__ cn,
@@ -416,8 +422,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // last through HBM measured slower than hashing again)
       float ran = 0.f;
       if (noise)
-        ran = SMOOTH && kp.ruido_tipo == PF_RUIDO_SMOOTH ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
-                                                         : (float)hash_unit(hbase);
+        ran = (VAR & kVarSmooth) && kp.ruido_tipo == PF_RUIDO_SMOOTH
+                  ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
+                  : (float)hash_unit(hbase);
       sched_fence<R>();
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
@@ -497,7 +504,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
           R hh, hl2;
           h_of(pnew, quint, hh, hl2);
           R lap = (R)0;
-          if (s.Ddt != (R)0 && own) {
+          if ((VAR & kVarTDiff) && s.Ddt != (R)0 && own) {
             const R TP2 = (R)2 * s.TP;
             const R *tc = &at(Tcur + pk, ooff);
             lap = (tc[1] - TP2 + tc[-1]) * s.idx2 + (tc[nx] - TP2 + tc[-nx]) * s.idy2 +

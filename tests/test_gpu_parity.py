@@ -180,6 +180,29 @@ def test_fused_vs_oracle(case, oracle_lib):
     sim.close()
 
 
+def test_fused_thermal_diffusion_matches_dropin():
+    """With D != 0 the fused sweeps run the kernel variant that carries the
+    thermal Laplacian of temp_body.inc (the decks' D = 0 variant drops it);
+    it agrees with the drop-in kernels, and the T field really diffuses."""
+    deck = _growth_deck(n=(40, 36, 44))
+    probe = Simulation(deck)
+    D = 0.05 * probe.p.dx ** 2 / probe.p.dt  # D dt / dx^2 = 0.05: stable
+    probe.close()
+    print(f"[thermal diffusion] D = {D:.3e}", flush=True)
+    out = []
+    for fused, d in ((True, D), (False, D), (True, 0.0)):
+        sim = Simulation(deck, D=d)
+        sim.ctx.set_fused(fused)
+        sim.run(20)
+        out.append(sim.fields())
+        sim.close()
+    (a, b, c) = out
+    assert _abs(a["phi_new"], b["phi_new"]) <= PHI_TOL_FUSED
+    assert _abs(a["c_new"], b["c_new"]) <= 1e-10
+    assert _abs(a["T_new"], b["T_new"]) <= 1e-6
+    assert _abs(a["T_new"], c["T_new"]) > 1e-6  # the Laplacian did something
+
+
 def test_fused_deterministic_and_matches_dropin():
     deck = _growth_deck(n=(70, 40, 50))
     print("[determinism]", flush=True)
