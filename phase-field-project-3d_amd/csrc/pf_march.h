@@ -238,13 +238,6 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t boff) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
 
-// A scheduling fence in FP64 code: no instruction moves across it, which
-// bounds the registers live at once (phases of the sweep update).
-template <typename R>
-__device__ __forceinline__ void sched_fence() {
-  if constexpr (sizeof(R) == 8) __builtin_amdgcn_sched_barrier(0);
-}
-
 struct Geo {
   int nx, ny, nz;
   int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
@@ -425,7 +418,6 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         ran = (VAR & kVarSmooth) && kp.ruido_tipo == PF_RUIDO_SMOOTH
                   ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
                   : (float)hash_unit(hbase);
-      sched_fence<R>();
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = P0[c];
@@ -433,10 +425,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
       if constexpr (SWEEP_B) cPP = cZ; else cPP = at(co + pk, ooff);
-      // In FP64 the update runs in four phases, each reading its own phi and
-      // node values from LDS after a scheduling fence: the 33 values are
-      // never all live at once, which keeps the kernel within the registers
-      // of three waves per SIMD.
+      // the Laplacian-like term, then the three cross terms, each reading
+      // its own phi and node values
       const int nc = oly * QX + olx;
       const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
       const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
@@ -450,23 +440,19 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // every node value multiplies the sum of two phi differences across it
       // (the same sum; 39 operations instead of 48)
       const R dNS0 = PN - PS, dEW0 = PE - PW, dTB0 = PT - PB;
-      sched_fence<R>();
       const R PNE = P0[c + PX + 1], PSE = P0[c - PX + 1], PNW = P0[c + PX - 1],
               PSW = P0[c - PX - 1];
       const R xyE = xy[nc + 1], xyW = xy[nc - 1], xyN = xy[nc + QX], xyS = xy[nc - QX];
       const R Cxy = xyE * ((PNE - PSE) + dNS0) - xyW * ((PNW - PSW) + dNS0) -
                     xyN * ((PNE - PNW) + dEW0) + xyS * ((PSE - PSW) + dEW0);
-      sched_fence<R>();
       const R PNT = Pp[c + PX], PST = Pp[c - PX], PNB = Pm[c + PX], PSB = Pm[c - PX];
       const R yzN = yz[nc + QX], yzS = yz[nc - QX], yzT = nP.yz, yzB = nM.yz;
       const R Cyz = yzN * ((PNT - PNB) + dTB0) - yzS * ((PST - PSB) + dTB0) -
                     yzT * ((PNT - PST) + dNS0) + yzB * ((PNB - PSB) + dNS0);
-      sched_fence<R>();
       const R PET = Pp[c + 1], PWT = Pp[c - 1], PEB = Pm[c + 1], PWB = Pm[c - 1];
       const R zxE = zx[nc + 1], zxW = zx[nc - 1], zxT = nP.zx, zxB = nM.zx;
       const R Czx = zxT * ((PET - PWT) + dEW0) - zxE * ((PET - PEB) + dTB0) +
                     zxW * ((PWT - PWB) + dTB0) - zxB * ((PEB - PWB) + dEW0);
-      sched_fence<R>();
       AA = AA + s.cxy * Cxy + s.cyz * Cyz + s.czx * Czx;
       const R om = (R)1 - PP;
       const R gl = (R)2 * PP * om * (om - PP);  // 2p(1-p)^2 - 2p^2(1-p)
