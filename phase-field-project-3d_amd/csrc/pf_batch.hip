@@ -152,8 +152,7 @@ template <typename R, typename TL, int VAR>
 void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, const InstDev<R> *inst,
                           int sweep, hipStream_t st) {
   const int per_xcd = (ntiles + 7) / 8;
-  const size_t lds =
-      sizeof(R) * (size_t)(5 * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
+  const size_t lds = march_lds_bytes<R, TL>(sweep_b ? kSweepB : kSweepA);
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
   if (sweep_b)
     hipLaunchKernelGGL((k_march_batch<R, true, TL, VAR>), grd, blk, lds, st, tiles, ntiles,

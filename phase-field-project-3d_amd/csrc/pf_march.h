@@ -50,15 +50,37 @@ struct Tile {
   static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
 };
 
-// log and 1/x in the arithmetic type.  FP64: the ~1 ulp log and the
-// one-Newton-step reciprocal (<= 11 ulp) of pf_math.h.  FP32: the hardware
-// log2 / reciprocal (about 1 ulp).
+// Elements of the arithmetic type in a march_tile work-group's LDS arrays:
+// NSLOT phi planes, four node planes and, in sweep B, two solute planes.
+template <typename TL>
+constexpr size_t march_lds_words(bool sweep_b) {
+  return (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
+}
+
+// log and 1/x in the arithmetic type.  FP64: the ~1 ulp logarithms and the
+// one-Newton-step reciprocal (<= 11 ulp) of pf_math.h; FP32: the hardware
+// log2 / reciprocal (about 1 ulp).  The FP64 logarithm comes in two forms
+// of the same accuracy: dlog, the table-driven one (its table in LDS, `lt`;
+// fewer instructions, one LDS read), for the solute's node potential, and
+// dlog_div, the division-based one (pure VALU), for the phase update's free
+// energy, where the LDS read's latency measured slower (the phase update
+// already waits on two dozen stencil reads) and the table form brought the
+// sweep kernels nothing.
 template <typename R>
-__device__ __forceinline__ R dlog(R x);
+__device__ __forceinline__ R dlog(R x, const LogEnt *lt);
 template <>
-__device__ __forceinline__ double dlog<double>(double x) { return log_fast(x); }
+__device__ __forceinline__ double dlog<double>(double x, const LogEnt *lt) {
+  return log_tab(x, lt);
+}
 template <>
-__device__ __forceinline__ float dlog<float>(float x) { return __logf(x); }
+__device__ __forceinline__ float dlog<float>(float x, const LogEnt *) { return __logf(x); }
+
+template <typename R>
+__device__ __forceinline__ R dlog_div(R x);
+template <>
+__device__ __forceinline__ double dlog_div<double>(double x) { return log_div(x); }
+template <>
+__device__ __forceinline__ float dlog_div<float>(float x) { return __logf(x); }
 
 template <typename R>
 __device__ __forceinline__ R rcpn(R x);
@@ -66,6 +88,35 @@ template <>
 __device__ __forceinline__ double rcpn<double>(double x) { return rcp_fast(x); }
 template <>
 __device__ __forceinline__ float rcpn<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// 1/x for a denominator that is positive and finite for finite inputs.
+template <typename R>
+__device__ __forceinline__ R rcpn_pos(R x);
+template <>
+__device__ __forceinline__ double rcpn_pos<double>(double x) { return rcp_pos(x); }
+template <>
+__device__ __forceinline__ float rcpn_pos<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// The FP64 logarithm's table in LDS: kLogTabN 16-byte entries placed after
+// a kernel's other LDS arrays (`words` elements of R from the start), filled
+// by the work-group before its first barrier.  The FP32 form has none.
+template <typename R>
+constexpr size_t log_tab_bytes() {
+  return sizeof(R) == 8 ? sizeof(LogEnt) * (size_t)kLogTabN : 0;
+}
+template <typename R>
+constexpr size_t log_tab_offset(size_t words) {
+  return (words * sizeof(R) + 15) & ~(size_t)15;
+}
+template <typename R>
+__device__ __forceinline__ const LogEnt *log_tab_fill(unsigned char *smem, size_t words,
+                                                      int tid, int nt) {
+  LogEnt *lt = reinterpret_cast<LogEnt *>(smem + log_tab_offset<R>(words));
+  if constexpr (sizeof(R) == 8) {
+    for (int t = tid; t < kLogTabN; t += nt) lt[t] = LogEnt{kLogTab[t][0], kLogTab[t][1]};
+  }
+  return lt;
+}
 
 // Scalars of one sweep, in the arithmetic type.
 template <typename R>
@@ -119,7 +170,7 @@ struct Node {
 template <typename R>
 __device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
   const R gx2 = gx * gx, gy2 = gy * gy, gz2 = gz * gz;
-  const R rq = rcpn<R>(gx2 + gy2 + gz2 + s.small);
+  const R rq = rcpn_pos<R>(gx2 + gy2 + gz2 + s.small);
   const R a = gx2 * rq, b = gy2 * rq, c = gz2 * rq;
   const R eps = s.E0a + s.E0b * (a * a + b * b + c * c);
   const R f = s.K * eps;
@@ -153,7 +204,8 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
 // difference and would random-walk c by ~1 ulp per step; the FP32 form takes
 // the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
 template <typename R>
-__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s, R &ig, R &q) {
+__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s,
+                                            const LogEnt *lt, R &ig, R &q) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
@@ -161,7 +213,7 @@ __device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
   ig = rcpn<R>(g);
-  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc));
+  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc), lt);
   else q = CL;
 }
 
@@ -308,6 +360,10 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   const size_t sz = (size_t)nx * ny;
   const int quint = kp.h_quintico;
   const bool noise = s.AAA != (R)0;
+  // visible to the whole work-group after the first plane's barrier, before
+  // any logarithm
+  const LogEnt *lt =
+      log_tab_fill<R>(smem_raw, march_lds_words<TL>(MODE == kSweepB), tid, NT);
 
   // ring node handled by this thread (tid < NRING): its node-frame coords
   int rlx = 0, rly = 0;
@@ -401,9 +457,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         const R *P0 = slot(q - 1);
         const size_t pk = (size_t)(kn - 1) * sz;
         cnew_col = at(co + pk, ooff);
-        solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, igP, muP);
+        solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, lt, igP, muP);
         if (tid < NRING)
-          solute_node(P0[(rly + 1) * PX + rlx + 1], at(co + pk, roff), quint, s, rig, rmu);
+          solute_node(P0[(rly + 1) * PX + rlx + 1], at(co + pk, roff), quint, s, lt, rig, rmu);
       }
     }
     if constexpr (ST) {
@@ -461,7 +517,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
       const R den = s.Ke * h + ((R)1 - h);
       const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
-      const R GG = hl * s.RTV * dlog<R>(fabs(arg)) - s.W0 * gl;
+      const R GG = hl * s.RTV * dlog_div<R>(fabs(arg)) - s.W0 * gl;
       R ruidos = (R)0;
       if (noise) {
         const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
@@ -554,8 +610,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
 // LDS bytes of a march_tile work-group.
 template <typename R, typename TL>
 constexpr size_t march_lds_bytes(int mode) {
-  return sizeof(R) *
-         (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (mode == kSweepB ? 2 * TL::QPL : 0));
+  return log_tab_offset<R>(march_lds_words<TL>(mode == kSweepB)) + log_tab_bytes<R>();
 }
 
 // One work-group's tile and z-segment of the solute update alone
@@ -587,6 +642,8 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   const int ke = min(ks + g.seg - 1, g.k_hi);
   const size_t sz = (size_t)nx * ny;
   const int quint = kp.h_quintico;
+  const LogEnt *lt = log_tab_fill<R>(smem_raw, (size_t)(6 * QPL), tid, NT);
+  if constexpr (sizeof(R) == 8) __syncthreads();  // the first node's log
   int rlx = 0, rly = 0;
   if (tid < NRING) {
     if (tid < TX) { rlx = tid + 1; rly = 0; }
@@ -621,12 +678,12 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
     const R pcur = pq, ccur = cq, rpc = rp, rcc = rcv;
     if (q < q_last) fetch(q + 1, pq, cq, rp, rcv);
     R *gw = sgm + (q % 3) * 2 * QPL;
-    solute_node(pcur, ccur, quint, s, igP, muP);
+    solute_node(pcur, ccur, quint, s, lt, igP, muP);
     gw[oc] = igP;
     gw[QPL + oc] = muP;
     if (tid < NRING) {
       R rig, rmu;
-      solute_node(rpc, rcc, quint, s, rig, rmu);
+      solute_node(rpc, rcc, quint, s, lt, rig, rmu);
       gw[rc] = rig;
       gw[QPL + rc] = rmu;
     }
@@ -663,7 +720,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
 
 template <typename TL, typename R>
 constexpr size_t solute_lds_bytes() {
-  return sizeof(R) * (size_t)(6 * TL::QPL);
+  return log_tab_offset<R>((size_t)(6 * TL::QPL)) + log_tab_bytes<R>();
 }
 
 

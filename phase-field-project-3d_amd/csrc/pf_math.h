@@ -3,7 +3,7 @@
 // The fused sweeps evaluate two logarithms and about a dozen divisions per
 // cell; the library forms cost ~100 VALU instructions per log and ~10 per
 // division on gfx950.  These are shorter forms with an error of about one
-// ulp.  Written __host__ __device__ so that the host build can check the
+// ulp.  Written __host__ __device__ so that a host build checks the
 // algorithm's accuracy against a long-double logarithm (the device form
 // differs only in the reciprocal's starting estimate).
 #pragma once
@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+
+#include "pf_log_table.h"
 
 namespace pf {
 
@@ -33,7 +35,8 @@ __host__ __device__ __forceinline__ double rcp_refined(double x) {
 // 1/x with one Newton step: measured on gfx950 the estimate is good to about
 // 2^-24 and one step leaves at most 11 ulp (2.4e-15 relative; two steps are
 // correctly rounded).  For the stencil's divisions, whose operands already
-// carry rounding noise far above that.
+// carry rounding noise far above that.  For x = 0, +-inf or NaN the estimate
+// itself is returned (inf, 0, NaN), as a division would give.
 __host__ __device__ __forceinline__ double rcp_fast(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   const double r0 = __builtin_amdgcn_rcp(x);
@@ -44,19 +47,33 @@ __host__ __device__ __forceinline__ double rcp_fast(double x) {
   return r != r ? r0 : r;
 }
 
+// 1/x with one Newton step and no special-case guard, for a denominator that
+// is positive and finite whenever the inputs are finite (the node's squared
+// gradient plus a positive floor): 0 * inf never arises there, and
+// non-finite inputs give NaN, as the division does in that expression.
+__host__ __device__ __forceinline__ double rcp_pos(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r0 = __builtin_amdgcn_rcp(x);
+#else
+  const double r0 = 1.0 / x;
+#endif
+  return fma(r0, fma(-x, r0, 1.0), r0);
+}
+
 // ln 2 split so that e * kLn2Hi is exact for |e| < 2^21 (kLn2Hi keeps 32
 // significant bits; kLn2Lo = ln 2 - kLn2Hi rounded).
 constexpr double kLn2Hi = 0x1.62e42fee00000p-1;
 constexpr double kLn2Lo = 0x1.a39ef35793c76p-33;
 
-// Natural logarithm.  x = 2^e m with m in [sqrt(1/2), sqrt(2)), f = m - 1,
-// s = f/(2+f); then log m = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R)) with
+// Natural logarithm without a table.  x = 2^e m with m in [sqrt(1/2),
+// sqrt(2)), f = m - 1, s = f/(2+f); then log m = 2 atanh(s) =
+// f - (f^2/2 - s (f^2/2 + R)) with
 // R = sum_{k>=1} 2 s^(2k)/(2k+1), the atanh Taylor series, which is exact
 // algebra (2s = f - s f) that keeps the exact f as the leading term.  With
 // |s| <= 0.1716, ten terms leave a truncation below 1e-17 relative.  Then
 // log x = e kLn2Hi + (log m + e kLn2Lo).  x = 0 gives -inf, x < 0 and NaN
 // give NaN, +inf gives +inf, as log() does.
-__host__ __device__ __forceinline__ double log_fast(double x) {
+__host__ __device__ __forceinline__ double log_div(double x) {
   int e;
   double m = frexp(x, &e);  // [0.5, 1) for finite x != 0
   if (m < 0.70710678118654752440) {
@@ -84,6 +101,52 @@ __host__ __device__ __forceinline__ double log_fast(double x) {
   if (!(x > 0.0)) r = x == 0.0 ? -__builtin_huge_val() : __builtin_nan("");
   if (x == __builtin_huge_val()) r = x;
   return r;
+}
+
+// One entry of the logarithm's table (pf_log_table.h): inv = 1/c_j rounded,
+// L = -ln(inv) rounded, for c_j = 1 + j/128.  16-byte aligned so that an
+// entry is one 128-bit LDS read.
+struct alignas(16) LogEnt {
+  double inv, L;
+};
+
+// Natural logarithm by table lookup (P. T. P. Tang, "Table-driven
+// implementation of the logarithm function in IEEE floating-point
+// arithmetic", ACM TOMS 16(4), 1990, in the variant with a reciprocal
+// table):  x = 2^e m with m in [sqrt(1/2), sqrt(2)); j = trunc(128 (m - 1))
+// picks c_j = 1 + j/128 between 1 and m (so ln m, ln c_j and r below share a
+// sign and never cancel); r = m inv_j - 1 in one FMA rounding, |r| < 2^-6.5,
+// and ln m = L_j + ln(1 + r) exactly, whatever the rounding of inv_j;
+// ln(1 + r) is its Taylor series to r^8 (the next term is below 2^-55 |r|).
+// Then ln x = (e kLn2Hi + L_j) + (e kLn2Lo + ln(1 + r)).  About one ulp;
+// x = 0 gives -inf, x < 0 and NaN give NaN, +inf gives +inf, as log() does.
+// `tab` holds the kLogTabN entries of pf_log_table.h (an LDS copy on the
+// device).
+__host__ __device__ __forceinline__ double log_tab(double x, const LogEnt *tab) {
+  int e;
+  double m = frexp(x, &e);  // [0.5, 1) for finite x != 0
+  if (m < 0.70710678118654752440) {
+    m = m + m;
+    e -= 1;
+  }
+  // the clamp only matters for x <= 0, inf and NaN, whose result is
+  // replaced below; it keeps the index (and the conversion) in range
+  const double u = fmin(fmax((m - 1.0) * 128.0, (double)kLogTabLo),
+                        (double)(kLogTabLo + kLogTabN - 1));
+  const LogEnt t = tab[(int)u - kLogTabLo];
+  const double r = fma(m, t.inv, -1.0);
+  double q = fma(r, -1.0 / 8.0, 1.0 / 7.0);
+  q = fma(r, q, -1.0 / 6.0);
+  q = fma(r, q, 1.0 / 5.0);
+  q = fma(r, q, -1.0 / 4.0);
+  q = fma(r, q, 1.0 / 3.0);
+  q = fma(r, q, -0.5);
+  const double p = fma(r * r, q, r);
+  const double de = (double)e;
+  double res = fma(de, kLn2Hi, fma(de, kLn2Lo, t.L + p));
+  if (!(x > 0.0)) res = x == 0.0 ? -__builtin_huge_val() : __builtin_nan("");
+  if (x == __builtin_huge_val()) res = x;
+  return res;
 }
 
 }  // namespace pf
