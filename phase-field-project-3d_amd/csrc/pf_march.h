@@ -290,6 +290,27 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t boff) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
 
+// Store of one element at a per-lane byte offset from a uniform plane base,
+// without a branch: a lane that must not store passes kNoStore, an offset
+// beyond the range of the buffer resource built here, and the hardware drops
+// its write (raw buffer accesses are range-checked against num_records; a
+// plane is under 4 GiB, so every real offset is in range).  With every
+// vector memory instruction of a plane unconditional, the compiler waits for
+// exactly the loads it needs; a store behind a branch makes it wait for all
+// outstanding memory operations at the next plane, stores included.
+constexpr uint32_t kNoStore = 0xffffffffu;
+typedef unsigned int pf_u32x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ void bstore(T *base, uint32_t boff, T v) {
+  // word 3 of a raw (untyped, stride 0) buffer resource on gfx9-family parts
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)0xffffffffu, 0x00020000);
+  if constexpr (sizeof(T) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pf_u32x2, v), r, boff, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, boff, 0, 0);
+}
+
 struct Geo {
   int nx, ny, nz;
   int nbx, nby, nseg, seg, k_lo, k_hi;  // cell planes [k_lo,k_hi] split in segments
@@ -380,7 +401,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   uint32_t loff[NLD];  // byte offsets
 #pragma unroll
   for (int u = 0; u < NLD; ++u) {
-    const int e = tid + u * NT;
+    const int e = min(tid + u * NT, PPL - 1);  // lanes past the window reload its last element
     const int gi = clampi(i0 - 2 + (e % PX), 1, nx), gj = clampi(j0 - 2 + (e / PX), 1, ny);
     loff[u] = (uint32_t)((gi - 1) + (gj - 1) * nx) * (uint32_t)sizeof(R);
   }
@@ -396,7 +417,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     const R *src = po + plane(q);
 #pragma unroll
     for (int u = 0; u < NLD; ++u)
-      if (tid + u * NT < PPL) pre[u] = at(src, loff[u]);
+      pre[u] = at(src, loff[u]);
   };
   auto slot = [&](int q) -> R * { return sphi + (q % NSLOT) * PPL; };  // q >= 0
   auto lstore = [&](int q) {
@@ -439,27 +460,36 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // scheduling region.
   auto plane_step = [&](int q, auto steady) __attribute__((always_inline)) {
     constexpr bool ST = decltype(steady)::value;
-    lstore(q);
-    if (q < q_last) gload(q + 1);  // prefetch; lands while this plane computes
-    __syncthreads();
     const int kn = q - 1;  // plane whose node / solute quantities are built now
     // (steady planes: kn in [ks+1, ke+1], inside [3, nz]; a node of plane nz
     // is built from clamped planes and never used)
+    // This plane's loads of c go first, the prefetch of phi plane q + 1
+    // second: waiting for c then never waits for the prefetch, and the wait
+    // for the prefetch (before its LDS store at the end of the plane) only for
+    // what was issued after it in this same plane, not for the stores.
+    const bool cnode = SWEEP_B && (ST || (q >= ks && kn >= 1 && kn <= nz));
+    R cnew_col = 0, cring = 0, cPPv = 0;
+    if (cnode) {
+      const size_t pk = (size_t)(kn - 1) * sz;
+      cnew_col = at(co + pk, ooff);
+      cring = at(co + pk, roff);  // every lane: no branch around the load
+    }
+    if constexpr (ST && !SWEEP_B) cPPv = at(co + (size_t)(q - 3) * sz, ooff);  // plane q - 2
+    gload(min(q + 1, q_last));  // prefetch; lands while this plane computes
+    __syncthreads();            // phi plane q (stored at the end of plane q - 1) in LDS
     const bool build = ST || (q >= ks && kn >= 2 && kn <= nz - 1);
     Node<R> nr{};
-    R rig = 0, rmu = 0, cnew_col = 0;
+    R rig = 0, rmu = 0;
     if (build) {
       nP = node_at(q, olx, oly);
       if (tid < NRING) nr = node_at(q, rlx, rly);
     }
     if constexpr (SWEEP_B) {
-      if (ST || (q >= ks && kn >= 1 && kn <= nz)) {
+      if (cnode) {
         const R *P0 = slot(q - 1);
-        const size_t pk = (size_t)(kn - 1) * sz;
-        cnew_col = at(co + pk, ooff);
         solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, lt, igP, muP);
         if (tid < NRING)
-          solute_node(P0[(rly + 1) * PX + rlx + 1], at(co + pk, roff), quint, s, lt, rig, rmu);
+          solute_node(P0[(rly + 1) * PX + rlx + 1], cring, quint, s, lt, rig, rmu);
       }
     }
     if constexpr (ST) {
@@ -480,7 +510,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const size_t pk = (size_t)(k - 1) * sz;  // plane k of a field
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
-      if constexpr (SWEEP_B) cPP = cZ; else cPP = at(co + pk, ooff);
+      if constexpr (SWEEP_B) cPP = cZ; else cPP = cPPv;
       // the Laplacian-like term, then the three cross terms, each reading
       // its own phi and node values
       const int nc = oly * QX + olx;
@@ -528,17 +558,15 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R cl = pstar > (R)1 ? (R)1 : pstar;
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;
-      if (upd) at(pn + pk, ooff) = pnew;
+      bstore(pn + pk, upd ? ooff : kNoStore, pnew);
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
         const R *ig = sgm, *mu = sgm + QPL;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
-        if (own) {
-          at(cn + pk, ooff) = cPP + inc;
-          macc += inc;
-        }
+        bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+        macc += own ? inc : (R)0;
       }
       if constexpr (T_OUT) {
         if (write_T) {
@@ -552,9 +580,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
             lap = (tc[1] - TP2 + tc[-1]) * s.idx2 + (tc[nx] - TP2 + tc[-nx]) * s.idy2 +
                   (tc[sz] - TP2 + tc[-(ptrdiff_t)sz]) * s.idz2;
           }
-          if (own) at(Tnext + pk, ooff) = s.Ddt * lap + s.fonte * hl2 * dppv + s.TP;
-        } else if (upd) {
-          at(dpp + pk, ooff) = dppv;
+          bstore(Tnext + pk, own ? ooff : kNoStore, s.Ddt * lap + s.fonte * hl2 * dppv + s.TP);
+        } else {
+          bstore(dpp + pk, upd ? ooff : kNoStore, dppv);
         }
       } else {
         (void)Tcur; (void)Tnext; (void)dpp; (void)write_T; (void)dppv;
@@ -585,8 +613,14 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       igM = igZ; muM = muZ; igZ = igP; muZ = muP;
       cZ = cnew_col;
     }
+    // phi plane q + 1 into its slot, (q + 1) % NSLOT = (q - 4) % NSLOT, which
+    // no thread reads in this plane (nodes read slots q-2..q, the update
+    // q-3..q-1) and every thread finished reading in plane q - 1; the next
+    // plane's first barrier publishes it
+    lstore(q + 1);
   };
   gload(q_first);
+  lstore(q_first);
   for (int q = q_first; q < ks + 2; ++q) plane_step(q, std::false_type{});
   for (int q = ks + 2; q <= q_last; ++q) plane_step(q, std::true_type{});
   if constexpr (SWEEP_B) {
@@ -664,19 +698,24 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   // node plane q (all in [1, nz]); prefetched one plane ahead
   const int q_first = ks - 1, q_last = ke + 1;
   R pq = 0, cq = 0, rp = 0, rcv = 0;
+  // every lane loads (the ring values of non-ring lanes are clamped in-grid
+  // duplicates, unused): no branch around a load
   auto fetch = [&](int q, R &p, R &c, R &p2, R &c2) {
     const size_t pk = (size_t)(q - 1) * sz;
     p = at(po + pk, ooff);
     c = at(co + pk, ooff);
-    if (tid < NRING) {
-      p2 = at(po + pk, roff);
-      c2 = at(co + pk, roff);
-    }
+    p2 = at(po + pk, roff);
+    c2 = at(co + pk, roff);
   };
   fetch(q_first, pq, cq, rp, rcv);
-  for (int q = q_first; q <= q_last; ++q) {
+  // One node plane.  The first two (ST false) only build and publish nodes;
+  // from plane ks + 1 on each also updates cell plane q - 1, every lane
+  // computing (lanes beyond the grid on clamped data) and only the stores
+  // masked, so the plane's memory instructions are the same on every path.
+  auto step = [&](int q, auto steady) __attribute__((always_inline)) {
+    constexpr bool ST = decltype(steady)::value;
     const R pcur = pq, ccur = cq, rpc = rp, rcc = rcv;
-    if (q < q_last) fetch(q + 1, pq, cq, rp, rcv);
+    fetch(min(q + 1, q_last), pq, cq, rp, rcv);
     R *gw = sgm + (q % 3) * 2 * QPL;
     solute_node(pcur, ccur, quint, s, lt, igP, muP);
     gw[oc] = igP;
@@ -688,8 +727,8 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       gw[QPL + rc] = rmu;
     }
     __syncthreads();
-    const int k = q - 1;  // cell plane updated now
-    if (q >= ks + 1 && own) {
+    if constexpr (ST) {
+      const int k = q - 1;  // cell plane updated now
       const size_t pk = (size_t)(k - 1) * sz;
       const R PP = pZ, cPP = cZ;
       const int nc = oc;
@@ -698,12 +737,14 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const R igc = igZ, muc = muZ;
       const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
-      at(cn + pk, ooff) = cPP + inc;
-      macc += inc;
+      bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+      macc += own ? inc : (R)0;
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
     pZ = pcur; cZ = ccur;
-  }
+  };
+  for (int q = q_first; q < ks + 1; ++q) step(q, std::false_type{});
+  for (int q = ks + 1; q <= q_last; ++q) step(q, std::true_type{});
   if (mslot) {  // uniform over the work-group
     __shared__ double wsum[NT / 64];
     double v = (double)macc;
