@@ -442,6 +442,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   Node<R> nM{}, nZ{}, nP{};
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R cZ = 0;
+  R cPPcur = 0;  // c of the plane updated next (sweeps A and phase + T)
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
   // solute increments of this column for the ledger, summed in the
   // arithmetic type over the column's z-segment (a few dozen cells) and in
@@ -467,14 +468,20 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     // second: waiting for c then never waits for the prefetch, and the wait
     // for the prefetch (before its LDS store at the end of the plane) only for
     // what was issued after it in this same plane, not for the stores.
+    // FP64: the c of the next plane's update (sweeps A and phase + T) is
+    // loaded one plane ahead, before this plane's stores are issued, so that
+    // using it never waits for them (measured 3% faster phase + T; the FP32
+    // sweep A, with twice the waves to hide latency, measured 3% slower).
+    constexpr bool C_AHEAD = !SWEEP_B && sizeof(R) == 8;
     const bool cnode = SWEEP_B && (ST || (q >= ks && kn >= 1 && kn <= nz));
-    R cnew_col = 0, cring = 0, cPPv = 0;
+    R cnew_col = 0, cring = 0, cPPnext = 0, cPPnow = 0;
     if (cnode) {
       const size_t pk = (size_t)(kn - 1) * sz;
       cnew_col = at(co + pk, ooff);
       cring = at(co + pk, roff);  // every lane: no branch around the load
     }
-    if constexpr (ST && !SWEEP_B) cPPv = at(co + (size_t)(q - 3) * sz, ooff);  // plane q - 2
+    if constexpr (C_AHEAD) cPPnext = at(co + plane(q - 1), ooff);  // plane q - 1 (clamped)
+    if constexpr (ST && !SWEEP_B && !C_AHEAD) cPPnow = at(co + (size_t)(q - 3) * sz, ooff);
     gload(min(q + 1, q_last));  // prefetch; lands while this plane computes
     __syncthreads();            // phi plane q (stored at the end of plane q - 1) in LDS
     const bool build = ST || (q >= ks && kn >= 2 && kn <= nz - 1);
@@ -510,7 +517,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const size_t pk = (size_t)(k - 1) * sz;  // plane k of a field
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
-      if constexpr (SWEEP_B) cPP = cZ; else cPP = cPPv;
+      if constexpr (SWEEP_B) cPP = cZ;
+      else if constexpr (C_AHEAD) cPP = cPPcur;
+      else cPP = cPPnow;  // plane q - 2
       // the Laplacian-like term, then the three cross terms, each reading
       // its own phi and node values
       const int nc = oly * QX + olx;
@@ -612,6 +621,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     if constexpr (SWEEP_B) {
       igM = igZ; muM = muZ; igZ = igP; muZ = muP;
       cZ = cnew_col;
+    } else if constexpr (C_AHEAD) {
+      cPPcur = cPPnext;
     }
     // phi plane q + 1 into its slot, (q + 1) % NSLOT = (q - 4) % NSLOT, which
     // no thread reads in this plane (nodes read slots q-2..q, the update
