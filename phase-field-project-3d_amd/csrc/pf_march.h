@@ -57,15 +57,10 @@ constexpr size_t march_lds_words(bool sweep_b) {
   return (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
 }
 
-// log and 1/x in the arithmetic type.  FP64: the ~1 ulp logarithms and the
-// one-Newton-step reciprocal (<= 11 ulp) of pf_math.h; FP32: the hardware
-// log2 / reciprocal (about 1 ulp).  The FP64 logarithm comes in two forms
-// of the same accuracy: dlog, the table-driven one (its table in LDS, `lt`;
-// fewer instructions, one LDS read), for the solute's node potential, and
-// dlog_div, the division-based one (pure VALU), for the phase update's free
-// energy, where the LDS read's latency measured slower (the phase update
-// already waits on two dozen stencil reads) and the table form brought the
-// sweep kernels nothing.
+// log and 1/x in the arithmetic type.  FP64: the ~1 ulp table-driven
+// logarithm (its table in LDS, `lt`) and the one-Newton-step reciprocal
+// (<= 11 ulp) of pf_math.h; FP32: the hardware log2 / reciprocal (about
+// 1 ulp).
 template <typename R>
 __device__ __forceinline__ R dlog(R x, const LogEnt *lt);
 template <>
@@ -74,13 +69,6 @@ __device__ __forceinline__ double dlog<double>(double x, const LogEnt *lt) {
 }
 template <>
 __device__ __forceinline__ float dlog<float>(float x, const LogEnt *) { return __logf(x); }
-
-template <typename R>
-__device__ __forceinline__ R dlog_div(R x);
-template <>
-__device__ __forceinline__ double dlog_div<double>(double x) { return log_div(x); }
-template <>
-__device__ __forceinline__ float dlog_div<float>(float x) { return __logf(x); }
 
 template <typename R>
 __device__ __forceinline__ R rcpn(R x);
@@ -556,7 +544,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
       const R den = s.Ke * h + ((R)1 - h);
       const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
-      const R GG = hl * s.RTV * dlog_div<R>(fabs(arg)) - s.W0 * gl;
+      const R GG = hl * s.RTV * dlog<R>(fabs(arg), lt) - s.W0 * gl;
       R ruidos = (R)0;
       if (noise) {
         const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA

@@ -1,12 +1,8 @@
-// Host check of the FP64 logarithms of the fused sweeps (pf::log_tab and
-// pf::log_div in csrc/pf_math.h) against the long-double logarithm.  The
-// device code runs the same expressions (frexp and fma are exact IEEE
-// operations on both; log_div's reciprocal starts from the hardware
-// estimate on the device and from a division here, and its two Newton steps
-// round both to the same result but in rare ties), so this bounds the device
-// results too.  argv[2] selects the function ("tab" or "div").  Prints
-// "max_ulp mean_ulp n" and exits non-zero when a special value differs from
-// log().
+// Host check of the FP64 logarithm of the fused sweeps (pf::log_tab in
+// csrc/pf_math.h) against the long-double logarithm.  The device code runs
+// the same expression (frexp and fma are exact IEEE operations on both), so
+// this bounds the device results too.  Prints "max_ulp mean_ulp n" and exits
+// non-zero when a special value differs from log().
 #include <cfloat>
 #include <cinttypes>
 #include <cmath>
@@ -26,7 +22,6 @@ static double ulp_of(double v) {
 
 int main(int argc, char **argv) {
   const long n = argc > 1 ? std::atol(argv[1]) : 2000000;
-  const bool use_div = argc > 2 && std::strcmp(argv[2], "div") == 0;
   pf::LogEnt tab[pf::kLogTabN];
   for (int t = 0; t < pf::kLogTabN; ++t) tab[t] = pf::LogEnt{pf::kLogTab[t][0], pf::kLogTab[t][1]};
   uint64_t s = 0x9e3779b97f4a7c15ull;
@@ -37,7 +32,7 @@ int main(int argc, char **argv) {
   double maxu = 0.0, sumu = 0.0, worst = 0.0;
   long cnt = 0;
   auto check = [&](double x) {
-    const double got = use_div ? pf::log_div(x) : pf::log_tab(x, tab);
+    const double got = pf::log_tab(x, tab);
     const long double ref = logl((long double)x);
     const double u = (double)fabsl((long double)got - ref) / ulp_of((double)ref);
     if (u > maxu) { maxu = u; worst = x; }
@@ -76,7 +71,7 @@ int main(int argc, char **argv) {
   int bad = 0;
   const double sp[] = {0.0, -0.0, -1.0, INFINITY, -INFINITY, NAN, DBL_MIN, DBL_TRUE_MIN, DBL_MAX};
   for (double x : sp) {
-    const double got = use_div ? pf::log_div(x) : pf::log_tab(x, tab), ref = std::log(x);
+    const double got = pf::log_tab(x, tab), ref = std::log(x);
     const bool same = (std::isnan(got) && std::isnan(ref)) || got == ref ||
                       (std::isfinite(ref) && std::fabs(got - ref) <= 2 * ulp_of(ref));
     if (!same) {
