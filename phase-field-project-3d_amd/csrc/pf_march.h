@@ -428,6 +428,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // own-column registers: nodes of planes k-1 (M), k (Z), k+1 (P) and the
   // solute quantities of planes k-1, k, k+1
   Node<R> nM{}, nZ{}, nP{};
+  // phi of the own column in planes q-1, q-2, q-3 (read from LDS once, at
+  // plane q, and carried down the column in registers)
+  R a1 = 0, a2 = 0, a3 = 0;
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R cZ = 0;
   R cPPcur = 0;  // c of the plane updated next (sweeps A and phase + T)
@@ -473,16 +476,21 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     gload(min(q + 1, q_last));  // prefetch; lands while this plane computes
     __syncthreads();            // phi plane q (stored at the end of plane q - 1) in LDS
     const bool build = ST || (q >= ks && kn >= 2 && kn <= nz - 1);
+    const R aq = slot(q)[(oly + 1) * PX + (olx + 1)];  // phi of plane q, own column
     Node<R> nr{};
     R rig = 0, rmu = 0;
     if (build) {
-      nP = node_at(q, olx, oly);
+      // own node: phi of plane q and q-2 at the own column from the carry
+      const R *P0 = slot(q - 1);
+      const int c = (oly + 1) * PX + (olx + 1);
+      nP = node_from((P0[c + 1] - P0[c - 1]) * s.hx, (P0[c + PX] - P0[c - PX]) * s.hy,
+                     (aq - a2) * s.hz, s);
       if (tid < NRING) nr = node_at(q, rlx, rly);
     }
     if constexpr (SWEEP_B) {
       if (cnode) {
         const R *P0 = slot(q - 1);
-        solute_node(P0[(oly + 1) * PX + olx + 1], cnew_col, quint, s, lt, igP, muP);
+        solute_node(a1, cnew_col, quint, s, lt, igP, muP);  // plane q - 1, own column
         if (tid < NRING)
           solute_node(P0[(rly + 1) * PX + rlx + 1], cring, quint, s, lt, rig, rmu);
       }
@@ -501,7 +509,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
                   : (float)hash_unit(hbase);
       const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
       const int c = (ty + 2) * PX + (tx + 2);
-      const R PP = P0[c];
+      const R PP = a2;  // planes k, k+1, k-1 of the own column
+
       const size_t pk = (size_t)(k - 1) * sz;  // plane k of a field
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
@@ -513,7 +522,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const int nc = oly * QX + olx;
       const R *e2 = snode, *xy = snode + QPL, *yz = snode + 2 * QPL, *zx = snode + 3 * QPL;
       const R PE = P0[c + 1], PW = P0[c - 1], PN = P0[c + PX], PS = P0[c - PX];
-      const R PT = Pp[c], PB = Pm[c];
+      const R PT = a1, PB = a3;
       const R e2E = e2[nc + 1], e2W = e2[nc - 1], e2N = e2[nc + QX], e2S = e2[nc - QX];
       R AA = (e2E * (PE - PP) + e2W * (PW - PP)) * s.idx2 +
              (e2N * (PN - PP) + e2S * (PS - PP)) * s.idy2 +
@@ -606,6 +615,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       }
     }
     nM = nZ; nZ = nP;
+    a3 = a2; a2 = a1; a1 = aq;
     if constexpr (SWEEP_B) {
       igM = igZ; muM = muZ; igZ = igP; muZ = muP;
       cZ = cnew_col;
