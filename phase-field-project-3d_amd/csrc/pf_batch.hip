@@ -439,7 +439,7 @@ int rebuild(pf_batch *b) {
   HIP_OK(hipStreamSynchronize(b->stream));
   b->dirty = false;
   if (b->graph) {
-    hipGraphExecDestroy(b->graph);
+    (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
   b->graph_valid = false;
@@ -516,7 +516,7 @@ int enqueue_step(pf_batch *b) {
 
 int capture_graph(pf_batch *b) {
   if (b->graph) {
-    hipGraphExecDestroy(b->graph);
+    (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
   b->graph_valid = false;
@@ -526,12 +526,12 @@ int capture_graph(pf_batch *b) {
   for (int q = 0; q < b->graph_steps && rc == PF_OK; ++q) rc = enqueue_step(b);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc != PF_OK) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (e != hipSuccess) return fail_msg(PF_E_HIP, "graph capture: %s", hipGetErrorString(e));
   e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-  hipGraphDestroy(g);
+  (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
     b->graph = nullptr;
     return fail_msg(PF_E_HIP, "graph instantiate: %s", hipGetErrorString(e));
@@ -606,13 +606,16 @@ int pf_batch_create(size_t hbm_budget, pf_batch **out) {
     return fail_msg(PF_E_NODEV, "no HIP device visible");
   pf_batch *b = new pf_batch();
   for (int c = 0; c < pf::kNClasses; ++c) b->cls_shape[c] = 3;  // 16x16 until autotuned
-  hipGetDevice(&b->device);
+  if (hipGetDevice(&b->device) != hipSuccess) {
+    delete b;
+    return fail_msg(PF_E_HIP, "hipGetDevice");
+  }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
     return fail_msg(PF_E_HIP, "hipStreamCreate");
   }
   size_t fr = 0, tot = 0;
-  hipMemGetInfo(&fr, &tot);
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;  // then only an explicit budget admits
   b->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
   if (hipHostMalloc((void **)&b->pinned, 8 * sizeof(double)) != hipSuccess) b->pinned = nullptr;
   *out = b;
@@ -621,19 +624,22 @@ int pf_batch_create(size_t hbm_budget, pf_batch **out) {
 
 int pf_batch_destroy(pf_batch *b) {
   if (!b) return PF_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->graph) hipGraphExecDestroy(b->graph);
+  // an error of work still in flight is reported here; the batch is freed
+  // either way
+  const hipError_t e = b->stream ? hipStreamSynchronize(b->stream) : hipSuccess;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (auto &s : b->slots)
     for (void *p : s.buf)
-      if (p) hipFree(p);
+      if (p) (void)hipFree(p);
   for (int prec = 0; prec < 2; ++prec)
-    if (b->inst_dev[prec]) hipFree(b->inst_dev[prec]);
-  if (b->tiles_dev) hipFree(b->tiles_dev);
-  if (b->chunks_dev) hipFree(b->chunks_dev);
-  if (b->diag_part) hipFree(b->diag_part);
-  if (b->pinned) hipHostFree(b->pinned);
-  if (b->stream) hipStreamDestroy(b->stream);
+    if (b->inst_dev[prec]) (void)hipFree(b->inst_dev[prec]);
+  if (b->tiles_dev) (void)hipFree(b->tiles_dev);
+  if (b->chunks_dev) (void)hipFree(b->chunks_dev);
+  if (b->diag_part) (void)hipFree(b->diag_part);
+  if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+  if (e != hipSuccess) return fail_msg(PF_E_HIP, "pending work failed: %s", hipGetErrorString(e));
   return PF_OK;
 }
 
@@ -672,13 +678,13 @@ int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nste
   s.steps_total = s.steps_left = nsteps;
   for (int f = 0; f < 6; ++f) {
     if (hipMalloc(&s.buf[f], s.n * esz) != hipSuccess) {
-      for (int g = 0; g < f; ++g) hipFree(s.buf[g]);
+      for (int g = 0; g < f; ++g) (void)hipFree(s.buf[g]);
       return fail_msg(PF_E_NOMEM, "hipMalloc of %zu bytes failed", s.n * esz);
     }
   }
   auto cleanup = [&](int code) {
     for (void *q : s.buf)
-      if (q) hipFree(q);
+      if (q) (void)hipFree(q);
     return code;
   };
   // phi_old, c_old, T_old into both buffers of each pair (the new buffers are
@@ -792,7 +798,7 @@ int pf_batch_set_graph_steps(pf_batch *b, int32_t k) {
   b->graph_steps = k;
   if (b->graph) {
     HIP_OK(hipStreamSynchronize(b->stream));
-    hipGraphExecDestroy(b->graph);
+    (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
   b->graph_valid = false;
@@ -1010,9 +1016,10 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
     }
     b->cls_shape[cls] = bs;
   }
-  hipStreamSynchronize(b->stream);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  if (hipStreamSynchronize(b->stream) != hipSuccess && rc == PF_OK)
+    rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(hipGetLastError()));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   b->dirty = true;
   return rc;
 }

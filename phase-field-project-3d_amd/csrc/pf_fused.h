@@ -56,14 +56,14 @@ struct KTimer {
       pool.push_back(e);
     }
     kind.push_back(k);
-    hipEventRecord(pool[2 * r], s);
+    if (hipEventRecord(pool[2 * r], s) != hipSuccess) on = false;  // timing only: stop timing
   }
   void end(hipStream_t s) {
     if (!on || kind.empty()) return;
-    hipEventRecord(pool[2 * (kind.size() - 1) + 1], s);
+    if (hipEventRecord(pool[2 * (kind.size() - 1) + 1], s) != hipSuccess) on = false;
   }
   void release() {
-    for (hipEvent_t e : pool) hipEventDestroy(e);
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
     pool.clear();
     kind.clear();
   }

@@ -338,7 +338,7 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
                            : make_geo(nx, ny, nz, 3, nz - 2, seg, tile);
         hipError_t e = hipSuccess;
         for (int r = 0; r <= reps && e == hipSuccess; ++r) {  // r = 0 is the warm-up
-          if (r == 1) hipEventRecord(e0, st);
+          if (r == 1 && (e = hipEventRecord(e0, st)) != hipSuccess) break;
           if (kind == 3)
             e = launch_solute_split<R>(tile, g, k, po, co, out + n, nullptr, st);
           else
@@ -377,10 +377,10 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
     res->n = cand < PF_AUTOTUNE_MAX ? cand : PF_AUTOTUNE_MAX;
     for (int q = 0; q < 4; ++q) res->best_ms[q] = best[q];
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  hipStreamSynchronize(st);
-  hipFree(out);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (hipStreamSynchronize(st) != hipSuccess && rc == PF_OK) rc = PF_E_HIP;
+  (void)hipFree(out);
   return rc;
 }
 
@@ -527,7 +527,7 @@ int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
   if (need > S.nslot) {  // slots are folded (zero) between runs: safe to replace
     if (S.mslot) {
       if (hipStreamSynchronize(st) != hipSuccess) return PF_E_HIP;
-      hipFree(S.mslot);
+      (void)hipFree(S.mslot);
       S.mslot = nullptr;
       S.nslot = 0;
     }
@@ -591,9 +591,9 @@ int fused_mass_read(FusedState &S, double out[3], hipStream_t st) {
 
 void fused_free(FusedState &s) {
   s.enabled = false;
-  if (s.mass) hipFree(s.mass);
-  if (s.mpart) hipFree(s.mpart);
-  if (s.mslot) hipFree(s.mslot);
+  if (s.mass) (void)hipFree(s.mass);
+  if (s.mpart) (void)hipFree(s.mpart);
+  if (s.mslot) (void)hipFree(s.mslot);
   s.mass = s.mpart = s.mslot = nullptr;
   s.nslot = 0;
   s.ledger_valid = false;

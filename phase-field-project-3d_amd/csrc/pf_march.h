@@ -676,7 +676,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R *sgm = reinterpret_cast<R *>(smem_raw);  // 3 x {1/gamma, mu} x QPL
 
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  const int nx = g.nx, ny = g.ny;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
   const int i0 = 2 + bx * TX, j0 = 2 + by * TL::TY;
   const int i = i0 + tx, j = j0 + ty;

@@ -217,23 +217,24 @@ int pf_stream_copy(size_t bytes, int32_t iters, double *gbs) {
              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     rc = fail(PF_E_HIP, "stream-copy stream/events");
   } else {
-    hipMemsetAsync(a, 0, n2 * 16, s);
     const dim3 grd(8192), blk(256);
+    bool ok = hipMemsetAsync(a, 0, n2 * 16, s) == hipSuccess;
     hipLaunchKernelGGL(k_stream_copy, grd, blk, 0, s, a, b, n2);  // warm-up
-    hipEventRecord(e0, s);
+    ok = ok && hipEventRecord(e0, s) == hipSuccess;
     for (int q = 0; q < iters; ++q) hipLaunchKernelGGL(k_stream_copy, grd, blk, 0, s, a, b, n2);
-    hipEventRecord(e1, s);
+    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, s) == hipSuccess;
     float ms = 0.f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    if (!ok || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
       rc = fail(PF_E_HIP, "stream-copy timing");
     else
       *gbs = 2.0 * (double)(n2 * 16) * iters / ((double)ms * 1e-3) / 1e9;
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (s) hipStreamDestroy(s);
-  if (a) hipFree(a);
-  if (b) hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
   return rc;
 }
 
@@ -302,18 +303,21 @@ int pf_init(const pf_params *p, const double *phi_old, const double *phi_new, co
 
 int pf_finalize(pf_ctx *ctx) {
   if (!ctx) return PF_OK;
-  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  // an error of work still in flight is reported here; the context is freed
+  // either way
+  const hipError_t e = ctx->stream ? hipStreamSynchronize(ctx->stream) : hipSuccess;
   pf::fused_free(ctx->fused);
   ctx->timer.release();
   void *bufs[7] = {ctx->phi.buf[0], ctx->phi.buf[1], ctx->c.buf[0], ctx->c.buf[1],
                    ctx->T.buf[0], ctx->T.buf[1], ctx->dpp};
   for (void *b : bufs)
-    if (b) hipFree(b);
-  if (ctx->diag_part) hipFree(ctx->diag_part);
-  if (ctx->probe_dev) hipFree(ctx->probe_dev);
-  if (ctx->pinned) hipHostFree(ctx->pinned);
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (b) (void)hipFree(b);
+  if (ctx->diag_part) (void)hipFree(ctx->diag_part);
+  if (ctx->probe_dev) (void)hipFree(ctx->probe_dev);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  if (e != hipSuccess) return fail(PF_E_HIP, "pending work failed: %s", hipGetErrorString(e));
   return PF_OK;
 }
 
