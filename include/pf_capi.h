@@ -225,7 +225,7 @@ int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
  * untouched), keeps the fastest per kernel and uses the split form of sweep B
  * when its two kernels together are faster.  If ms is non-NULL it receives 4
  * doubles per candidate {kind (0 = A, 1 = fused B, 2 = phase + T, 3 =
- * solute), tile, target_groups, average ms}, at most 64 candidates, their
+ * solute), tile, target_groups, average ms}, at most 256 candidates, their
  * number in *ncand.  Synchronous. */
 int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms);
 /* Solute-mass correction of the fused path (no counterpart in the

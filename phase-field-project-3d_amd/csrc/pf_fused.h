@@ -89,7 +89,7 @@ int fused_get_kind(const FusedState &s, int kind, int out[3]);  // {tile, target
 // and the two kernels of its split form on the given fields (writing into
 // scratch buffers of its own) and keeps the fastest; the split form is
 // chosen when its two kernels together beat the fused sweep.
-constexpr int PF_AUTOTUNE_MAX = 64;
+constexpr int PF_AUTOTUNE_MAX = 256;
 struct AutotuneResult {
   int n = 0;
   struct { int sweep, tile, target; float ms; } cand[PF_AUTOTUNE_MAX];

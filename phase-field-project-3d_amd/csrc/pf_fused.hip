@@ -322,7 +322,11 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PF_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PF_E_HIP;
-  static const int targets[] = {1024, 2048, 4096, 8192};
+  // work-group targets: powers of two and multiples of 768, the work-groups
+  // that fill 256 CUs at three per CU (the FP64 sweeps' occupancy; the FP32
+  // ones hold six): a count just above a whole number of such rounds leaves
+  // most of the GPU idle for its last work-groups
+  static const int targets[] = {1024, 1536, 2048, 2304, 3072, 3840, 4096, 4608, 6144, 8192};
   const R *po = (const R *)phi_old, *co = (const R *)c_old, *Tc = (const R *)T_old;
   // kinds: 0 sweep A, 1 fused sweep B, 2 phase + T of the split form,
   // 3 solute kernel of the split form

@@ -316,7 +316,7 @@ class Context:
         (kind, tile, target_groups, ms) with kind A (sweep A), B (fused sweep
         B), P (phase + T of the split form), S (its solute kernel)."""
         n = C.c_int32(0)
-        ms = np.zeros(4 * 64)
+        ms = np.zeros(4 * 256)  # pf_autotune writes at most 256 candidates
         check(self.lib.pf_autotune(self.h, C.byref(p), reps, C.byref(n), _ptr(ms)), self.lib)
         return [("ABPS"[int(ms[4 * q])], int(ms[4 * q + 1]), int(ms[4 * q + 2]),
                  float(ms[4 * q + 3])) for q in range(n.value)]

@@ -246,7 +246,9 @@ def test_tiling_does_not_change_results(precision):
     sim.run(5)
     before = sim.fields()
     cands = sim.ctx.autotune(sim.p, reps=2)
-    assert len(cands) == 4 * 4 * 4 and all(c[3] > 0 for c in cands)
+    # 4 kernel kinds x 4 tile shapes x 10 work-group targets
+    assert len(cands) == 4 * 4 * 10 and all(c[3] > 0 for c in cands)
+    assert {c[2] for c in cands} == {1024, 1536, 2048, 2304, 3072, 3840, 4096, 4608, 6144, 8192}
     after = sim.fields()
     for nm in FIELDS:
         assert np.array_equal(before[nm], after[nm]), nm
