@@ -134,13 +134,14 @@ PF_HD size_t cidx(int i, int j, int k, int nx, int ny) {
 // Cell t (0-based) of the boundary shell of thickness w (2 for the phase
 // decoration, 1 for the solute faces), enumerated plane by plane: the w
 // planes at each z face are whole; every other plane contributes its w rows
-// at each y face (x-contiguous) and the w cells at each x face of its other
-// rows.  Returns false when t is past the shell.
+// at each y face (x-contiguous) and, if xf, the w cells at each x face of its
+// other rows (without them: the cells the fused sweeps write themselves).
+// Returns false when t is past the shell.
 __host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny, int nz, int w, int &i,
-                                           int &j, int &k) {
+                                           int &j, int &k, bool xf = true) {
   const long long plane = (long long)nx * ny;
   const long long full = (long long)w * plane;           // the w planes at one z face
-  const long long per = 2LL * w * nx + 2LL * w * (ny - 2 * w);  // one middle plane
+  const long long per = 2LL * w * nx + (xf ? 2LL * w * (ny - 2 * w) : 0LL);  // one middle plane
   const long long mid = per * (nz - 2 * w);
   if (t < 0 || t >= 2 * full + mid) return false;
   long long r;
@@ -171,8 +172,8 @@ __host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny,
   return true;
 }
 
-__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w) {
-  return 2LL * w * nx * ny + (2LL * w * nx + 2LL * w * (ny - 2 * w)) * (nz - 2 * w);
+__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w, bool xf = true) {
+  return 2LL * w * nx * ny + (2LL * w * nx + (xf ? 2LL * w * (ny - 2 * w) : 0LL)) * (nz - 2 * w);
 }
 
 // One cell of the phase decoration (equacao_campo_fase.f90:151-167 in

@@ -206,7 +206,7 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
       tm.begin(PF_KT_BOUNDARY, st);
-      e = launch_phase_boundary<R>(k, po, pn, st);
+      e = launch_phase_boundary<R>(k, po, pn, st, false);  // x faces written by the sweep
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
     } else {
@@ -237,8 +237,9 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
         if (e != hipSuccess) return PF_E_HIP;
       }
       tm.begin(PF_KT_BOUNDARY, st);
-      e = launch_phase_boundary<R>(k, po, pn, st);
-      if (e == hipSuccess) e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st);
+      e = launch_phase_boundary<R>(k, po, pn, st, false);  // x faces written by the sweeps
+      if (e == hipSuccess)
+        e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st, false);
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
     }

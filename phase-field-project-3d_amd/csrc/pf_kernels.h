@@ -19,11 +19,15 @@ template <typename R>
 hipError_t launch_diag(const KParams &p, const R *ph, const R *cc, double *part, int nblk,
                        double *out, hipStream_t s);
 
-// The boundary decorations alone (also used by the fused sweeps).
+// The boundary decorations alone (also used by the fused sweeps).  xf =
+// false leaves out the x-face cells of the interior rows, which the fused
+// sweeps write themselves.
 template <typename R>
-hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s);
+hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s,
+                                 bool xf = true);
 template <typename R>
-hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s);
+hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s,
+                                  bool xf = true);
 // Face cells take their interior source cell's value again (in place).
 template <typename R>
 hipError_t launch_solute_refresh(const KParams &p, R *c, hipStream_t s);

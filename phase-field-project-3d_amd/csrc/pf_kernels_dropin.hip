@@ -198,13 +198,15 @@ __global__ void __launch_bounds__(256) k_phase_interior(KParams p, const R *__re
 
 // The 12-plane decoration for the cells outside [3,n-2]^3.  Reads phi_new
 // only inside [3,n-2]^3 (never written here) and phi_old elsewhere.
+// xf = 0 leaves out the x-face cells of the rows inside [3,n-2] in y and z,
+// which the fused sweeps write in their own launch.
 template <typename R>
 __global__ void __launch_bounds__(256) k_phase_boundary(KParams p, const R *__restrict__ po,
-                                                        R *__restrict__ pn) {
+                                                        R *__restrict__ pn, int xf) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   int i, j, k;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-       shell_flat(t, nx, ny, nz, 2, i, j, k); t += (long long)gridDim.x * blockDim.x) {
+       shell_flat(t, nx, ny, nz, 2, i, j, k, xf != 0); t += (long long)gridDim.x * blockDim.x) {
     phase_deco_cell(i, j, k, nx, ny, nz, po, pn);
   }
 }
@@ -270,13 +272,15 @@ __global__ void __launch_bounds__(256) k_solute_interior(KParams p, const R *__r
 
 // Solute faces: c_new(face) = snapshot(inner neighbour), with the snapshot
 // equal to the swept c_new inside [2,n-1]^3 and to c_old on the faces.
+// xf = 0 leaves out the x-face cells of the rows inside [2,n-1] in y and z,
+// which the fused solute sweeps write in their own launch.
 template <typename R>
 __global__ void __launch_bounds__(256) k_solute_boundary(KParams p, const R *__restrict__ co,
-                                                         R *__restrict__ cn) {
+                                                         R *__restrict__ cn, int xf) {
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   int i, j, k;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-       shell_flat(t, nx, ny, nz, 1, i, j, k); t += (long long)gridDim.x * blockDim.x) {
+       shell_flat(t, nx, ny, nz, 1, i, j, k, xf != 0); t += (long long)gridDim.x * blockDim.x) {
     solute_deco_cell(i, j, k, nx, ny, nz, co, cn);
   }
 }
@@ -403,21 +407,23 @@ hipError_t launch_temperatura(const KParams &p, const R *Told, const R *phi, con
   return hipGetLastError();
 }
 
-static unsigned shell_blocks(const KParams &p, int w) {
-  const long long n = shell_count(p.nx, p.ny, p.nz, w);
+static unsigned shell_blocks(const KParams &p, int w, bool xf = true) {
+  const long long n = shell_count(p.nx, p.ny, p.nz, w, xf);
   const long long b = (n + 255) / 256;
   return (unsigned)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
 
 template <typename R>
-hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s) {
-  hipLaunchKernelGGL(k_phase_boundary<R>, dim3(shell_blocks(p, 2)), dim3(256), 0, s, p, po, pn);
+hipError_t launch_phase_boundary(const KParams &p, const R *po, R *pn, hipStream_t s, bool xf) {
+  hipLaunchKernelGGL(k_phase_boundary<R>, dim3(shell_blocks(p, 2, xf)), dim3(256), 0, s, p, po, pn,
+                     xf ? 1 : 0);
   return hipGetLastError();
 }
 
 template <typename R>
-hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s) {
-  hipLaunchKernelGGL(k_solute_boundary<R>, dim3(shell_blocks(p, 1)), dim3(256), 0, s, p, co, cn);
+hipError_t launch_solute_boundary(const KParams &p, const R *co, R *cn, hipStream_t s, bool xf) {
+  hipLaunchKernelGGL(k_solute_boundary<R>, dim3(shell_blocks(p, 1, xf)), dim3(256), 0, s, p, co,
+                     cn, xf ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -448,7 +454,7 @@ hipError_t launch_phase(const KParams &p, const R *po, const R *co, R *pn, R *dp
                      p, po, co, pn, dpp);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_phase_boundary<R>(p, po, pn, s);
+  return launch_phase_boundary<R>(p, po, pn, s, true);
 }
 
 template <typename R>
@@ -458,7 +464,7 @@ hipError_t launch_solute(const KParams &p, const R *po, const R *co, R *cn, hipS
                      p, po, co, cn);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  return launch_solute_boundary<R>(p, co, cn, s);
+  return launch_solute_boundary<R>(p, co, cn, s, true);
 }
 
 template <typename R>
@@ -479,8 +485,10 @@ hipError_t launch_diag(const KParams &p, const R *ph, const R *cc, double *part,
   template hipError_t launch_solute<R>(const KParams &, const R *, const R *, R *, hipStream_t);  \
   template hipError_t launch_diag<R>(const KParams &, const R *, const R *, double *, int,       \
                                      double *, hipStream_t);                                    \
-  template hipError_t launch_phase_boundary<R>(const KParams &, const R *, R *, hipStream_t);    \
-  template hipError_t launch_solute_boundary<R>(const KParams &, const R *, R *, hipStream_t);   \
+  template hipError_t launch_phase_boundary<R>(const KParams &, const R *, R *, hipStream_t,    \
+                                               bool);                                           \
+  template hipError_t launch_solute_boundary<R>(const KParams &, const R *, R *, hipStream_t,   \
+                                                bool);                                          \
   template hipError_t launch_solute_refresh<R>(const KParams &, R *, hipStream_t);               \
   template hipError_t launch_probe<R>(const R *, size_t, size_t, double *, hipStream_t);
 PF_INST(double)

@@ -565,6 +565,17 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;
       bstore(pn + pk, upd ? ooff : kNoStore, pnew);
+      {
+        // the x faces of an interior row (equacao_campo_fase.f90:151-167 in
+        // closed form: the face cells of rows inside [3,n-2] in y and z):
+        // cell 2 (nx-1) takes the swept value of cell 3 (nx-2), cell 1 (nx)
+        // the pre-sweep value of cell 2 (nx-1), i.e. PW (PE) of cell 3 (nx-2);
+        // the shell kernel after the sweep leaves these cells out
+        constexpr uint32_t E = (uint32_t)sizeof(R);
+        const bool xw = upd && i == 3, xe = upd && i == nx - 2;
+        bstore(pn + pk, xw ? ooff - E : xe ? ooff + E : kNoStore, pnew);
+        bstore(pn + pk, xw ? ooff - 2 * E : xe ? ooff + 2 * E : kNoStore, xw ? PW : PE);
+      }
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
         const R *ig = sgm, *mu = sgm + QPL;
@@ -572,6 +583,12 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
         bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+        // x faces of the row (concentracao_soluto.f90:116-124: a face cell of
+        // a row inside [2,n-1] in y and z copies its inner neighbour's new
+        // value); the face kernel after the sweep leaves them out
+        constexpr uint32_t E = (uint32_t)sizeof(R);
+        bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+               cPP + inc);
         macc += own ? inc : (R)0;
       }
       if constexpr (T_OUT) {
@@ -747,6 +764,10 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
       bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+      // x faces of the row, as in march_tile's sweep B
+      constexpr uint32_t E = (uint32_t)sizeof(R);
+      bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+             cPP + inc);
       macc += own ? inc : (R)0;
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
