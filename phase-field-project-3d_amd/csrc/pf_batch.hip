@@ -137,12 +137,13 @@ __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ ch
   if (!I.active) return;
   const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
   const int w = which == 0 ? 2 : 1;
-  const long long cnt = shell_count(nx, ny, nz, w);
+  // the x-face cells of the interior rows are written by the sweeps
+  const long long cnt = shell_count(nx, ny, nz, w, false);
   const long long end = ch.first + kChunkCells < cnt ? ch.first + kChunkCells : cnt;
   const int pin = I.po ^ ((sweep - 1) & 1);
   for (long long t = ch.first + threadIdx.x; t < end; t += 256) {
     int i, j, k;
-    shell_flat(t, nx, ny, nz, w, i, j, k);
+    shell_flat(t, nx, ny, nz, w, i, j, k, false);
     if (which == 0) phase_deco_cell(i, j, k, nx, ny, nz, (const R *)I.phi[pin], I.phi[pin ^ 1]);
     else solute_deco_cell(i, j, k, nx, ny, nz, (const R *)I.c[I.co], I.c[I.co ^ 1]);
   }
@@ -392,7 +393,7 @@ int rebuild(pf_batch *b) {
       const size_t off = chunks.size();
       for (size_t q = 0; q < ids.size(); ++q) {
         const pf_batch_slot &s = b->slots[ids[q]];
-        const long long cnt = pf::shell_count(s.nx, s.ny, s.nz, which == 0 ? 2 : 1);
+        const long long cnt = pf::shell_count(s.nx, s.ny, s.nz, which == 0 ? 2 : 1, false);
         for (long long f = 0; f < cnt; f += pf::kChunkCells) chunks.push_back({(int)q, f});
       }
       b->decos.push_back({prec, which, off, (int)(chunks.size() - off)});
