@@ -141,7 +141,9 @@ int pf_initial_fields(int32_t nx, int32_t ny, int32_t nz, int32_t tipo,
 int pf_init(const pf_params *p, const double *phi_old, const double *phi_new,
             const double *c_old, const double *c_new, const double *T_old,
             const double *T_new, const double *dpp, pf_ctx **out);
-/* source/principal.f90:268-272 (target exit data). Synchronous. */
+/* source/principal.f90:268-272 (target exit data). Synchronous: waits for
+ * the context's queued work, then frees it either way; returns PF_E_HIP
+ * (with the message) if that work failed. */
 int pf_finalize(pf_ctx *ctx);
 
 /* ---- the drop-in kernel calls (one per reference subroutine) ------------- */
@@ -262,6 +264,8 @@ typedef struct pf_batch pf_batch;
  * device's free memory now).  Instances use 6 fields (FP64: 48 B/cell,
  * FP32-mixed: 24 B/cell). */
 int pf_batch_create(size_t hbm_budget, pf_batch **out);
+/* Waits for the batch's queued work, then frees it either way; returns
+ * PF_E_HIP (with the message) if that work failed. */
 int pf_batch_destroy(pf_batch *b);
 /* Admits an instance that will run nsteps steps from the given state
  * (phi_old, c_old, T_old as for pf_init; DPP = 0); runs its first
