@@ -39,11 +39,30 @@ __global__ void __launch_bounds__(TL::NT)
                                   mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
 }
 
-// The solute update alone (split form of sweep B).
+// The solute update alone (split form of sweep B).  The ndeco work-groups
+// after the tiles' decorate the phase sweep's phi (dpn, from the swept
+// interior and the pre-sweep dpo) in the same launch: the solute reads only
+// the pre-sweep phi and c, so the two are independent, and the decoration
+// needs no launch of its own.  They run the shell rows only (the sweep wrote
+// the x faces of the interior rows).
 template <typename R, typename TL>
 __global__ void __launch_bounds__(TL::NT)
     k_solute(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
-             R *__restrict__ cn, double *__restrict__ mslot) {
+             R *__restrict__ cn, double *__restrict__ mslot, const R *__restrict__ dpo,
+             R *__restrict__ dpn, int ndeco) {
+  const int nb = 8 * g.per_xcd;
+  if ((int)blockIdx.x >= nb) {  // decoration work-group
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long cnt = shell_count(nx, ny, nz, 2, false);
+    const int tid = threadIdx.y * TL::TX + threadIdx.x;
+    for (long long t = (long long)((int)blockIdx.x - nb) * TL::NT + tid; t < cnt;
+         t += (long long)ndeco * TL::NT) {
+      int i, j, k;
+      shell_flat(t, nx, ny, nz, 2, i, j, k, false);
+      phase_deco_cell(i, j, k, nx, ny, nz, dpo, dpn);
+    }
+    return;
+  }
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
   solute_tile<R, TL>(g, kp, s, po, co, cn, mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
@@ -137,22 +156,30 @@ hipError_t launch_march(int mode, int tile, const Geo &g, const KParams &kp, con
 
 template <typename R, typename TL>
 hipError_t launch_solute_t(const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
-                           const R *co, R *cn, double *mslot, hipStream_t st) {
-  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
+                           const R *co, R *cn, double *mslot, const R *dpo, R *dpn,
+                           hipStream_t st) {
+  long long nd = 0;
+  if (dpn) {
+    nd = (shell_count(g.nx, g.ny, g.nz, 2, false) + TL::NT - 1) / TL::NT;
+    nd = nd < 1 ? 1 : (nd > 2048 ? 2048 : nd);
+  }
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd + (unsigned)nd, 1, 1);
   const size_t lds = solute_lds_bytes<TL, R>();
-  hipLaunchKernelGGL((k_solute<R, TL>), grd, blk, lds, st, g, kp, s, po, co, cn, mslot);
+  hipLaunchKernelGGL((k_solute<R, TL>), grd, blk, lds, st, g, kp, s, po, co, cn, mslot, dpo, dpn,
+                     (int)nd);
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_solute_split(int tile, const Geo &g, const KParams &kp, const R *po, const R *co,
-                               R *cn, double *mslot, hipStream_t st) {
+                               R *cn, double *mslot, hipStream_t st, const R *dpo = nullptr,
+                               R *dpn = nullptr) {
   const SC<R> s = make_sc<R>(kp);
   switch (tile) {
-    case 1: return launch_solute_t<R, Tile32x8>(g, kp, s, po, co, cn, mslot, st);
-    case 2: return launch_solute_t<R, Tile32x16>(g, kp, s, po, co, cn, mslot, st);
-    case 3: return launch_solute_t<R, Tile16x16>(g, kp, s, po, co, cn, mslot, st);
-    default: return launch_solute_t<R, Tile64x8>(g, kp, s, po, co, cn, mslot, st);
+    case 1: return launch_solute_t<R, Tile32x8>(g, kp, s, po, co, cn, mslot, dpo, dpn, st);
+    case 2: return launch_solute_t<R, Tile32x16>(g, kp, s, po, co, cn, mslot, dpo, dpn, st);
+    case 3: return launch_solute_t<R, Tile16x16>(g, kp, s, po, co, cn, mslot, dpo, dpn, st);
+    default: return launch_solute_t<R, Tile64x8>(g, kp, s, po, co, cn, mslot, dpo, dpn, st);
   }
 }
 
@@ -223,9 +250,10 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
         tm.end(st);
         if (e != hipSuccess) return PF_E_HIP;
         const Geo gs = solute_geo(S);
+        // with the phase decoration of this sweep in the same launch
         tm.begin(PF_KT_SOLUTE, st);
         e = launch_solute_split<R>(S.tile_s, gs, k, po, (const R *)c[*c_old], (R *)c[cnw],
-                                   mass_on(S, gs) ? S.mslot : nullptr, st);
+                                   mass_on(S, gs) ? S.mslot : nullptr, st, po, pn);
         tm.end(st);
         if (e != hipSuccess) return PF_E_HIP;
       } else {
@@ -237,7 +265,9 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
         if (e != hipSuccess) return PF_E_HIP;
       }
       tm.begin(PF_KT_BOUNDARY, st);
-      e = launch_phase_boundary<R>(k, po, pn, st, false);  // x faces written by the sweeps
+      // x faces written by the sweeps; the split form's phase decoration ran
+      // in the solute kernel's launch
+      e = S.split_b ? hipSuccess : launch_phase_boundary<R>(k, po, pn, st, false);
       if (e == hipSuccess)
         e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st, false);
       tm.end(st);
