@@ -161,12 +161,13 @@ __device__ __forceinline__ Node<R> node_from(R gx, R gy, R gz, const SC<R> &s) {
   const R rq = rcpn_pos<R>(gx2 + gy2 + gz2 + s.small);
   const R a = gx2 * rq, b = gy2 * rq, c = gz2 * rq;
   const R eps = s.E0a + s.E0b * (a * a + b * b + c * c);
-  const R f = s.K * eps;
+  // K eps (g_x g_y / |g|^2)(a - b) etc., with K eps / |g|^2 formed once
+  const R frq = s.K * eps * rq;
   Node<R> v;
   v.e2 = eps * eps;
-  v.xy = f * (gx * gy * rq) * (a - b);
-  v.yz = f * (gy * gz * rq) * (b - c);
-  v.zx = f * (gz * gx * rq) * (c - a);
+  v.xy = frq * (gx * gy) * (a - b);
+  v.yz = frq * (gy * gz) * (b - c);
+  v.zx = frq * (gz * gx) * (c - a);
   return v;
 }
 
