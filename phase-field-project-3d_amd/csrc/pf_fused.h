@@ -40,6 +40,9 @@ struct FusedState {
   bool ledger_valid = false;
   double *mass = nullptr, *mslot = nullptr, *mpart = nullptr;
   int nslot = 0;
+  // solute faces of the previous step still to copy (snapshot, new field)
+  const void *cf_src = nullptr;
+  void *cf_dst = nullptr;
 };
 
 // HIP-event pairs recorded around launches on one stream, by kernel kind.
@@ -73,6 +76,10 @@ struct FusedStep {
   bool first = false;  // run temperatura (T of this step) before the sweeps
   bool last = false;   // materialise DPP instead of the next step's T
   int nv = 1;          // phase sweeps per step
+  // the solute faces must be complete when the step ends (last step of a
+  // call, or a step the host or the mass correction reads c after);
+  // otherwise they are copied by extra work-groups of the next sweep A
+  bool flush = true;
 };
 
 int fused_init(FusedState &s, int nx, int ny, int nz, int prec);

@@ -33,7 +33,25 @@ __global__ void __launch_bounds__(TL::NT)
     __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, VAR>())))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
             R *__restrict__ pn, R *__restrict__ cn, const R *__restrict__ Tcur,
-            R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot) {
+            R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot,
+            const R *__restrict__ cfo, R *__restrict__ cfn, int ncf) {
+  const int nb = 8 * g.per_xcd;
+  if ((int)blockIdx.x >= nb) {
+    // the ncf work-groups after the tiles' copy the solute faces of the
+    // previous step (rows only: the sweeps wrote the x faces of the interior
+    // rows) into cfn from its interior and the snapshot cfo; this sweep reads
+    // c only at interior cells, so the two are independent
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const long long cnt = shell_count(nx, ny, nz, 1, false);
+    const int tid = threadIdx.y * TL::TX + threadIdx.x;
+    for (long long t = (long long)((int)blockIdx.x - nb) * TL::NT + tid; t < cnt;
+         t += (long long)ncf * TL::NT) {
+      int i, j, k;
+      shell_flat(t, nx, ny, nz, 1, i, j, k, false);
+      solute_deco_cell(i, j, k, nx, ny, nz, cfo, cfn);
+    }
+    return;
+  }
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
   march_tile<R, MODE, TL, VAR>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
@@ -105,30 +123,35 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
 template <typename R, typename TL, int VAR>
 void launch_march_v(int mode, const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
                     const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                    double *mslot, hipStream_t st) {
-  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd, 1, 1);
+                    double *mslot, hipStream_t st, const R *cfo, R *cfn) {
+  long long ncf = 0;  // solute-face work-groups (sweep A only)
+  if (cfn && mode == kSweepA) {
+    ncf = (shell_count(g.nx, g.ny, g.nz, 1, false) + TL::NT - 1) / TL::NT;
+    ncf = ncf < 1 ? 1 : (ncf > 2048 ? 2048 : ncf);
+  }
+  const dim3 blk(TL::TX, TL::TY, 1), grd(8 * g.per_xcd + (unsigned)ncf, 1, 1);
   const size_t lds = march_lds_bytes<R, TL>(mode);
   if (mode == kSweepB)
     hipLaunchKernelGGL((k_march<R, kSweepB, TL, VAR>), grd, blk, lds, st, g, kp, s, po, co,
-                       pn, cn, Tcur, Tnext, dpp, write_T, mslot);
+                       pn, cn, Tcur, Tnext, dpp, write_T, mslot, nullptr, nullptr, 0);
   else if (mode == kSweepPT)
     hipLaunchKernelGGL((k_march<R, kSweepPT, TL, VAR>), grd, blk, lds, st, g, kp, s, po, co,
-                       pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
+                       pn, cn, Tcur, Tnext, dpp, write_T, nullptr, nullptr, nullptr, 0);
   else
     hipLaunchKernelGGL((k_march<R, kSweepA, TL, VAR & kVarSmooth>), grd, blk, lds, st, g, kp, s,
-                       po, co,
-                       pn, cn, Tcur, Tnext, dpp, write_T, nullptr);
+                       po, co, pn, cn, Tcur, Tnext, dpp, write_T, nullptr, cfo, cfn, (int)ncf);
 }
 
 template <typename R, typename TL>
 hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
-                          int write_T, double *mslot, hipStream_t st) {
+                          int write_T, double *mslot, hipStream_t st, const R *cfo, R *cfn) {
   // the optional paths are compiled only into the kernels that need them
   const int var = (kp.AAA != 0.0 && kp.ruido_tipo == PF_RUIDO_SMOOTH ? kVarSmooth : 0) |
                   (kp.D != 0.0 ? kVarTDiff : 0);
 #define PF_VAR(V) \
-  launch_march_v<R, TL, V>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, st)
+  launch_march_v<R, TL, V>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, st, \
+                           cfo, cfn)
   switch (var) {
     case 0: PF_VAR(0); break;
     case kVarSmooth: PF_VAR(kVarSmooth); break;
@@ -139,13 +162,16 @@ hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R>
   return hipGetLastError();
 }
 
+// cfo/cfn (sweep A only): copy the previous step's solute faces into cfn in
+// the same launch.
 template <typename R>
 hipError_t launch_march(int mode, int tile, const Geo &g, const KParams &kp, const R *po,
                         const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                        double *mslot, hipStream_t st) {
+                        double *mslot, hipStream_t st, const R *cfo = nullptr, R *cfn = nullptr) {
   const SC<R> s = make_sc<R>(kp);
-#define PF_MARCH(TL) \
-  return launch_march_t<R, TL>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, st)
+#define PF_MARCH(TL)                                                                           \
+  return launch_march_t<R, TL>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, \
+                               st, cfo, cfn)
   switch (tile) {
     case 1: PF_MARCH(Tile32x8);
     case 2: PF_MARCH(Tile32x16);
@@ -228,9 +254,13 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
     const R *po = (const R *)phi[*phi_old];
     R *pn = (R *)phi[pnw];
     if (sw < fs.nv) {
+      // with the previous step's solute faces, if still to copy
       tm.begin(PF_KT_SWEEP_A, st);
       e = launch_march<R>(kSweepA, S.tile_a, ga, k, po, (const R *)c[*c_old], pn, nullptr,
-                          nullptr, nullptr, nullptr, 0, nullptr, st);
+                          nullptr, nullptr, nullptr, 0, nullptr, st, (const R *)S.cf_src,
+                          (R *)S.cf_dst);
+      S.cf_src = nullptr;
+      S.cf_dst = nullptr;
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
       tm.begin(PF_KT_BOUNDARY, st);
@@ -267,10 +297,15 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
       }
       tm.begin(PF_KT_BOUNDARY, st);
       // x faces written by the sweeps; the split form's phase decoration ran
-      // in the solute kernel's launch
+      // in the solute kernel's launch; the solute faces are copied now or by
+      // the next sweep A
       e = S.split_b ? hipSuccess : launch_phase_boundary<R>(k, po, pn, st, false);
-      if (e == hipSuccess)
+      if (e == hipSuccess && (fs.flush || fs.nv < 2)) {
         e = launch_solute_boundary<R>(k, (const R *)c[*c_old], (R *)c[cnw], st, false);
+      } else if (e == hipSuccess) {
+        S.cf_src = c[*c_old];
+        S.cf_dst = c[cnw];
+      }
       tm.end(st);
       if (e != hipSuccess) return PF_E_HIP;
     }

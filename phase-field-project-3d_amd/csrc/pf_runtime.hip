@@ -716,6 +716,7 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
       fs.first = first;
       fs.last = last;
       fs.nv = nv;
+      fs.flush = last || (fst.mass_every > 0 && it % fst.mass_every == 0);
       rc = pf::fused_step(ctx->fused, k_temp, k, fs, ctx->phi.buf, &ctx->phi.old_idx,
                           &ctx->phi.alias, ctx->c.buf, &ctx->c.old_idx, &ctx->c.alias, ctx->T.buf,
                           &ctx->T.old_idx, &ctx->T.alias, ctx->dpp, pr, ctx->stream,
