@@ -222,6 +222,35 @@ def test_fused_deterministic_and_matches_dropin():
     assert _abs(a["T_new"], c["T_new"]) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", [capi.PF_FP64, capi.PF_FP32_MIXED])
+def test_fused_steps_split_over_calls(precision):
+    """A step's solute face copy may run inside the next step's first sweep;
+    a call's last step and a step followed by the solute-mass correction
+    copy them at once.  Splitting a run into calls of 1-5 steps and reading
+    the fields between calls must give the one-call run bit for bit; with
+    the correction every 4 steps, calls that end on correction steps (the
+    ledger is then folded at the same steps) must too."""
+    deck = _growth_deck(n=(40, 37, 45))
+    print(f"[split calls] precision {precision}", flush=True)
+    for every, splits in ((0, ((12,), (1, 4, 2, 5), (3, 3, 1, 1, 4))),
+                          (4, ((12,), (4, 4, 4), (8, 4)))):
+        out = []
+        for chunks in splits:
+            sim = Simulation(deck, precision=precision)
+            sim.ctx.set_mass_correction(every)
+            for n in chunks:
+                sim.run(n)
+                f = sim.fields()
+            assert sim.p.it == 12
+            out.append((f, sim.diagnostics()))
+            sim.close()
+        a, da = out[0]
+        for (b, db), chunks in zip(out[1:], splits[1:]):
+            for nm in ("phi_new", "c_new", "T_new", "phi_old", "c_old", "T_old"):
+                assert np.array_equal(a[nm], b[nm]), (every, chunks, nm)
+            assert np.array_equal(da, db), (every, chunks)
+
+
 @pytest.mark.parametrize("n", [(6, 6, 6), (7, 9, 8), (6, 13, 11), (17, 6, 7), (35, 19, 9)])
 def test_fused_small_grids_match_dropin(n):
     """Smallest grids the fused sweeps run (every extent >= 6), and extents
