@@ -148,3 +148,66 @@ def run_steps(p, L, nsteps, F, probes=False):
     load().pfo_run_steps(C.byref(p), C.byref(L), nsteps, *F.as_list(),
                          pr.ctypes.data_as(C.c_void_p) if pr is not None else None)
     return pr
+
+
+# ---- Karma model (karma/karma3d.f90), oracle/karma_oracle.c ----------------
+
+class KarmaPar(C.Structure):
+    _fields_ = [("lambda_", C.c_double), ("eps4", C.c_double), ("dx", C.c_double),
+                ("dt", C.c_double), ("Fnoise", C.c_double), ("k", C.c_double),
+                ("a2", C.c_double)]
+
+
+def karma_par(lambda_=4.0, eps4=0.02, dx=0.8, dt=0.008, Fnoise=0.0, k=0.14):
+    """The step parameters of karma.nml (karma3d.f90:71-86); a2 = 0.6267."""
+    return KarmaPar(lambda_, eps4, dx, dt, Fnoise, k, 0.6267)
+
+
+def karma_omega(k=0.14, Tm=933.3, mliq=640.0, C0=0.0196, TP=909.5):
+    """(c_l0, Omega) of karma3d.f90:112-114."""
+    cl0 = (Tm - TP) / mliq
+    return cl0, (cl0 - C0) / ((1.0 - k) * cl0)
+
+
+def _karma_lib():
+    lib = load()
+    if not getattr(lib, "_karma_set", False):
+        i32, f64 = C.c_int32, C.c_double
+        lib.pfo_karma_init.argtypes = [i32, f64, f64, f64, _dp, _dp]
+        lib.pfo_karma_init.restype = None
+        lib.pfo_karma_mirror.argtypes = [i32, _dp]
+        lib.pfo_karma_mirror.restype = None
+        lib.pfo_karma_noise.argtypes = [i32, i32, i32, i32]
+        lib.pfo_karma_noise.restype = f64
+        lib.pfo_karma_step.argtypes = [i32, C.POINTER(KarmaPar), i32, _dp, _dp]
+        lib.pfo_karma_step.restype = None
+        lib._karma_set = True
+    return lib
+
+
+def karma_init(n, dx=0.8, R0=8.0, Omega=None):
+    """Padded (n+2)^3 phi and U (index i + (n+2)(j + (n+2)k)) of the corner
+    seed."""
+    if Omega is None:
+        Omega = karma_omega()[1]
+    s = (n + 2) ** 3
+    phi, U = np.zeros(s), np.zeros(s)
+    _karma_lib().pfo_karma_init(n, dx, R0, Omega, phi, U)
+    return phi, U
+
+
+def karma_step(n, par, it, phi, U):
+    _karma_lib().pfo_karma_step(n, C.byref(par), it, phi, U)
+
+
+def karma_mirror(n, f):
+    _karma_lib().pfo_karma_mirror(n, f)
+
+
+def karma_noise(i, j, k, it):
+    return _karma_lib().pfo_karma_noise(i, j, k, it)
+
+
+def karma_interior(n, f):
+    """The cells 1..n of a padded field as an (n, n, n) array indexed [k, j, i]."""
+    return np.asarray(f).reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]

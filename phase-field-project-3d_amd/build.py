@@ -25,7 +25,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/llvm/bin/flang")
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 
-HIP_SOURCES = ["pf_kernels_dropin.hip", "pf_fused.hip", "pf_batch.hip", "pf_runtime.hip",
+HIP_SOURCES = ["pf_kernels_dropin.hip", "pf_fused.hip", "pf_batch.hip", "pf_runtime.hip", "pf_karma.hip",
                "pf_host.cpp"]
 FORTRAN_SOURCES = ["pf_capi.f90", "pf_state.f90", "pf_kernels.f90", "pf_io.f90", "principal.f90"]
 
