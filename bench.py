@@ -285,6 +285,69 @@ def serving_study(n=24, rate=4.0):
             "solves_per_s_ratio": round(sb["solves_per_s"] / ss["solves_per_s"], 3)}
 
 
+def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
+    """The quantitative Karma model (karma/karma3d.f90) at its own default
+    size, 200^3 with the namelist defaults: GPU steps timed after a warm-up,
+    and the reference program compiled from its source (oracle/_ref) on a
+    bounded sample (96^3, 200 steps, this host's cores), as it times itself
+    (its loop wall clock, printed before its output file is written)."""
+    import re
+    import subprocess
+    import tempfile
+    import time
+
+    import numpy as np
+
+    from pfamd.karma import Karma, KarmaDeck
+
+    m = Karma(KarmaDeck(N=n))
+    try:
+        m.run(warm)
+        m.synchronize()
+        t0 = time.perf_counter()
+        m.run(steps)
+        m.synchronize()
+        wall = time.perf_counter() - t0
+        phi, _, it = m.fields()
+    finally:
+        m.close()
+    mlups = n ** 3 * steps / wall / 1e6
+    # per cell-step: phase kernel reads phi, U and writes dphi, phi'; the
+    # solute kernel reads phi, U, dphi and writes U' (ghost copies aside)
+    out = {"config": f"{n}^3, karma.nml defaults (lambda 4, eps4 0.02, dx 0.8, dt 0.008, R0 8), "
+                     f"{warm} warm-up + {steps} timed steps, FP64",
+           "mlups": round(mlups, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+           "alg_bytes_per_cell_step": 64,
+           "achieved_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
+           "frac_of_hbm_peak": round(mlups * 1e6 * 64 / 8.0e12, 4),
+           "tip_cells_on_x_axis": int(np.max(np.nonzero(phi[0, 0, :] > 0)[0], initial=-1) + 1),
+           "reference_published_mlups": 200.0,
+           "reference_published_note": "200^3 for 45k steps in ~30 min on the reference's GPU"}
+    log(f"[bench] karma {n}^3: {out['mlups']} MLUPS, {out['ms_per_step']} ms/step")
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_karma3d")
+    if os.path.exists(exe):
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        threads = max(1, min(16, threads))
+        with tempfile.TemporaryDirectory() as d:
+            with open(os.path.join(d, "karma.nml"), "w") as f:
+                f.write(f"&karma\n N = {cpu_n}\n nsteps = {cpu_steps}\n nout = {cpu_steps}\n"
+                        " tip_stop = 2.0\n/\n")
+            os.makedirs(os.path.join(d, "output"))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+                       OMP_PLACES="cores", OMP_TARGET_OFFLOAD="DISABLED")
+            r = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=300)
+        w = re.findall(r"wall=\s*([0-9.]+)\s*s", r.stdout)
+        if r.returncode == 0 and w and float(w[-1]) > 0:
+            cpu = cpu_n ** 3 * cpu_steps / float(w[-1]) / 1e6
+            out["cpu_reference"] = {"value": round(cpu, 2), "unit": "MLUPS", "cores": threads,
+                                    "kind": "reference",
+                                    "sample": f"{cpu_n}^3, {cpu_steps} steps, its own loop wall "
+                                              f"clock {w[-1]} s (0.1 s resolution)"}
+            out["gpu_vs_cpu"] = round(mlups / cpu, 1)
+            log(f"[bench] karma CPU reference: {cpu:.1f} MLUPS on {threads} threads")
+    return out
+
+
 def cpu_baseline():
     """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
     kernels compiled from source with flang -O2 -fopenmp), timed on this
@@ -364,6 +427,7 @@ def main(argv=None):
     ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-karma", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling")
     ap.add_argument("--tiling", type=parse_tiling, default=None,
@@ -444,6 +508,9 @@ def main(argv=None):
     if not a.no_serving:
         log("[bench] serving (Poisson stream)")
         out["serving"] = serving_study()
+    if not a.no_karma:
+        log("[bench] Karma model")
+        out["karma"] = karma_study()
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (reference OpenMP path)")
         cb = cpu_baseline()
