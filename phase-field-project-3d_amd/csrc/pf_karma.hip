@@ -5,8 +5,9 @@
 //   step) -> ghosts of dphi -> solute equation with the anti-trapping current
 //   (U of the next step), then the two buffer pairs swap.
 // One thread per cell, x fastest (32x8 threads per work-group).  Every
-// expression keeps the reference's operation order with FP contraction off,
-// so each cell rounds as the reference's CPU build rounds it.
+// expression keeps the reference's operation order with FP contraction off
+// (division and square root correctly rounded), so each cell rounds as a
+// plain IEEE-754 C build of the same expressions does.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
