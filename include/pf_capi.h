@@ -232,16 +232,21 @@ int pf_get_tiling(const pf_ctx *ctx, int32_t out[4]);
 int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, double *ms);
 /* Solute-mass correction of the fused path (no counterpart in the
  * reference, which is FP64 throughout).  The context keeps an FP64 ledger of
- * the interior ([2,n-1]^3) solute mass: the FP64 sum of the field when a run
- * starts plus the FP64 sum of every increment the solute update computes.
- * Every `every` steps (p->it a multiple of it) the FP64 interior sum of the
- * stored field is compared with the ledger and the interior is rescaled by
- * ledger/sum (the FP64 product rounded stochastically to FP32, unbiased,
- * drawn from a counter hash of cell and step), removing the drift that FP32
- * storage rounding accumulates.  The ledger follows the scheme's own
- * increments: the reference's solute update is not conservative by itself
- * (the face mobility takes the centre cell's D), and that change is kept.  Default: 100 for PF_FP32_MIXED, 0 (off) for PF_FP64.
- * Setting it restarts the ledger from the field at the next pf_run_steps. */
+ * the interior ([2,n-1]^3) solute mass: the FP64 mass when it started plus
+ * the sum of every increment the solute update computes, each rounded to
+ * 2^-44 and summed as a 64-bit integer (exact, so the sum does not depend on
+ * the tiling, on how a run is split into calls or on the batch it runs in).
+ * It starts from the FP64 interior sum of the c_old array given to pf_init
+ * while no step has run, else from the stored field.  Every `every` steps
+ * (p->it a multiple of it) the FP64 interior sum of the stored field is
+ * compared with the ledger and the interior is rescaled by ledger/sum (the
+ * FP64 product rounded stochastically to FP32, unbiased, drawn from a
+ * counter hash of cell and step), removing the drift that FP32 storage
+ * rounding accumulates (and the FP32 rounding of the initial field).  The
+ * ledger follows the scheme's own increments: the reference's solute update
+ * is not conservative by itself (the face mobility takes the centre cell's
+ * D), and that change is kept.  Default: 100 for PF_FP32_MIXED, 0 (off) for
+ * PF_FP64.  Setting it restarts the ledger at the next pf_run_steps. */
 int pf_set_mass_correction(pf_ctx *ctx, int32_t every);
 /* Synchronous: out = {ledger, interior sum at the last correction, last
  * factor ledger/sum}; zeros before the first run with the correction on. */
@@ -257,7 +262,10 @@ int pf_synchronize(pf_ctx *ctx);
  * and K steps are captured once in a hipGraph and replayed.  No counterpart
  * in the reference, which runs one problem per process.  An instance's
  * result does not depend on the other instances or on the tiling: it equals
- * pf_run_steps on the same instance (tested), except that DPP is not kept.
+ * pf_run_steps on the same instance (tested), except that DPP is not kept --
+ * including the solute-mass correction, which an instance keeps as
+ * pf_run_steps does (default every 100 steps for PF_FP32_MIXED, off for
+ * PF_FP64; the ledger starts from the FP64 c given to pf_batch_add).
  * Instances of one batch share n_varreduras; every extent must be >= 6. */
 typedef struct pf_batch pf_batch;
 /* hbm_budget: bytes of instance fields the batch may hold (0 = 90% of the
@@ -308,6 +316,12 @@ int pf_batch_set_group_target(pf_batch *b, int32_t target_groups);
  * T_new reported for the instances is not kept until their next step).
  * Synchronous. */
 int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
+/* The instance's solute-mass correction period (as pf_set_mass_correction;
+ * 0 = off); only before its first step (PF_E_STATE after). */
+int pf_batch_set_mass_correction(pf_batch *b, int32_t slot, int32_t every);
+/* Synchronous: out = {ledger, interior sum at the last correction, last
+ * factor ledger/sum} of the instance (as pf_mass_ledger). */
+int pf_batch_mass_ledger(pf_batch *b, int32_t slot, double out[3]);
 
 /* ---- measurement ----------------------------------------------------------
  * Per-kernel durations from HIP events recorded around each launch on the
@@ -321,10 +335,11 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
 #define PF_KT_MASS 5        /* solute-mass correction (FP32-mixed form)     */
 #define PF_KT_SOLUTE 6      /* solute kernel of the split form of sweep B   */
 #define PF_KT_COUNT 7
-/* STREAM-style copy of `bytes` bytes (FP64 elements, coalesced, one read
- * and one write per element) repeated `iters` times on the current device;
- * reports the copy bandwidth (read + write bytes / average kernel time).
- * Used as the measured HBM peak and to calibrate HBM byte counters. */
+/* STREAM-style copy of `bytes` bytes (16-byte elements, coalesced, one read
+ * and one write per element) repeated `iters` times on the current device,
+ * in three forms (1, 4 and 8 loads in flight per thread); reports the best
+ * copy bandwidth (read + write bytes / average kernel time).  Used as the
+ * measured HBM peak and to calibrate HBM byte counters. */
 int pf_stream_copy(size_t bytes, int32_t iters, double *gbs);
 /* on != 0 starts recording (and clears earlier records); 0 stops. */
 int pf_timing_enable(pf_ctx *ctx, int32_t on);

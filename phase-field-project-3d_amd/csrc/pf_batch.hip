@@ -29,6 +29,7 @@
 #include "pf_common.h"
 #include "pf_kernels.h"
 #include "pf_march.h"
+#include "pf_mass.h"
 
 namespace pf {
 namespace {
@@ -54,8 +55,15 @@ struct InstDev {
   // host-loop scalars (principal.f90:96-157)
   double T0, taxa, Tm, Me;
   float Sigma, Beta, ajuste, time, Ke;
-  int pad;
+  int mass_every;       // solute-mass correction period (0 = off), as pf_set_mass_correction
+  MassLedger m;         // the instance's solute-mass ledger (pf_mass.h)
 };
+
+// The instance corrects its solute mass after the current step.
+template <typename R>
+__device__ __forceinline__ bool mass_due(const InstDev<R> &I) {
+  return I.active && I.mass_every > 0 && I.k.it % I.mass_every == 0;
+}
 
 struct TileRef {
   int inst, bx, by, sg;
@@ -107,7 +115,7 @@ template <typename R, bool SWEEP_B, typename TL, int VAR>
 __global__ void __launch_bounds__(TL::NT)
     __attribute__((amdgpu_waves_per_eu(march_waves<R, SWEEP_B ? kSweepB : kSweepA, VAR>())))
     k_march_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
-                  const InstDev<R> *__restrict__ inst, int sweep) {
+                  InstDev<R> *__restrict__ inst, int sweep, int ledger) {
   const int b = blockIdx.x;
   const int t = (b & 7) * per_xcd + (b >> 3);  // each XCD gets a contiguous run
   if (t >= ntiles) return;
@@ -118,7 +126,8 @@ __global__ void __launch_bounds__(TL::NT)
   if constexpr (SWEEP_B)
     march_tile<R, kSweepB, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        I.c[I.co ^ 1], I.T[I.to], I.T[I.to ^ 1], nullptr, 1,
-                                       nullptr, d.bx, d.by, d.sg);
+                                       ledger && I.mass_every > 0 ? &inst[d.inst].m.acc : nullptr,
+                                       d.bx, d.by, d.sg);
   else
     march_tile<R, kSweepA, TL, VAR>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, d.bx,
@@ -150,46 +159,119 @@ __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ ch
 }
 
 template <typename R, typename TL, int VAR>
-void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, const InstDev<R> *inst,
-                          int sweep, hipStream_t st) {
+void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, InstDev<R> *inst,
+                          int sweep, int ledger, hipStream_t st) {
   const int per_xcd = (ntiles + 7) / 8;
   const size_t lds = march_lds_bytes<R, TL>(sweep_b ? kSweepB : kSweepA);
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
   if (sweep_b)
     hipLaunchKernelGGL((k_march_batch<R, true, TL, VAR>), grd, blk, lds, st, tiles, ntiles,
-                       per_xcd, inst, sweep);
+                       per_xcd, inst, sweep, ledger);
   else
     hipLaunchKernelGGL((k_march_batch<R, false, TL, VAR & kVarSmooth>), grd, blk, lds, st, tiles,
-                       ntiles, per_xcd, inst, sweep);
+                       ntiles, per_xcd, inst, sweep, 0);
 }
 
 // var: the optional paths (kVar*) some instance of the group needs (the
-// kernels without a path must not see an instance that needs it)
+// kernels without a path must not see an instance that needs it); ledger:
+// sweep B adds its solute increments to the instances' ledgers (off for the
+// autotuner's timing launches)
 template <typename R, typename TL>
 hipError_t launch_batch_march(bool sweep_b, int var, const TileRef *tiles, int ntiles,
-                              const InstDev<R> *inst, int sweep, hipStream_t st) {
+                              InstDev<R> *inst, int sweep, int ledger, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
   switch (var) {
-    case 0: launch_batch_march_v<R, TL, 0>(sweep_b, tiles, ntiles, inst, sweep, st); break;
+    case 0: launch_batch_march_v<R, TL, 0>(sweep_b, tiles, ntiles, inst, sweep, ledger, st); break;
     case kVarSmooth:
-      launch_batch_march_v<R, TL, kVarSmooth>(sweep_b, tiles, ntiles, inst, sweep, st);
+      launch_batch_march_v<R, TL, kVarSmooth>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
       break;
     case kVarTDiff:
-      launch_batch_march_v<R, TL, kVarTDiff>(sweep_b, tiles, ntiles, inst, sweep, st);
+      launch_batch_march_v<R, TL, kVarTDiff>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
       break;
-    default: launch_batch_march_v<R, TL, kVarAll>(sweep_b, tiles, ntiles, inst, sweep, st);
+    default:
+      launch_batch_march_v<R, TL, kVarAll>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
   }
   return hipGetLastError();
 }
 
 template <typename R>
 hipError_t launch_batch_shape(int shape, bool sweep_b, int var, const TileRef *tiles, int ntiles,
-                              const InstDev<R> *inst, int sweep, hipStream_t st) {
+                              InstDev<R> *inst, int sweep, int ledger, hipStream_t st) {
   switch (shape) {
-    case 1: return launch_batch_march<R, Tile32x8>(sweep_b, var, tiles, ntiles, inst, sweep, st);
-    case 2: return launch_batch_march<R, Tile32x16>(sweep_b, var, tiles, ntiles, inst, sweep, st);
-    case 3: return launch_batch_march<R, Tile16x16>(sweep_b, var, tiles, ntiles, inst, sweep, st);
-    default: return launch_batch_march<R, Tile64x8>(sweep_b, var, tiles, ntiles, inst, sweep, st);
+    case 1:
+      return launch_batch_march<R, Tile32x8>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
+    case 2:
+      return launch_batch_march<R, Tile32x16>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
+    case 3:
+      return launch_batch_march<R, Tile16x16>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
+    default:
+      return launch_batch_march<R, Tile64x8>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
+  }
+}
+
+// ---- solute-mass correction of the instances due this step (pf_mass.h) -----
+// The same decomposition and trees as the single-instance correction
+// (pf_fused.hip), per instance, so a corrected instance gives the same bits
+// batched and alone.  Each kernel covers every instance and skips those not
+// due, so a captured step needs no host decision.
+
+// block b: partial b of every due instance's interior sum of its new c
+template <typename R>
+__global__ void __launch_bounds__(256) k_bmass_interior(const InstDev<R> *__restrict__ inst, int n,
+                                                        double *__restrict__ part) {
+  __shared__ double sh[256];
+  for (int q = 0; q < n; ++q) {
+    const InstDev<R> &I = inst[q];
+    if (!mass_due(I)) continue;  // uniform over the block
+    const double t = block_sum256(mass_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz,
+                                            (int)blockIdx.x), sh);
+    if (threadIdx.x == 0) part[(size_t)q * kMassBlocks + blockIdx.x] = t;
+  }
+}
+
+// block q: instance q's interior sum and factor
+template <typename R>
+__global__ void __launch_bounds__(256) k_bmass_update(InstDev<R> *__restrict__ inst, int n,
+                                                      const double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x;
+  if (q >= n || !mass_due(inst[q])) return;
+  const double S = mass_fold(part + (size_t)q * kMassBlocks, sh);
+  if (threadIdx.x == 0) {
+    MassLedger &m = inst[q].m;
+    m.S = S;
+    m.f = ledger_value(m) / S;
+  }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restrict__ inst, int n) {
+  for (int q = 0; q < n; ++q) {
+    const InstDev<R> &I = inst[q];
+    if (!mass_due(I)) continue;
+    mass_scale_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, I.m.f,
+                    (int)blockIdx.x);
+  }
+}
+
+// the face planes of a corrected instance from their interior sources
+// (concentracao_soluto.f90:116-124, as after the single-instance correction)
+constexpr int kRefreshBlocks = 512;
+template <typename R>
+__global__ void __launch_bounds__(256) k_bmass_refresh(const InstDev<R> *__restrict__ inst, int n) {
+  for (int q = 0; q < n; ++q) {
+    const InstDev<R> &I = inst[q];
+    if (!mass_due(I)) continue;
+    const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
+    R *c = I.c[I.co ^ 1];
+    int i, j, k;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; shell_flat(t, nx, ny, nz, 1, i, j, k);
+         t += (long long)kRefreshBlocks * 256) {
+      int si = i, sj = j, sk = k;
+      solute_source(si, sj, sk, nx, ny, nz);
+      if (si > 1 && si < nx && sj > 1 && sj < ny && sk > 1 && sk < nz)
+        c[cidx(i, j, k, nx, ny)] = c[cidx(si, sj, sk, nx, ny)];
+    }
   }
 }
 
@@ -233,6 +315,8 @@ struct pf_batch_slot {
   size_t n = 0, bytes = 0;
   void *buf[6] = {};  // phi0, phi1, c0, c1, T0, T1
   int steps_total = 0, steps_left = 0;
+  int mass_every = 0;      // solute-mass correction period
+  double host_mass = 0.0;  // FP64 interior mass of the admitted c field
 };
 
 struct pf_batch {
@@ -277,6 +361,10 @@ struct pf_batch {
   double *diag_part = nullptr;
   int diag_cap = 0;
   double *pinned = nullptr;
+  // partial interior sums of the solute-mass correction (kMassBlocks per
+  // instance of the larger per-precision array)
+  double *mass_part = nullptr;
+  size_t mass_part_cap = 0;
 };
 
 namespace {
@@ -334,6 +422,8 @@ pf::InstDev<R> make_inst(const pf_batch_slot &s) {
   I.ajuste = s.L.ajuste;
   I.time = s.L.time;
   I.Ke = s.p.Ke;
+  I.mass_every = s.mass_every;
+  I.m.L0 = s.host_mass;  // the ledger starts from the caller's FP64 field
   return I;
 }
 
@@ -413,6 +503,14 @@ int rebuild(pf_batch *b) {
     HIP_OK(hipMalloc((void **)&b->chunks_dev, chunks.size() * sizeof(pf::Chunk)));
     b->chunks_cap = chunks.size();
   }
+  const size_t nmp = (size_t)std::max(b->slot_of[0].size(), b->slot_of[1].size()) * pf::kMassBlocks;
+  if (nmp > b->mass_part_cap) {
+    if (b->mass_part) HIP_OK(hipFree(b->mass_part));
+    b->mass_part = nullptr;
+    b->mass_part_cap = 0;
+    HIP_OK(hipMalloc((void **)&b->mass_part, nmp * sizeof(double)));
+    b->mass_part_cap = nmp;
+  }
   if (!tiles.empty())
     HIP_OK(hipMemcpyAsync(b->tiles_dev, tiles.data(), tiles.size() * sizeof(pf::TileRef),
                           hipMemcpyHostToDevice, b->stream));
@@ -468,10 +566,10 @@ int enqueue_step(pf_batch *b) {
       hipError_t e;
       if (g.prec == PF_FP64)
         e = pf::launch_batch_shape<double>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
-                                           (const pf::InstDev<double> *)b->inst_dev[0], sweep, st);
+                                           (pf::InstDev<double> *)b->inst_dev[0], sweep, 1, st);
       else
         e = pf::launch_batch_shape<float>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
-                                          (const pf::InstDev<float> *)b->inst_dev[1], sweep, st);
+                                          (pf::InstDev<float> *)b->inst_dev[1], sweep, 1, st);
       if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
     }
     return PF_OK;
@@ -500,6 +598,34 @@ int enqueue_step(pf_batch *b) {
   if ((rc = sweeps(true, nv)) != PF_OK) return rc;
   if ((rc = decos(0, nv)) != PF_OK) return rc;
   if ((rc = decos(1, nv)) != PF_OK) return rc;
+  // solute-mass corrections of the instances due this step (after the
+  // step's face copy, as in the single-instance loop)
+  for (int prec = 0; prec < 2; ++prec) {
+    const int n = (int)b->slot_of[prec].size();
+    bool any = false;
+    for (int id : b->slot_of[prec]) any = any || b->slots[id].mass_every > 0;
+    if (!n || !any) continue;
+    if (prec == PF_FP64) {
+      auto *I = (pf::InstDev<double> *)b->inst_dev[prec];
+      hipLaunchKernelGGL(pf::k_bmass_interior<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
+                         n, b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, I, n,
+                         b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n);
+      hipLaunchKernelGGL(pf::k_bmass_refresh<double>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
+                         I, n);
+    } else {
+      auto *I = (pf::InstDev<float> *)b->inst_dev[prec];
+      hipLaunchKernelGGL(pf::k_bmass_interior<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
+                         n, b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, I, n,
+                         b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n);
+      hipLaunchKernelGGL(pf::k_bmass_refresh<float>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
+                         I, n);
+    }
+    HIP_OK(hipGetLastError());
+  }
   for (int prec = 0; prec < 2; ++prec) {
     const int n = (int)b->slot_of[prec].size();
     if (!n) continue;
@@ -637,6 +763,7 @@ int pf_batch_destroy(pf_batch *b) {
   if (b->tiles_dev) (void)hipFree(b->tiles_dev);
   if (b->chunks_dev) (void)hipFree(b->chunks_dev);
   if (b->diag_part) (void)hipFree(b->diag_part);
+  if (b->mass_part) (void)hipFree(b->mass_part);
   if (b->pinned) (void)hipHostFree(b->pinned);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -677,6 +804,9 @@ int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nste
     return fail_msg(PF_E_NOMEM, "instance needs %zu bytes, %zu of %zu in use", s.bytes, b->used,
                     b->budget);
   s.steps_total = s.steps_left = nsteps;
+  // the solute-mass ledger as pf_run_steps keeps it by default
+  s.mass_every = s.prec == PF_FP32_MIXED ? 100 : 0;
+  s.host_mass = pf::host_interior_mass(c, s.nx, s.ny, s.nz);
   for (int f = 0; f < 6; ++f) {
     if (hipMalloc(&s.buf[f], s.n * esz) != hipSuccess) {
       for (int g = 0; g < f; ++g) (void)hipFree(s.buf[g]);
@@ -992,13 +1122,14 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
         if (r == 1) e = hipEventRecord(e0, b->stream);
         for (const auto &g : b->groups) {
           if (e != hipSuccess) break;
+          // timing launches: the ledgers are left alone
           if (g.prec == PF_FP64)
             e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
-                                               (const pf::InstDev<double> *)b->inst_dev[0], 1,
+                                               (pf::InstDev<double> *)b->inst_dev[0], 1, 0,
                                                b->stream);
           else
             e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
-                                              (const pf::InstDev<float> *)b->inst_dev[1], 1,
+                                              (pf::InstDev<float> *)b->inst_dev[1], 1, 0,
                                               b->stream);
         }
       }
@@ -1023,6 +1154,45 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
   (void)hipEventDestroy(e1);
   b->dirty = true;
   return rc;
+}
+
+int pf_batch_set_mass_correction(pf_batch *b, int32_t slot, int32_t every) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  if (every < 0) return fail_msg(PF_E_ARG, "negative correction period");
+  pf_batch_slot &s = b->slots[slot];
+  if (s.steps_left != s.steps_total)
+    return fail_msg(PF_E_STATE, "the correction period is set before the instance's first step");
+  HIP_OK(hipStreamSynchronize(b->stream));
+  const size_t off = s.prec == PF_FP64 ? offsetof(pf::InstDev<double>, mass_every)
+                                       : offsetof(pf::InstDev<float>, mass_every);
+  HIP_OK(hipMemcpyAsync((char *)b->inst_dev[s.prec] + (size_t)s.idx * inst_size(s.prec) + off,
+                        &every, sizeof every, hipMemcpyHostToDevice, b->stream));
+  HIP_OK(hipStreamSynchronize(b->stream));
+  s.mass_every = every;
+  b->dirty = true;  // the captured steps launch the correction kernels only where needed
+  return PF_OK;
+}
+
+int pf_batch_mass_ledger(pf_batch *b, int32_t slot, double out[3]) {
+  int rc = check_slot(b, slot);
+  if (rc != PF_OK) return rc;
+  if (!out) return fail_msg(PF_E_ARG, "null output");
+  const pf_batch_slot &s = b->slots[slot];
+  pf::MassLedger m;
+  if (s.prec == PF_FP64) {
+    pf::InstDev<double> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    m = I.m;
+  } else {
+    pf::InstDev<float> I;
+    if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
+    m = I.m;
+  }
+  out[0] = pf::ledger_value(m);
+  out[1] = m.S;
+  out[2] = m.f;
+  return PF_OK;
 }
 
 }  // extern "C"

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "pf_common.h"
+#include "pf_mass.h"
 
 namespace pf {
 
@@ -33,13 +34,16 @@ struct FusedState {
   bool split_b = false;
   int tile_s = 0, seg_s = 0;
   bool enabled = false;
-  // solute-mass ledger (pf_fused.hip): period of the correction in steps
-  // (0 = off), device scalars {ledger, interior sum, factor}, one FP64 slot
-  // per sweep-B work-group, partial interior sums
+  // solute-mass ledger (pf_mass.h): period of the correction in steps
+  // (0 = off), the device ledger, partial interior sums; the FP64 interior
+  // mass of the host fields the context was created from (the ledger's
+  // start while no step has changed them)
   int mass_every = 0;
   bool ledger_valid = false;
-  double *mass = nullptr, *mslot = nullptr, *mpart = nullptr;
-  int nslot = 0;
+  MassLedger *mass = nullptr;
+  double *mpart = nullptr;
+  double host_mass = 0.0;
+  bool host_mass_ok = false;
   // solute faces of the previous step still to copy (snapshot, new field)
   const void *cf_src = nullptr;
   void *cf_dst = nullptr;
@@ -104,14 +108,15 @@ struct AutotuneResult {
 };
 int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, void *T_old,
                    int reps, hipStream_t stream, AutotuneResult *res);
-// Solute-mass ledger: prepare before a run (sizes slots, starts the ledger
-// from the current field when it is not valid), fold the slots into the
-// ledger after a run, correct the field (every mass_every steps), read
-// {ledger, interior sum at the last correction, last factor}.
+// Solute-mass ledger: prepare before a run (starts the ledger when it is not
+// valid: from host_mass while that is valid, else from the stored field),
+// correct the field (every mass_every steps), read {ledger, interior sum at
+// the last correction, last factor}, read / restore the device record.
 int fused_mass_prepare(FusedState &s, const void *c_old, hipStream_t stream);
-int fused_mass_fold(FusedState &s, hipStream_t stream);
 int fused_mass_correct(FusedState &s, const KParams &k, void *c, hipStream_t stream);
 int fused_mass_read(FusedState &s, double out[3], hipStream_t stream);
+int fused_mass_state(FusedState &s, MassLedger *out, hipStream_t stream);
+int fused_mass_restore(FusedState &s, const MassLedger &m, hipStream_t stream);
 void fused_free(FusedState &s);
 size_t fused_bytes(const FusedState &s);
 bool fused_supported(const FusedState &s, const pf_params &p);

@@ -10,6 +10,7 @@
 #include "pf_fused.h"
 #include "pf_kernels.h"
 #include "pf_march.h"
+#include "pf_mass.h"
 #include "pf_math.h"
 
 namespace pf {
@@ -33,8 +34,9 @@ __global__ void __launch_bounds__(TL::NT)
     __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, VAR>())))
     k_march(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
             R *__restrict__ pn, R *__restrict__ cn, const R *__restrict__ Tcur,
-            R *__restrict__ Tnext, R *__restrict__ dpp, int write_T, double *__restrict__ mslot,
-            const R *__restrict__ cfo, R *__restrict__ cfn, int ncf) {
+            R *__restrict__ Tnext, R *__restrict__ dpp, int write_T,
+            unsigned long long *__restrict__ macc, const R *__restrict__ cfo, R *__restrict__ cfn,
+            int ncf) {
   const int nb = 8 * g.per_xcd;
   if ((int)blockIdx.x >= nb) {
     // the ncf work-groups after the tiles' copy the solute faces of the
@@ -54,8 +56,8 @@ __global__ void __launch_bounds__(TL::NT)
   }
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
-  march_tile<R, MODE, TL, VAR>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
-                                  mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
+  march_tile<R, MODE, TL, VAR>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, macc, bx, by,
+                                  sg);
 }
 
 // The solute update alone (split form of sweep B).  The ndeco work-groups
@@ -67,7 +69,7 @@ __global__ void __launch_bounds__(TL::NT)
 template <typename R, typename TL>
 __global__ void __launch_bounds__(TL::NT)
     k_solute(Geo g, KParams kp, SC<R> s, const R *__restrict__ po, const R *__restrict__ co,
-             R *__restrict__ cn, double *__restrict__ mslot, const R *__restrict__ dpo,
+             R *__restrict__ cn, unsigned long long *__restrict__ macc, const R *__restrict__ dpo,
              R *__restrict__ dpn, int ndeco) {
   const int nb = 8 * g.per_xcd;
   if ((int)blockIdx.x >= nb) {  // decoration work-group
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(TL::NT)
   }
   int bx, by, sg;
   if (!tile_of(g, bx, by, sg)) return;
-  solute_tile<R, TL>(g, kp, s, po, co, cn, mslot ? mslot + blockIdx.x : nullptr, bx, by, sg);
+  solute_tile<R, TL>(g, kp, s, po, co, cn, macc, bx, by, sg);
 }
 
 using Tile64x8 = Tile<64, 8>;
@@ -123,7 +125,7 @@ int pick_seg(int nx, int ny, int ncell, int tile, int target) {
 template <typename R, typename TL, int VAR>
 void launch_march_v(int mode, const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
                     const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                    double *mslot, hipStream_t st, const R *cfo, R *cfn) {
+                    unsigned long long *mslot, hipStream_t st, const R *cfo, R *cfn) {
   long long ncf = 0;  // solute-face work-groups (sweep A only)
   if (cfn && mode == kSweepA) {
     ncf = (shell_count(g.nx, g.ny, g.nz, 1, false) + TL::NT - 1) / TL::NT;
@@ -145,7 +147,8 @@ void launch_march_v(int mode, const Geo &g, const KParams &kp, const SC<R> &s, c
 template <typename R, typename TL>
 hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R> &s,
                           const R *po, const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp,
-                          int write_T, double *mslot, hipStream_t st, const R *cfo, R *cfn) {
+                          int write_T, unsigned long long *mslot, hipStream_t st, const R *cfo,
+                          R *cfn) {
   // the optional paths are compiled only into the kernels that need them
   const int var = (kp.AAA != 0.0 && kp.ruido_tipo == PF_RUIDO_SMOOTH ? kVarSmooth : 0) |
                   (kp.D != 0.0 ? kVarTDiff : 0);
@@ -167,7 +170,8 @@ hipError_t launch_march_t(int mode, const Geo &g, const KParams &kp, const SC<R>
 template <typename R>
 hipError_t launch_march(int mode, int tile, const Geo &g, const KParams &kp, const R *po,
                         const R *co, R *pn, R *cn, const R *Tcur, R *Tnext, R *dpp, int write_T,
-                        double *mslot, hipStream_t st, const R *cfo = nullptr, R *cfn = nullptr) {
+                        unsigned long long *mslot, hipStream_t st, const R *cfo = nullptr,
+                        R *cfn = nullptr) {
   const SC<R> s = make_sc<R>(kp);
 #define PF_MARCH(TL)                                                                           \
   return launch_march_t<R, TL>(mode, g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T, mslot, \
@@ -183,7 +187,7 @@ hipError_t launch_march(int mode, int tile, const Geo &g, const KParams &kp, con
 
 template <typename R, typename TL>
 hipError_t launch_solute_t(const Geo &g, const KParams &kp, const SC<R> &s, const R *po,
-                           const R *co, R *cn, double *mslot, const R *dpo, R *dpn,
+                           const R *co, R *cn, unsigned long long *mslot, const R *dpo, R *dpn,
                            hipStream_t st) {
   long long nd = 0;
   if (dpn) {
@@ -199,7 +203,8 @@ hipError_t launch_solute_t(const Geo &g, const KParams &kp, const SC<R> &s, cons
 
 template <typename R>
 hipError_t launch_solute_split(int tile, const Geo &g, const KParams &kp, const R *po, const R *co,
-                               R *cn, double *mslot, hipStream_t st, const R *dpo = nullptr,
+                               R *cn, unsigned long long *mslot, hipStream_t st,
+                               const R *dpo = nullptr,
                                R *dpn = nullptr) {
   const SC<R> s = make_sc<R>(kp);
   switch (tile) {
@@ -210,11 +215,10 @@ hipError_t launch_solute_split(int tile, const Geo &g, const KParams &kp, const 
   }
 }
 
-// The ledger is kept while it is valid and has a slot for every work-group
-// of the launch that updates the solute (fused_mass_prepare sizes the slots
-// before a run).
-bool mass_on(const FusedState &S, const Geo &g) {
-  return S.mass_every > 0 && S.ledger_valid && S.mslot && 8 * g.per_xcd <= S.nslot;
+// The ledger is kept while it is valid (fused_mass_prepare starts it before
+// a run); the solute launch adds its increments to the ledger's sum.
+unsigned long long *mass_acc(const FusedState &S) {
+  return S.mass_every > 0 && S.ledger_valid && S.mass ? &S.mass->acc : nullptr;
 }
 
 // Geometry of the launch that updates the solute: sweep B, or the solute
@@ -284,14 +288,14 @@ int step_impl(FusedState &S, const KParams &kt, const KParams &k, const FusedSte
         // with the phase decoration of this sweep in the same launch
         tm.begin(PF_KT_SOLUTE, st);
         e = launch_solute_split<R>(S.tile_s, gs, k, po, (const R *)c[*c_old], (R *)c[cnw],
-                                   mass_on(S, gs) ? S.mslot : nullptr, st, po, pn);
+                                   mass_acc(S), st, po, pn);
         tm.end(st);
         if (e != hipSuccess) return PF_E_HIP;
       } else {
         tm.begin(PF_KT_SWEEP_B, st);
         e = launch_march<R>(kSweepB, S.tile_b, gb, k, po, (const R *)c[*c_old], pn, (R *)c[cnw],
                             (const R *)T[t_cur], (R *)T[t_nxt], (R *)dpp, fs.last ? 0 : 1,
-                            mass_on(S, gb) ? S.mslot : nullptr, st);
+                            mass_acc(S), st);
         tm.end(st);
         if (e != hipSuccess) return PF_E_HIP;
       }
@@ -488,114 +492,40 @@ int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, 
   return autotune_impl<float>(s, k, phi_old, c_old, T_old, reps, stream, res);
 }
 
-// ---- solute-mass ledger (FP32-mixed form) -----------------------------------
-// The ledger is the FP64 interior solute mass the scheme's own increments
-// imply: its initial FP64 sum plus the FP64 sum of every FP32 increment
-// computed by sweep B (before the increment is rounded into the FP32 field).
-// Every mass_every steps the FP64 sum of the stored field over the same
-// interior [2,n-1]^3 is compared with it and the interior is rescaled by
-// ledger/sum (FP64 product, unbiased stochastic rounding to FP32), which
-// removes the drift that FP32 storage rounding accumulates.  The face planes
-// are then refreshed from their interior sources: left ~1 ulp above the
-// rescaled interior (the factor is mostly < 1), they would feed solute back
-// through the domain faces at every correction.
+// ---- solute-mass ledger (pf_mass.h) -------------------------------------------
 
 namespace {
-
-constexpr int kMassBlocks = 1024;
-
-__device__ __forceinline__ double block_sum256(double v, double *sh) {
-  const int t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w) sh[t] += sh[t + w];
-    __syncthreads();
-  }
-  return sh[0];
-}
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_mass_interior(const R *__restrict__ c, int nx, int ny,
                                                        int nz, double *__restrict__ part) {
   __shared__ double sh[256];
-  const int nrow = (ny - 2) * (nz - 2);
-  const size_t sz = (size_t)nx * ny;
-  double acc = 0.0;
-  for (int r = blockIdx.x; r < nrow; r += gridDim.x) {
-    const int j = 2 + r % (ny - 2), k = 2 + r / (ny - 2);
-    const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
-    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) acc += (double)c[row + i - 1];
-  }
-  const double t = block_sum256(acc, sh);
+  const double t = block_sum256(mass_rows(c, nx, ny, nz, (int)blockIdx.x), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// mode 0: ledger := interior sum (slots discarded); 1: ledger += slots;
-// 2: ledger += slots, mass[1] := interior sum, mass[2] := ledger / sum.
-// Slots are zeroed in every mode.
-__global__ void __launch_bounds__(256) k_mass_update(const double *__restrict__ part, int npart,
-                                                     double *__restrict__ mslot, int nslot,
-                                                     double *__restrict__ mass, int mode) {
+// mode 0: the ledger starts from the interior sum of the partials (acc = 0);
+// mode 1: S := interior sum, f := ledger / S.
+__global__ void __launch_bounds__(256) k_mass_update(const double *__restrict__ part,
+                                                     MassLedger *__restrict__ m, int mode) {
   __shared__ double sh[256];
-  double a = 0.0, b = 0.0;
-  for (int q = threadIdx.x; q < npart; q += 256) a += part[q];
-  for (int q = threadIdx.x; q < nslot; q += 256) {
-    b += mslot[q];
-    mslot[q] = 0.0;
-  }
-  const double S = block_sum256(a, sh);
-  __syncthreads();
-  const double L = block_sum256(b, sh);
+  const double S = mass_fold(part, sh);
   if (threadIdx.x == 0) {
     if (mode == 0) {
-      mass[0] = S;
+      m->L0 = S;
+      m->acc = 0ull;
     } else {
-      mass[0] += L;
-      if (mode == 2) {
-        mass[1] = S;
-        mass[2] = mass[0] / S;
-      }
+      m->S = S;
+      m->f = ledger_value(*m) / S;
     }
   }
 }
 
-// Rescales the interior cells by mass[2].  FP32: the FP64 product is
-// rounded stochastically to one of its two neighbouring floats with the
-// probability that makes the expectation exact (a counter hash of cell and
-// step, salt 4, draws the choice), because the factor is typically 1 -+ 1e-9,
-// far below one FP32 ulp, and round-to-nearest would leave every cell as it
-// was.  The summed rounding error is then a zero-mean random walk of about
-// sqrt(n) * ulp / 2.
 template <typename R>
 __global__ void __launch_bounds__(256) k_mass_scale(R *__restrict__ c, int nx, int ny, int nz,
                                                     int it, uint64_t seed,
-                                                    const double *__restrict__ mass) {
-  const double f = mass[2];
-  const int nrow = (ny - 2) * (nz - 2);
-  const size_t sz = (size_t)nx * ny;
-  for (int r = blockIdx.x; r < nrow; r += gridDim.x) {
-    const int j = 2 + r % (ny - 2), k = 2 + r / (ny - 2);
-    const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
-    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) {
-      const size_t t = row + i - 1;
-      const double x = (double)c[t] * f;
-      if constexpr (sizeof(R) == 8) {
-        c[t] = (R)x;
-      } else {
-        const float nearest = (float)x;
-        float lo = nearest, hi = nearest;
-        if ((double)nearest > x) lo = nextafterf(nearest, -__builtin_huge_valf());
-        else if ((double)nearest < x) hi = nextafterf(nearest, __builtin_huge_valf());
-        float out = lo;
-        if (hi != lo) {
-          const double p = (x - (double)lo) / ((double)hi - (double)lo);
-          out = hash_unit(hash_base(i, j, k, it, 4, seed)) < p ? hi : lo;
-        }
-        c[t] = out;
-      }
-    }
-  }
+                                                    const MassLedger *__restrict__ m) {
+  mass_scale_rows(c, nx, ny, nz, it, seed, m->f, (int)blockIdx.x);
 }
 
 template <typename R>
@@ -609,59 +539,43 @@ hipError_t mass_interior(const FusedState &S, const void *c, hipStream_t st) {
 
 int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
   if (S.mass_every <= 0 || !S.enabled) return PF_OK;
-  const Geo gs = solute_geo(S);
-  const int need = 8 * gs.per_xcd;
   if (!S.mass) {
-    if (hipMalloc((void **)&S.mass, 4 * sizeof(double)) != hipSuccess) return PF_E_NOMEM;
+    if (hipMalloc((void **)&S.mass, sizeof(MassLedger)) != hipSuccess) return PF_E_NOMEM;
     if (hipMalloc((void **)&S.mpart, kMassBlocks * sizeof(double)) != hipSuccess)
       return PF_E_NOMEM;
-    if (hipMemsetAsync(S.mass, 0, 4 * sizeof(double), st) != hipSuccess) return PF_E_HIP;
-  }
-  if (need > S.nslot) {  // slots are folded (zero) between runs: safe to replace
-    if (S.mslot) {
-      if (hipStreamSynchronize(st) != hipSuccess) return PF_E_HIP;
-      (void)hipFree(S.mslot);
-      S.mslot = nullptr;
-      S.nslot = 0;
-    }
-    if (hipMalloc((void **)&S.mslot, (size_t)need * sizeof(double)) != hipSuccess)
-      return PF_E_NOMEM;
-    if (hipMemsetAsync(S.mslot, 0, (size_t)need * sizeof(double), st) != hipSuccess)
-      return PF_E_HIP;
-    S.nslot = need;
+    if (hipMemsetAsync(S.mass, 0, sizeof(MassLedger), st) != hipSuccess) return PF_E_HIP;
   }
   if (!S.ledger_valid) {
-    hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c_old, st)
-                                     : mass_interior<float>(S, c_old, st);
-    if (e != hipSuccess) return PF_E_HIP;
-    hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, kMassBlocks, S.mslot,
-                       S.nslot, S.mass, 0);
-    if (hipGetLastError() != hipSuccess) return PF_E_HIP;
+    if (S.host_mass_ok) {
+      // the FP64 mass of the fields the context was created from
+      MassLedger m{S.host_mass, 0ull, 0.0, 0.0};
+      if (hipMemcpyAsync(S.mass, &m, sizeof m, hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        return PF_E_HIP;
+    } else {
+      hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c_old, st)
+                                       : mass_interior<float>(S, c_old, st);
+      if (e != hipSuccess) return PF_E_HIP;
+      hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, S.mass, 0);
+      if (hipGetLastError() != hipSuccess) return PF_E_HIP;
+    }
     S.ledger_valid = true;
   }
   return PF_OK;
 }
 
-int fused_mass_fold(FusedState &S, hipStream_t st) {
-  if (!S.ledger_valid || !S.mslot) return PF_OK;
-  hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, 0, S.mslot, S.nslot,
-                     S.mass, 1);
-  return hipGetLastError() == hipSuccess ? PF_OK : PF_E_HIP;
-}
-
 int fused_mass_correct(FusedState &S, const KParams &k, void *c, hipStream_t st) {
-  if (!S.ledger_valid || !S.mslot) return PF_OK;
+  if (!S.ledger_valid || !S.mass) return PF_OK;
   const int it = k.it;
   const uint64_t seed = k.seed;
   hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c, st)
                                    : mass_interior<float>(S, c, st);
   if (e != hipSuccess) return PF_E_HIP;
-  hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, kMassBlocks, S.mslot,
-                     S.nslot, S.mass, 2);
+  hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, S.mass, 1);
   if (hipGetLastError() != hipSuccess) return PF_E_HIP;
   if (S.prec == PF_FP64) {
-    hipLaunchKernelGGL(k_mass_scale<double>, dim3(kMassBlocks), dim3(256), 0, st, (double *)c, S.nx,
-                       S.ny, S.nz, it, seed, S.mass);
+    hipLaunchKernelGGL(k_mass_scale<double>, dim3(kMassBlocks), dim3(256), 0, st, (double *)c,
+                       S.nx, S.ny, S.nz, it, seed, S.mass);
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_solute_refresh<double>(k, (double *)c, st);
   } else {
@@ -673,12 +587,31 @@ int fused_mass_correct(FusedState &S, const KParams &k, void *c, hipStream_t st)
   return e == hipSuccess ? PF_OK : PF_E_HIP;
 }
 
-int fused_mass_read(FusedState &S, double out[3], hipStream_t st) {
-  out[0] = out[1] = out[2] = 0.0;
+int fused_mass_state(FusedState &S, MassLedger *out, hipStream_t st) {
+  *out = MassLedger{0.0, 0ull, 0.0, 0.0};
   if (!S.mass) return PF_OK;
-  if (hipMemcpyAsync(out, S.mass, 3 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+  if (hipMemcpyAsync(out, S.mass, sizeof *out, hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return PF_E_HIP;
+  return PF_OK;
+}
+
+int fused_mass_restore(FusedState &S, const MassLedger &m, hipStream_t st) {
+  if (!S.mass) return PF_E_STATE;
+  if (hipMemcpyAsync(S.mass, &m, sizeof m, hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return PF_E_HIP;
+  return PF_OK;
+}
+
+int fused_mass_read(FusedState &S, double out[3], hipStream_t st) {
+  MassLedger m;
+  out[0] = out[1] = out[2] = 0.0;
+  if (fused_mass_state(S, &m, st) != PF_OK) return PF_E_HIP;
+  if (!S.mass) return PF_OK;
+  out[0] = ledger_value(m);
+  out[1] = m.S;
+  out[2] = m.f;
   return PF_OK;
 }
 
@@ -686,13 +619,12 @@ void fused_free(FusedState &s) {
   s.enabled = false;
   if (s.mass) (void)hipFree(s.mass);
   if (s.mpart) (void)hipFree(s.mpart);
-  if (s.mslot) (void)hipFree(s.mslot);
-  s.mass = s.mpart = s.mslot = nullptr;
-  s.nslot = 0;
+  s.mass = nullptr;
+  s.mpart = nullptr;
   s.ledger_valid = false;
 }
 size_t fused_bytes(const FusedState &s) {
-  return s.mass ? (4 + kMassBlocks + (size_t)s.nslot) * sizeof(double) : 0;
+  return s.mass ? sizeof(MassLedger) + kMassBlocks * sizeof(double) : 0;
 }
 
 bool fused_supported(const FusedState &s, const pf_params &p) {

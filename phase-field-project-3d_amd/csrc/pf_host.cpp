@@ -9,6 +9,7 @@
 
 #include "../../include/pf_capi.h"
 #include "pf_common.h"
+#include "pf_mass.h"
 
 #pragma clang fp contract(off)
 
@@ -122,3 +123,14 @@ int pf_initial_fields(int32_t nx, int32_t ny, int32_t nz, int32_t tipo, double R
 }
 
 }  // extern "C"
+
+// FP64 interior ([2,n-1]^3) sum of a host field, k, j, i in order.
+double pf::host_interior_mass(const double *c, int nx, int ny, int nz) {
+  double s = 0.0;
+  for (int k = 2; k <= nz - 1; ++k)
+    for (int j = 2; j <= ny - 1; ++j) {
+      const double *row = c + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
+      for (int i = 2; i <= nx - 1; ++i) s += row[i - 1];
+    }
+  return s;
+}

@@ -30,6 +30,7 @@
 #include <type_traits>
 
 #include "pf_common.h"
+#include "pf_mass.h"
 #include "pf_math.h"
 
 namespace pf {
@@ -316,9 +317,8 @@ enum : int {
 };
 
 // One work-group's tile (bx, by) and z-segment sg of a sweep of kind MODE.
-// With the solute (kSweepB) and mslot non-null, the work-group adds the FP64
-// sum of its cells' solute increments to *mslot (its own slot of the
-// solute-mass ledger, so no atomics and a fixed summation order).
+// With the solute (kSweepB) and macc non-null, the work-group adds its
+// cells' solute increments, in ledger units (pf_mass.h), to *macc.
 // Kernel variants (bits of VAR): paths that few decks take but whose
 // register demand would burden every other path are compiled only into the
 // kernels that need them; launches pick the variant from the deck (a kernel
@@ -346,7 +346,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
                                            R *__restrict__ pn, R *__restrict__ cn,
                                            const R *__restrict__ Tcur, R *__restrict__ Tnext,
                                            R *__restrict__ dpp, int write_T,
-                                           double *__restrict__ mslot, int bx, int by, int sg) {
+                                           unsigned long long *__restrict__ macc_dst, int bx,
+                                           int by, int sg) {
   // FMAs only within one source expression, decided by the front end, so the
   // single-instance and batched kernels that inline this body round alike
 #pragma clang fp contract(on)
@@ -436,10 +437,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   R cZ = 0;
   R cPPcur = 0;  // c of the plane updated next (sweeps A and phase + T)
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
-  // solute increments of this column for the ledger, summed in the
-  // arithmetic type over the column's z-segment (a few dozen cells) and in
-  // FP64 from there on (FP64 here would cost the FP32 form a wave per SIMD)
-  R macc = (R)0;
+  // solute increments of this column for the ledger, in ledger units
+  const bool acc_on = SWEEP_B && macc_dst != nullptr;  // uniform
+  long long macc = 0;
   R fzu_prev = (R)0;  // solute flux through the up face of the previous cell
 
   const int q_first = ks - 2, q_last = ke + 2;
@@ -590,7 +590,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         constexpr uint32_t E = (uint32_t)sizeof(R);
         bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
                cPP + inc);
-        macc += own ? inc : (R)0;
+        if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
       }
       if constexpr (T_OUT) {
         if (write_T) {
@@ -651,20 +651,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   for (int q = q_first; q < ks + 2; ++q) plane_step(q, std::false_type{});
   for (int q = ks + 2; q <= q_last; ++q) plane_step(q, std::true_type{});
   if constexpr (SWEEP_B) {
-    if (mslot) {  // uniform over the work-group
-      __shared__ double wsum[NT / 64];
-      double v = (double)macc;
-      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-      if ((tid & 63) == 0) wsum[tid >> 6] = v;
-      __syncthreads();
-      if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < NT / 64; ++w) t += wsum[w];
-        *mslot += t;
-      }
-    }
+    if (acc_on) mass_flush<NT>(macc, tid, macc_dst);
   } else {
-    (void)mslot; (void)macc; (void)cn;
+    (void)macc_dst; (void)macc; (void)cn;
   }
 }
 
@@ -686,8 +675,9 @@ constexpr size_t march_lds_bytes(int mode) {
 template <typename R, typename TL>
 __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, const SC<R> &s,
                                             const R *__restrict__ po, const R *__restrict__ co,
-                                            R *__restrict__ cn, double *__restrict__ mslot,
-                                            int bx, int by, int sg) {
+                                            R *__restrict__ cn,
+                                            unsigned long long *__restrict__ macc_dst, int bx,
+                                            int by, int sg) {
 #pragma clang fp contract(on)
   constexpr int TX = TL::TX, NT = TL::NT, QX = TL::QX, QY = TL::QY, QPL = TL::QPL;
   constexpr int NRING = TL::NRING;
@@ -721,7 +711,9 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
 
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R pZ = 0, cZ = 0;  // phi and c of the own cell of plane k
-  R macc = (R)0, fzu_prev = (R)0;
+  const bool acc_on = macc_dst != nullptr;  // uniform
+  long long macc = 0;
+  R fzu_prev = (R)0;
   // node plane q (all in [1, nz]); prefetched one plane ahead
   const int q_first = ks - 1, q_last = ke + 1;
   R pq = 0, cq = 0, rp = 0, rcv = 0;
@@ -769,25 +761,14 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       constexpr uint32_t E = (uint32_t)sizeof(R);
       bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
              cPP + inc);
-      macc += own ? inc : (R)0;
+      if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
     pZ = pcur; cZ = ccur;
   };
   for (int q = q_first; q < ks + 1; ++q) step(q, std::false_type{});
   for (int q = ks + 1; q <= q_last; ++q) step(q, std::true_type{});
-  if (mslot) {  // uniform over the work-group
-    __shared__ double wsum[NT / 64];
-    double v = (double)macc;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((tid & 63) == 0) wsum[tid >> 6] = v;
-    __syncthreads();
-    if (tid == 0) {
-      double t = 0.0;
-      for (int w = 0; w < NT / 64; ++w) t += wsum[w];
-      *mslot += t;
-    }
-  }
+  if (acc_on) mass_flush<NT>(macc, tid, macc_dst);
 }
 
 template <typename TL, typename R>

@@ -79,7 +79,7 @@ EXPORTS = (
     "pf_batch_set_graph_steps", "pf_batch_synchronize", "pf_batch_status", "pf_batch_scalars",
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
     "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_set_group_target",
-    "pf_batch_autotune",
+    "pf_batch_autotune", "pf_batch_set_mass_correction", "pf_batch_mass_ledger",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16), (16, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass", "solute")
@@ -144,6 +144,8 @@ def load(path: str = LIB_PATH):
         "pf_batch_get_tiling": ([vp, i32, C.POINTER(i32)], C.c_int),
         "pf_batch_set_group_target": ([vp, i32], C.c_int),
         "pf_batch_autotune": ([vp, i32, i32], C.c_int),
+        "pf_batch_set_mass_correction": ([vp, i32, i32], C.c_int),
+        "pf_batch_mass_ledger": ([vp, i32, vp], C.c_int),
         "pf_mass_ledger": ([vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
@@ -434,6 +436,14 @@ class Batch:
 
     def autotune(self, reps: int = 2, size_class: int = -1):
         check(self.lib.pf_batch_autotune(self.h, int(size_class), int(reps)), self.lib)
+
+    def set_mass_correction(self, slot: int, every: int):
+        check(self.lib.pf_batch_set_mass_correction(self.h, int(slot), int(every)), self.lib)
+
+    def mass_ledger(self, slot: int):
+        out = np.zeros(3)
+        check(self.lib.pf_batch_mass_ledger(self.h, int(slot), _ptr(out)), self.lib)
+        return tuple(float(v) for v in out)
 
     def close(self):
         if getattr(self, "h", None):

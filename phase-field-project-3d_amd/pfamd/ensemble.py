@@ -120,8 +120,6 @@ def run_sequential(problems: List[Problem], keep: bool = False, lib=None) -> Run
     for pr, (s, f) in zip(problems, prepared):
         t0 = time.perf_counter()
         sim = Simulation(pr.deck, precision=pr.precision, seed=pr.noise_seed, fields=f, lib=lib)
-        if pr.precision == capi.PF_FP32_MIXED:
-            sim.ctx.set_mass_correction(0)  # the batched form keeps no mass ledger
         sim.run(pr.steps)
         sim.ctx.synchronize()
         wall += time.perf_counter() - t0

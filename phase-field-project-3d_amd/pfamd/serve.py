@@ -191,8 +191,6 @@ def serve_sequential(requests: List[Request], keep: bool = False, lib=None,
         _, f = r.problem.setup(lib)
         pr = r.problem
         sim = Simulation(pr.deck, precision=pr.precision, seed=pr.noise_seed, fields=f, lib=lib)
-        if pr.precision == capi.PF_FP32_MIXED:
-            sim.ctx.set_mass_correction(0)  # as in the batched form
         sim.run(pr.steps)
         sim.ctx.synchronize()
         c = Completion(r.rid, r.arrival, adm, now())

@@ -40,8 +40,6 @@ def _add(batch, name, nsteps):
 def _single(name, nsteps):
     deck, prec, seed = PROBLEMS[name]
     sim = Simulation(deck, precision=prec, seed=seed)
-    if prec == capi.PF_FP32_MIXED:
-        sim.ctx.set_mass_correction(0)  # the batched FP32 form keeps no mass ledger
     sim.run(nsteps)
     f = sim.fields()
     out = (f["phi_new"], f["c_new"], f["T_new"], sim.p.it, sim.p.TP, sim.p.M0, float(sim.L.time),
@@ -125,3 +123,51 @@ def test_batch_tiling_and_budget():
             _add(b, "fp64_seed_noise", 1)  # over the budget
         used, budget = b.bytes()
         assert used == 2 * 48 * n and budget == 2 * 48 * n + 100
+
+
+def test_batch_fp32_mass_correction_equals_single():
+    """FP32 instances keep the solute-mass ledger in the batch as
+    pf_run_steps does (a correction every 100 steps by default): 230 steps
+    across two corrections, in a mix with an FP64 instance and with a graph
+    length that does not divide the period, equal the single-instance run
+    split over calls (100 + 30 + 100), bit for bit, ledger included.  With
+    the correction off the two forms agree as well, and differ from the
+    corrected run."""
+    print("[batch] FP32 solute-mass correction", flush=True)
+    names = ("fp32_planar_cooling", "fp32_seed_small")
+    with capi.Batch() as b:
+        b.set_graph_steps(7)
+        slots = {nm: _add(b, nm, 230) for nm in names}
+        mix = _add(b, "fp64_seed_noise", 150)
+        off = _add(b, "fp32_seed_small", 230)
+        b.set_mass_correction(off, 0)
+        b.run(230)
+        got = {nm: (_from_batch(b, sl, nm), b.mass_ledger(sl)) for nm, sl in slots.items()}
+        got_off = _from_batch(b, off, "fp32_seed_small")
+        assert b.mass_ledger(off)[1:] == (0.0, 0.0)  # never corrected
+        assert b.status(mix) == (150, 0)
+        with pytest.raises(capi.PfError):
+            b.set_mass_correction(slots[names[0]], 50)  # only before the first step
+    for nm in names:
+        deck, prec, seed = PROBLEMS[nm]
+        sim = Simulation(deck, precision=prec, seed=seed)
+        for n in (100, 30, 100):
+            sim.run(n)
+        f = sim.fields()
+        ref = (f["phi_new"], f["c_new"], f["T_new"], sim.p.it, sim.p.TP, sim.p.M0,
+               float(sim.L.time), sim.diagnostics())
+        led = sim.ctx.mass_ledger()
+        sim.close()
+        _same(got[nm][0], ref, nm)
+        assert got[nm][1] == led, (nm, got[nm][1], led)
+        assert led[2] != 0.0  # corrected at least once
+    deck, prec, seed = PROBLEMS["fp32_seed_small"]
+    sim = Simulation(deck, precision=prec, seed=seed)
+    sim.ctx.set_mass_correction(0)
+    sim.run(230)
+    f = sim.fields()
+    ref_off = (f["phi_new"], f["c_new"], f["T_new"], sim.p.it, sim.p.TP, sim.p.M0,
+               float(sim.L.time), sim.diagnostics())
+    sim.close()
+    _same(got_off, ref_off, "correction off")
+    assert not np.array_equal(got_off[1], got["fp32_seed_small"][0][1])
