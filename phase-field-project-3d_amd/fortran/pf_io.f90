@@ -119,51 +119,50 @@ contains
     if (tipo_inicial == 4) call ler_restart()
   end subroutine entrada
 
-  ! tipo_inicial = 4: continue from restart.dat (a Tecplot POINT file of this
-  ! grid).  R_semente is the number k_desl of bottom layers dropped; the top
-  ! k_desl layers become liquid with the last layer's concentration.
+  ! tipo_inicial = 4: continue from restart.dat, a Tecplot POINT snapshot of
+  ! this grid (4 header records, then one record "x y z phi T c" per cell, x
+  ! fastest, then y, then z).  The whole file is read into buffers first;
+  ! the moving frame then drops the nshift = nint(R_semente) bottom layers
+  ! (file layer k becomes layer k - nshift) and makes the nshift top layers
+  ! liquid, each column with the concentration of the file's top layer.
+  ! T comes from the deck; c_new equals c_old afterwards.
   subroutine ler_restart()
-    integer :: u, ios, i, j, k, kn, k_desl
-    double precision :: xr, yr, zr, phir, tr, cr
-    k_desl = max(0, nint(R_semente))
-    if (k_desl >= Nz) then
-      write(*, '(A)') 'pf3d: tipo_inicial = 4: shift >= Nz'
-      error stop 1
-    end if
+    double precision, allocatable :: phi_f(:,:,:), c_f(:,:,:)
+    double precision :: rec(6)
+    integer :: u, ios, nshift, q, ncell, top
+    nshift = max(0, nint(R_semente))
+    if (nshift >= Nz) call restart_fail('the shift is not below Nz')
     open(newunit=u, file='restart.dat', status='old', action='read', iostat=ios)
-    if (ios /= 0) then
-      write(*, '(A)') 'pf3d: tipo_inicial = 4: restart.dat not found'
-      error stop 1
-    end if
-    do i = 1, 4
-      read(u, *)
-    end do
-    do k = 1, Nz
-      do j = 1, Ny
-        do i = 1, Nx
-          read(u, *, iostat=ios) xr, yr, zr, phir, tr, cr
-          if (ios /= 0) then
-            write(*, '(A)') 'pf3d: tipo_inicial = 4: restart.dat shorter than the grid'
-            error stop 1
-          end if
-          kn = k - k_desl
-          if (kn >= 1) then
-            phase_field_old(i, j, kn) = phir
-            conc_soluto_old(i, j, kn) = cr
-            conc_soluto_new(i, j, kn) = cr
-          end if
-          if (k == Nz .and. k_desl > 0) then
-            phase_field_old(i, j, Nz - k_desl + 1:Nz) = 0.d0
-            conc_soluto_old(i, j, Nz - k_desl + 1:Nz) = cr
-            conc_soluto_new(i, j, Nz - k_desl + 1:Nz) = cr
-          end if
-        end do
-      end do
+    if (ios /= 0) call restart_fail('restart.dat not found')
+    read(u, '(3/)', iostat=ios)  ! the title, variables and zone records
+    if (ios /= 0) call restart_fail('restart.dat has no header')
+    allocate(phi_f(Nx, Ny, Nz), c_f(Nx, Ny, Nz))
+    ncell = Nx * Ny * Nz
+    do q = 0, ncell - 1
+      read(u, *, iostat=ios) rec
+      if (ios /= 0) call restart_fail('restart.dat shorter than the grid')
+      phi_f(1 + mod(q, Nx), 1 + mod(q / Nx, Ny), 1 + q / (Nx * Ny)) = rec(4)
+      c_f(1 + mod(q, Nx), 1 + mod(q / Nx, Ny), 1 + q / (Nx * Ny)) = rec(6)
     end do
     close(u)
+    top = Nz - nshift  ! layers 1..top come from the file
+    phase_field_old(:, :, 1:top) = phi_f(:, :, nshift + 1:Nz)
+    conc_soluto_old(:, :, 1:top) = c_f(:, :, nshift + 1:Nz)
+    do q = top + 1, Nz
+      phase_field_old(:, :, q) = 0.d0
+      conc_soluto_old(:, :, q) = c_f(:, :, Nz)
+    end do
+    conc_soluto_new = conc_soluto_old
+    deallocate(phi_f, c_f)
     write(*, '(A,I0,A,I0,A,I0,A,I0,A)') 'restart.dat read (', Nx, ' x ', Ny, ' x ', Nz, &
-          '), shift = ', k_desl, ' layers'
+          '), shift = ', nshift, ' layers'
   end subroutine ler_restart
+
+  subroutine restart_fail(msg)
+    character(len=*), intent(in) :: msg
+    write(*, '(A,A)') 'pf3d: tipo_inicial = 4: ', msg
+    error stop 1
+  end subroutine restart_fail
 
   ! Tecplot POINT snapshot of phi_new, TP and c_new in ./output/, named
   ! saida_<IT (I6)>_time_<time (f14.7)>s.dat like the reference's files.
