@@ -57,15 +57,28 @@ def workload_deck():
                 WriteFileD_timeCounter=DIAG_EVERY)
 
 
-def _sync(sim):
-    sim.ctx.synchronize()
+_TORCH = None
+
+
+def _torch_init():
+    """Imports torch (for its device synchronize) before any GPU work: an
+    import between the tiling and the timed steps would leave the GPU idle
+    long enough for its clocks to drop, and a window of a few dozen steps is
+    too short for them to come back (measured: 20 steps after 0.2 s idle
+    1.10 ms/step, without the idle 1.00 ms/step)."""
+    global _TORCH
     try:
         import torch
 
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
+        _TORCH = torch if torch.cuda.is_available() else None
     except Exception:
-        pass
+        _TORCH = None
+
+
+def _sync(sim):
+    sim.ctx.synchronize()
+    if _TORCH is not None:
+        _TORCH.cuda.synchronize()
 
 
 def parse_tiling(spec):
@@ -99,10 +112,11 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     sim = Simulation(deck, precision=precision, seed=NOISE_SEED, n_seeds=N_SEEDS,
                      seed_rng=SEED_RNG)
     t_init = time.time() - t0
-    if warmup:
-        sim.run(warmup)
-    # pick the sweep tiling on this grid (times every candidate once, before
-    # the timed region; results do not depend on it)
+    # pick the sweep tiling on this grid (times every candidate on the
+    # initial fields, writing into scratch buffers: the state is untouched;
+    # results do not depend on the tiling), then the warm-up steps and the
+    # timed steps back to back, with no host work in between that would let
+    # the GPU idle
     cands = []
     if tiling is not None:
         for q in range(3):
@@ -113,6 +127,8 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     split = sim.ctx.sweep_split()
     tiling = sim.ctx.tiling()
     spec = tiling_spec(sim)
+    if warmup:
+        sim.run(warmup)
     _sync(sim)
     sim.ctx.timing(timing)
     done = 0
@@ -196,10 +212,33 @@ def precision_decks():
             "held at TP=905 K (growth)": Deck(taxa_resfr=0.0, TP=905.0, **base)}
 
 
+def perturbed_fields(deck, rel=2.0 ** -25, seed=5):
+    """The deck's initial fields with every interior c multiplied by
+    1 + u, u uniform in [-rel, rel] (half an FP32 ulp: the size of the
+    rounding FP32 storage applies once), then rescaled so that the interior
+    FP64 mass equals the unperturbed one."""
+    import numpy as np
+
+    from pfamd.run import initial_fields
+
+    d = deck.resolved()
+    f = initial_fields(deck)
+    c = f[2].reshape(d.Nz, d.Ny, d.Nx)
+    inner = (slice(1, -1),) * 3
+    m0 = float(np.sum(c[inner], dtype=np.float64))
+    u = np.random.default_rng(seed).uniform(-rel, rel, size=c[inner].shape)
+    c[inner] = c[inner] * (1.0 + u)
+    c[inner] = c[inner] * (m0 / float(np.sum(c[inner], dtype=np.float64)))
+    return f
+
+
 def precision_study(checkpoints=(5000, 10000, 20000)):
     """FP32-mixed against FP64: per-field max-abs, RMS and relative L2
     differences and the interior solute mass at each checkpoint (both runs
-    on the GPU, in lockstep)."""
+    on the GPU, in lockstep).  Beside them, a second FP64 run started from
+    the same fields with c perturbed by half an FP32 ulp per cell (same
+    interior mass): how far the FP64 scheme's own mass trajectory moves
+    under a perturbation of the size FP32 storage applies once."""
     import numpy as np
 
     from pfamd import capi
@@ -209,7 +248,8 @@ def precision_study(checkpoints=(5000, 10000, 20000)):
     for name, deck in precision_decks().items():
         d = deck.resolved()
         shape = (d.Nz, d.Ny, d.Nx)
-        sims = {"fp64": Simulation(deck), "fp32": Simulation(deck, precision=capi.PF_FP32_MIXED)}
+        sims = {"fp64": Simulation(deck), "fp32": Simulation(deck, precision=capi.PF_FP32_MIXED),
+                "fp64_perturbed": Simulation(deck, fields=perturbed_fields(deck))}
 
         def interior_mass(c):
             return float(np.sum(c.reshape(shape)[1:-1, 1:-1, 1:-1], dtype=np.float64))
@@ -229,6 +269,8 @@ def precision_study(checkpoints=(5000, 10000, 20000)):
                 row[f"{nm}_rms"] = float(np.sqrt(np.mean((a - b) ** 2)))
                 row[f"{nm}_rel_l2"] = float(np.linalg.norm(a - b) / nb) if nb > 1e-6 else None
             M64, M32 = interior_mass(f64["c_new"]), interior_mass(f32["c_new"])
+            Mp = interior_mass(sims["fp64_perturbed"].ctx.download()[1])
+            row["mass_fp64_perturbed_vs_fp64"] = (Mp - M64) / M64
             led = sims["fp32"].ctx.mass_ledger()
             row["phi_max_fp64"] = float(np.max(f64["phi_new"]))
             row["mass_fp64_change"] = (M64 - m0) / m0
@@ -326,8 +368,7 @@ def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
     log(f"[bench] karma {n}^3: {out['mlups']} MLUPS, {out['ms_per_step']} ms/step")
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_karma3d")
     if os.path.exists(exe):
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        threads = max(1, min(16, threads))
+        threads = host_cpu()["physical_cores"]
         with tempfile.TemporaryDirectory() as d:
             with open(os.path.join(d, "karma.nml"), "w") as f:
                 f.write(f"&karma\n N = {cpu_n}\n nsteps = {cpu_steps}\n nout = {cpu_steps}\n"
@@ -348,16 +389,53 @@ def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
     return out
 
 
+def host_cpu():
+    """The host's CPU as the baseline runs on it: model, the logical CPUs this
+    process may use, the physical cores among them (SMT siblings counted
+    once) and the NUMA nodes (from /proc/cpuinfo and sysfs)."""
+    cpus = sorted(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else \
+        list(range(os.cpu_count() or 1))
+    model, family, mnum = "unknown", None, None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            k, _, v = ln.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "model name" and model == "unknown":
+                model = v
+            elif k == "cpu family" and family is None:
+                family = v
+            elif k == "model" and mnum is None:
+                mnum = v
+    except OSError:
+        pass
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology"
+        try:
+            pkg = open(f"{base}/physical_package_id").read().strip()
+            core = open(f"{base}/core_id").read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    try:
+        nodes = sorted(d for d in os.listdir("/sys/devices/system/node") if d.startswith("node"))
+    except OSError:
+        nodes = []
+    return {"model": model, "cpu_family": family, "cpu_model": mnum, "logical_cpus": len(cpus),
+            "physical_cores": len(cores), "numa_nodes": max(1, len(nodes))}
+
+
 def cpu_baseline():
     """The reference's own OpenMP CPU path (oracle/_ref/ref_harness: its
-    kernels compiled from source with flang -O2 -fopenmp), timed on this
-    host's cores on 20 steps of the same grid after 1 warm-up step (about
-    10 s of CPU work on 16 cores).  Falls
-    back to the C oracle (a faithful restatement) when the reference build
-    is absent."""
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(16, threads))
-    nwarm, nsteps = 1, 20
+    kernels compiled from source with flang -O2 -fopenmp), timed on every
+    physical core this process may use (OMP_PLACES=cores,
+    OMP_PROC_BIND=close, as the reference's set_OpenMP_environment.bat) on
+    20 steps of the same grid after 2 warm-up steps (about 10 s of CPU work
+    on 16 cores).  Falls back to the C oracle (a faithful restatement) when
+    the reference build is absent."""
+    cpu = host_cpu()
+    threads = cpu["physical_cores"]
+    nwarm, nsteps = 2, 20
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
@@ -394,7 +472,9 @@ def cpu_baseline():
     mlups = NX * NY * NZ * nsteps / t / 1e6
     return {"value": mlups, "unit": "MLUPS", "cores": threads, "kind": kind,
             "sample": f"{nsteps} steps of the {NX}x{NY}x{NZ} FP64 step after {nwarm} warm-up, "
-                      f"OMP_NUM_THREADS={threads}, {t:.2f} s"}
+                      f"OMP_NUM_THREADS={threads} (every physical core), OMP_PLACES=cores, "
+                      f"OMP_PROC_BIND=close, {t:.2f} s",
+            "host": cpu}
 
 
 def log(msg):
@@ -445,6 +525,7 @@ def main(argv=None):
     if capi.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         return 1
+    _torch_init()
     log(f"[bench] FP64 {NX}x{NY}x{NZ}: {a.warmup} warm-up + {a.steps} timed steps")
     r64 = run_form(capi.PF_FP64, a.steps, a.warmup, autotune=not a.no_autotune, tiling=a.tiling)
     ncell = NX * NY * NZ
@@ -472,6 +553,12 @@ def main(argv=None):
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,
+        "whole_step_roofline": {"bytes_per_cell_step": 72,
+                                "note": "the survey's two-sweep minimum with T read (72 B FP64); "
+                                        "with D = 0 this schedule reads no T (64 B)",
+                                "achieved": round(mlups * 1e6 * 72 / 1e9, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(mlups * 1e6 * 72 / 1e9 / HBM_PEAK_GBS, 4)},
         "kernels_ms_total": {k: round(v[0], 3) for k, v in r64["kernels"].items()},
         "kernels_launches": {k: v[1] for k, v in r64["kernels"].items()},
         "whole_step_alg_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
