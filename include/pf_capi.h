@@ -166,6 +166,20 @@ int pf_end_step(pf_ctx *ctx);
  * conc_soluto_new: out = {L2(phi), max phi, min phi, sum c, free energy}.
  * Deterministic fixed-order FP64 reduction.  Synchronous. */
 int pf_diagnosticos(pf_ctx *ctx, const pf_params *p, double out[5]);
+/* No counterpart in the reference: pf_diagnosticos for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops).  The fields are the caller's device arrays of the stored type
+ * (double for PF_FP64, float for PF_FP32_MIXED; p->precision says which),
+ * out is 5 doubles of device memory ({L2, max, min, sum c, free energy} as
+ * pf_diagnosticos), the work goes to `stream` (a hipStream_t; NULL = the
+ * default stream) and the call returns without waiting.  No allocation:
+ * the partial sums live in a fixed device array of the library (2048 x 5
+ * doubles per device, held until the process ends), shared by every call
+ * on that device -- calls that overlap on one device race.  Grids need
+ * extents >= 3 and ny*nz < 2^31; same kernels, so the same bits as
+ * pf_diagnosticos. */
+int pf_diagnosticos_noalloc(const pf_params *p, const void *phi_new, const void *c_new,
+                            double *out, void *stream);
 /* source/principal.f90:182,248 (target update from).  Any pointer may be
  * NULL.  Synchronous. */
 int pf_download(pf_ctx *ctx, double *phi_new, double *c_new, double *T_new);

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,8 @@
 #include "pf_common.h"
 #include "pf_fused.h"
 #include "pf_kernels.h"
+
+static __device__ __align__(256) double scratch[2048 * 5];
 
 namespace {
 
@@ -217,9 +220,70 @@ void launch_stream_copy(const double2 *a, double2 *b, size_t n2, hipStream_t s) 
   hipLaunchKernelGGL(k_stream_copy<U>, dim3((unsigned)nb), dim3(256), 0, s, a, b, n2);
 }
 
+// ---- diagnostics without allocation ------------------------------------------
+// The partial sums of pf_diagnosticos_noalloc live in the fixed device array
+// `scratch` (file scope, below the namespace): diag_blocks() blocks x 5
+// doubles, at most kDiagScratchBlocks blocks, one copy per device (each
+// device holds it until the process ends).  Host code never passes the
+// array's name to a kernel: its device address on the current device comes
+// from hipGetSymbolAddress, looked up once per device and kept in
+// g_scratch_dev.
+constexpr int kDiagScratchBlocks = 2048;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on device dev (looked up on first use, with
+// dev current).
+int diag_scratch(int dev, double **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail(PF_E_ARG, "device %d out of range", dev);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(PF_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)ptr;
+  }
+  *out = g_scratch_dev[dev];
+  return PF_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int pf_diagnosticos_noalloc(const pf_params *p, const void *phi_new, const void *c_new,
+                            double *out, void *stream) {
+  if (!p || !phi_new || !c_new || !out) return fail(PF_E_ARG, "null argument");
+  if (p->nx < 3 || p->ny < 3 || p->nz < 3)
+    return fail(PF_E_ARG, "grid %dx%dx%d: each extent must be >= 3", p->nx, p->ny, p->nz);
+  if ((long long)p->ny * p->nz > 0x7fffffffLL)
+    return fail(PF_E_ARG, "grid %dx%dx%d: ny*nz must be below 2^31", p->nx, p->ny, p->nz);
+  if (p->precision != PF_FP64 && p->precision != PF_FP32_MIXED)
+    return fail(PF_E_ARG, "unknown precision %d", p->precision);
+  const int nblk = pf::diag_blocks(p->ny, p->nz);
+  static_assert(sizeof(scratch) == sizeof(double) * 5 * kDiagScratchBlocks, "scratch size");
+  if (nblk > kDiagScratchBlocks)
+    return fail(PF_E_ARG, "%d partial blocks exceed the fixed scratch of %d", nblk,
+                kDiagScratchBlocks);
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  double *part = nullptr;
+  int rc = diag_scratch(dev, &part);
+  if (rc != PF_OK) return rc;
+  const pf::KParams k = kparams(*p);
+  const hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  if (p->precision == PF_FP64)
+    e = pf::launch_diag<double>(k, (const double *)phi_new, (const double *)c_new, part, nblk, out,
+                                st);
+  else
+    e = pf::launch_diag<float>(k, (const float *)phi_new, (const float *)c_new, part, nblk, out,
+                               st);
+  if (e != hipSuccess) return fail(PF_E_HIP, "diag launch: %s", hipGetErrorString(e));
+  return PF_OK;
+}
 
 int pf_stream_copy(size_t bytes, int32_t iters, double *gbs) {
   if (!gbs || bytes < 1024 || iters < 1) return fail(PF_E_ARG, "bad stream-copy arguments");

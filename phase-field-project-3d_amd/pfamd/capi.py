@@ -80,6 +80,7 @@ EXPORTS = (
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
     "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_set_group_target",
     "pf_batch_autotune", "pf_batch_set_mass_correction", "pf_batch_mass_ledger",
+    "pf_diagnosticos_noalloc",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16), (16, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass", "solute")
@@ -146,6 +147,7 @@ def load(path: str = LIB_PATH):
         "pf_batch_autotune": ([vp, i32, i32], C.c_int),
         "pf_batch_set_mass_correction": ([vp, i32, i32], C.c_int),
         "pf_batch_mass_ledger": ([vp, i32, vp], C.c_int),
+        "pf_diagnosticos_noalloc": ([P, vp, vp, vp, vp], C.c_int),
         "pf_mass_ledger": ([vp, vp], C.c_int),
         "pf_run_steps": ([vp, P, L, i32, vp], C.c_int),
         "pf_synchronize": ([vp], C.c_int),
