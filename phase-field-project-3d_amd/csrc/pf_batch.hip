@@ -76,13 +76,19 @@ constexpr int kChunkCells = 1024;
 
 // ---- per-step scalar kernels ------------------------------------------------
 
+// due[q] (when non-null): instance q corrects its solute mass after this
+// step -- a compact array the correction kernels read instead of every
+// instance's record
 template <typename R>
-__global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n) {
+__global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n, int *__restrict__ due) {
 #pragma clang fp contract(off)
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   InstDev<R> &I = inst[q];
-  if (!I.active) return;
+  if (!I.active) {
+    if (due) due[q] = 0;
+    return;
+  }
   I.k.it += 1;
   const int it = I.k.it;
   I.time = (float)(I.k.dt + (double)I.time);  // principal.f90:96
@@ -90,6 +96,7 @@ __global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n) {
   if (I.taxa > 0.0) I.k.TP = I.T0 - I.taxa * I.k.dt * (double)it;  // :104
   equilibrium(I.k.TP, I.Tm, I.Ke, I.Me, I.k.Cs_eq, I.k.Cl_eq);
   I.s = make_sc<R>(I.k);
+  if (due) due[q] = mass_due(I) ? 1 : 0;
 }
 
 template <typename R>
@@ -218,13 +225,14 @@ hipError_t launch_batch_shape(int shape, bool sweep_b, int var, const TileRef *t
 // block b: partial b of every due instance's interior sum of its new c
 template <typename R>
 __global__ void __launch_bounds__(256) k_bmass_interior(const InstDev<R> *__restrict__ inst, int n,
+                                                        const int *__restrict__ due,
                                                         double *__restrict__ part) {
   __shared__ double sh[256];
   for (int q = 0; q < n; ++q) {
+    if (!due[q]) continue;  // uniform over the block
     const InstDev<R> &I = inst[q];
-    if (!mass_due(I)) continue;  // uniform over the block
     const double t = block_sum256(mass_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz,
-                                            (int)blockIdx.x), sh);
+                                            (int)blockIdx.x, I.k.c_off), sh);
     if (threadIdx.x == 0) part[(size_t)q * kMassBlocks + blockIdx.x] = t;
   }
 }
@@ -232,10 +240,11 @@ __global__ void __launch_bounds__(256) k_bmass_interior(const InstDev<R> *__rest
 // block q: instance q's interior sum and factor
 template <typename R>
 __global__ void __launch_bounds__(256) k_bmass_update(InstDev<R> *__restrict__ inst, int n,
+                                                      const int *__restrict__ due,
                                                       const double *__restrict__ part) {
   __shared__ double sh[256];
   const int q = blockIdx.x;
-  if (q >= n || !mass_due(inst[q])) return;
+  if (q >= n || !due[q]) return;
   const double S = mass_fold(part + (size_t)q * kMassBlocks, sh);
   if (threadIdx.x == 0) {
     MassLedger &m = inst[q].m;
@@ -245,12 +254,13 @@ __global__ void __launch_bounds__(256) k_bmass_update(InstDev<R> *__restrict__ i
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restrict__ inst, int n) {
+__global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restrict__ inst, int n,
+                                                     const int *__restrict__ due) {
   for (int q = 0; q < n; ++q) {
+    if (!due[q]) continue;
     const InstDev<R> &I = inst[q];
-    if (!mass_due(I)) continue;
     mass_scale_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, I.m.f,
-                    (int)blockIdx.x);
+                    (int)blockIdx.x, I.k.c_off);
   }
 }
 
@@ -258,10 +268,11 @@ __global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restric
 // (concentracao_soluto.f90:116-124, as after the single-instance correction)
 constexpr int kRefreshBlocks = 512;
 template <typename R>
-__global__ void __launch_bounds__(256) k_bmass_refresh(const InstDev<R> *__restrict__ inst, int n) {
+__global__ void __launch_bounds__(256) k_bmass_refresh(const InstDev<R> *__restrict__ inst, int n,
+                                                       const int *__restrict__ due) {
   for (int q = 0; q < n; ++q) {
+    if (!due[q]) continue;
     const InstDev<R> &I = inst[q];
-    if (!mass_due(I)) continue;
     const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
     R *c = I.c[I.co ^ 1];
     int i, j, k;
@@ -317,6 +328,7 @@ struct pf_batch_slot {
   int steps_total = 0, steps_left = 0;
   int mass_every = 0;      // solute-mass correction period
   double host_mass = 0.0;  // FP64 interior mass of the admitted c field
+  double c_off = 0.0;      // FP32-mixed: c stored as c - c_off (as pf_init)
 };
 
 struct pf_batch {
@@ -362,9 +374,12 @@ struct pf_batch {
   int diag_cap = 0;
   double *pinned = nullptr;
   // partial interior sums of the solute-mass correction (kMassBlocks per
-  // instance of the larger per-precision array)
+  // instance of the larger per-precision array) and, per precision, the
+  // instances due for a correction after the current step
   double *mass_part = nullptr;
   size_t mass_part_cap = 0;
+  int *mass_due[2] = {nullptr, nullptr};
+  int mass_due_cap[2] = {0, 0};
 };
 
 namespace {
@@ -399,6 +414,7 @@ pf::InstDev<R> make_inst(const pf_batch_slot &s) {
   pf::InstDev<R> I;
   memset(&I, 0, sizeof I);
   I.k = pf::kparams_from(s.p);
+  I.k.c_off = s.c_off;
   I.kt = I.k;
   I.s = pf::make_sc<R>(I.k);
   for (int f = 0; f < 2; ++f) {
@@ -503,6 +519,16 @@ int rebuild(pf_batch *b) {
     HIP_OK(hipMalloc((void **)&b->chunks_dev, chunks.size() * sizeof(pf::Chunk)));
     b->chunks_cap = chunks.size();
   }
+  for (int prec = 0; prec < 2; ++prec) {
+    const int need = (int)b->slot_of[prec].size();
+    if (need > b->mass_due_cap[prec]) {
+      if (b->mass_due[prec]) HIP_OK(hipFree(b->mass_due[prec]));
+      b->mass_due[prec] = nullptr;
+      b->mass_due_cap[prec] = 0;
+      HIP_OK(hipMalloc((void **)&b->mass_due[prec], (size_t)need * sizeof(int)));
+      b->mass_due_cap[prec] = need;
+    }
+  }
   const size_t nmp = (size_t)std::max(b->slot_of[0].size(), b->slot_of[1].size()) * pf::kMassBlocks;
   if (nmp > b->mass_part_cap) {
     if (b->mass_part) HIP_OK(hipFree(b->mass_part));
@@ -554,10 +580,10 @@ int enqueue_step(pf_batch *b) {
     const unsigned nb = (unsigned)((n + 63) / 64);
     if (prec == PF_FP64)
       hipLaunchKernelGGL(pf::k_batch_begin<double>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<double> *)b->inst_dev[prec], n);
+                         (pf::InstDev<double> *)b->inst_dev[prec], n, b->mass_due[prec]);
     else
       hipLaunchKernelGGL(pf::k_batch_begin<float>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<float> *)b->inst_dev[prec], n);
+                         (pf::InstDev<float> *)b->inst_dev[prec], n, b->mass_due[prec]);
     HIP_OK(hipGetLastError());
   }
   auto sweeps = [&](bool sweep_b, int sweep) -> int {
@@ -605,24 +631,27 @@ int enqueue_step(pf_batch *b) {
     bool any = false;
     for (int id : b->slot_of[prec]) any = any || b->slots[id].mass_every > 0;
     if (!n || !any) continue;
+    const int *due = b->mass_due[prec];
     if (prec == PF_FP64) {
       auto *I = (pf::InstDev<double> *)b->inst_dev[prec];
       hipLaunchKernelGGL(pf::k_bmass_interior<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
-                         n, b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, I, n,
+                         n, due, b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, I, n, due,
                          b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n);
+      hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n,
+                         due);
       hipLaunchKernelGGL(pf::k_bmass_refresh<double>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                         I, n);
+                         I, n, due);
     } else {
       auto *I = (pf::InstDev<float> *)b->inst_dev[prec];
       hipLaunchKernelGGL(pf::k_bmass_interior<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
-                         n, b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, I, n,
+                         n, due, b->mass_part);
+      hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, I, n, due,
                          b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n);
+      hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n,
+                         due);
       hipLaunchKernelGGL(pf::k_bmass_refresh<float>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                         I, n);
+                         I, n, due);
     }
     HIP_OK(hipGetLastError());
   }
@@ -668,7 +697,7 @@ int capture_graph(pf_batch *b) {
 }
 
 template <typename R>
-int download_field(pf_batch *b, const void *src, size_t n, double *dst) {
+int download_field(pf_batch *b, const void *src, size_t n, double *dst, double off = 0.0) {
   if (!dst) return PF_OK;
   if (sizeof(R) == 8) {
     HIP_OK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyDeviceToHost, b->stream));
@@ -678,18 +707,19 @@ int download_field(pf_batch *b, const void *src, size_t n, double *dst) {
   std::vector<float> tmp(n);
   HIP_OK(hipMemcpyAsync(tmp.data(), src, n * 4, hipMemcpyDeviceToHost, b->stream));
   HIP_OK(hipStreamSynchronize(b->stream));
-  for (size_t q = 0; q < n; ++q) dst[q] = (double)tmp[q];
+  for (size_t q = 0; q < n; ++q) dst[q] = (double)tmp[q] + off;
   return PF_OK;
 }
 
-int upload_field(pf_batch *b, void *dst, size_t n, int prec, const double *src) {
+int upload_field(pf_batch *b, void *dst, size_t n, int prec, const double *src,
+                 double off = 0.0) {
   if (prec == PF_FP64) {
     HIP_OK(hipMemcpyAsync(dst, src, n * 8, hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     return PF_OK;
   }
   std::vector<float> tmp(n);
-  for (size_t q = 0; q < n; ++q) tmp[q] = (float)src[q];
+  for (size_t q = 0; q < n; ++q) tmp[q] = (float)(src[q] - off);
   HIP_OK(hipMemcpyAsync(dst, tmp.data(), n * 4, hipMemcpyHostToDevice, b->stream));
   HIP_OK(hipStreamSynchronize(b->stream));
   return PF_OK;
@@ -764,6 +794,8 @@ int pf_batch_destroy(pf_batch *b) {
   if (b->chunks_dev) (void)hipFree(b->chunks_dev);
   if (b->diag_part) (void)hipFree(b->diag_part);
   if (b->mass_part) (void)hipFree(b->mass_part);
+  for (int prec = 0; prec < 2; ++prec)
+    if (b->mass_due[prec]) (void)hipFree(b->mass_due[prec]);
   if (b->pinned) (void)hipHostFree(b->pinned);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -821,7 +853,8 @@ int pf_batch_add(pf_batch *b, const pf_params *p, const pf_loop *L, int32_t nste
   // phi_old, c_old, T_old into both buffers of each pair (the new buffers are
   // fully rewritten before they are read; T faces stay TOLD in both)
   if ((rc = upload_field(b, s.buf[0], s.n, s.prec, phi)) != PF_OK) return cleanup(rc);
-  if ((rc = upload_field(b, s.buf[2], s.n, s.prec, c)) != PF_OK) return cleanup(rc);
+  if (s.prec == PF_FP32_MIXED) s.c_off = c[s.n - 1];  // as pf_init: the top corner cell
+  if ((rc = upload_field(b, s.buf[2], s.n, s.prec, c, s.c_off)) != PF_OK) return cleanup(rc);
   if ((rc = upload_field(b, s.buf[4], s.n, s.prec, T)) != PF_OK) return cleanup(rc);
   if ((rc = upload_field(b, s.buf[5], s.n, s.prec, T)) != PF_OK) return cleanup(rc);
   // temperatura of step 1 (principal.f90:98) with DPP = 0: c[1] is dead
@@ -1006,7 +1039,7 @@ int pf_batch_download(pf_batch *b, int32_t slot, double *phi, double *c, double 
     return download_field<double>(b, s.buf[4 + (to ^ 1)], s.n, T);
   }
   if ((rc = download_field<float>(b, s.buf[po], s.n, phi)) != PF_OK) return rc;
-  if ((rc = download_field<float>(b, s.buf[2 + co], s.n, c)) != PF_OK) return rc;
+  if ((rc = download_field<float>(b, s.buf[2 + co], s.n, c, s.c_off)) != PF_OK) return rc;
   return download_field<float>(b, s.buf[4 + (to ^ 1)], s.n, T);
 }
 
@@ -1035,7 +1068,8 @@ int pf_batch_diagnosticos(pf_batch *b, int32_t slot, double out[5]) {
     if ((rc = read_inst(b, s, I)) != PF_OK) return rc;
     po = I.po; co = I.co;
   }
-  const pf::KParams k = pf::kparams_from(p);
+  pf::KParams k = pf::kparams_from(p);
+  k.c_off = s.c_off;
   double *outd = b->diag_part + 5 * nblk;
   hipError_t e;
   if (s.prec == PF_FP64)

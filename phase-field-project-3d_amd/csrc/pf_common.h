@@ -210,6 +210,10 @@ struct KParams {
   double R, Vm, E0, W0, M0, delta_eps, AAA, D_liq, D_sol;
   double Ke;  // (double)(float)Ke
   double Cs_eq, Cl_eq;
+  // The FP32-mixed form stores c as c - c_off (c_off: a liquid value of
+  // the initial field), which keeps the solute increments of the near-
+  // uniform liquid that an FP32 c would round away; 0 for FP64 storage.
+  double c_off;
 };
 
 // The kernel scalars of a pf_params (Cs_eq/Cl_eq recomputed from TP, as
@@ -224,6 +228,7 @@ inline KParams kparams_from(const pf_params &p) {
   k.AAA = p.AAA; k.D_liq = p.D_liq; k.D_sol = p.D_sol;
   k.Ke = (double)p.Ke;
   equilibrium(p.TP, p.Tm, p.Ke, p.Me, k.Cs_eq, k.Cl_eq);
+  k.c_off = 0.0;  // set by the owner of the stored field
   return k;
 }
 

@@ -112,7 +112,7 @@ int fused_autotune(FusedState &s, const KParams &k, void *phi_old, void *c_old, 
 // valid: from host_mass while that is valid, else from the stored field),
 // correct the field (every mass_every steps), read {ledger, interior sum at
 // the last correction, last factor}, read / restore the device record.
-int fused_mass_prepare(FusedState &s, const void *c_old, hipStream_t stream);
+int fused_mass_prepare(FusedState &s, const void *c_old, double c_off, hipStream_t stream);
 int fused_mass_correct(FusedState &s, const KParams &k, void *c, hipStream_t stream);
 int fused_mass_read(FusedState &s, double out[3], hipStream_t stream);
 int fused_mass_state(FusedState &s, MassLedger *out, hipStream_t stream);

@@ -498,9 +498,10 @@ namespace {
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_mass_interior(const R *__restrict__ c, int nx, int ny,
-                                                       int nz, double *__restrict__ part) {
+                                                       int nz, double coff,
+                                                       double *__restrict__ part) {
   __shared__ double sh[256];
-  const double t = block_sum256(mass_rows(c, nx, ny, nz, (int)blockIdx.x), sh);
+  const double t = block_sum256(mass_rows(c, nx, ny, nz, (int)blockIdx.x, coff), sh);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -523,21 +524,21 @@ __global__ void __launch_bounds__(256) k_mass_update(const double *__restrict__ 
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_mass_scale(R *__restrict__ c, int nx, int ny, int nz,
-                                                    int it, uint64_t seed,
+                                                    int it, uint64_t seed, double coff,
                                                     const MassLedger *__restrict__ m) {
-  mass_scale_rows(c, nx, ny, nz, it, seed, m->f, (int)blockIdx.x);
+  mass_scale_rows(c, nx, ny, nz, it, seed, m->f, (int)blockIdx.x, coff);
 }
 
 template <typename R>
-hipError_t mass_interior(const FusedState &S, const void *c, hipStream_t st) {
+hipError_t mass_interior(const FusedState &S, const void *c, double coff, hipStream_t st) {
   hipLaunchKernelGGL(k_mass_interior<R>, dim3(kMassBlocks), dim3(256), 0, st, (const R *)c, S.nx,
-                     S.ny, S.nz, S.mpart);
+                     S.ny, S.nz, coff, S.mpart);
   return hipGetLastError();
 }
 
 }  // namespace
 
-int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
+int fused_mass_prepare(FusedState &S, const void *c_old, double coff, hipStream_t st) {
   if (S.mass_every <= 0 || !S.enabled) return PF_OK;
   if (!S.mass) {
     if (hipMalloc((void **)&S.mass, sizeof(MassLedger)) != hipSuccess) return PF_E_NOMEM;
@@ -553,8 +554,8 @@ int fused_mass_prepare(FusedState &S, const void *c_old, hipStream_t st) {
           hipStreamSynchronize(st) != hipSuccess)
         return PF_E_HIP;
     } else {
-      hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c_old, st)
-                                       : mass_interior<float>(S, c_old, st);
+      hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c_old, 0.0, st)
+                                       : mass_interior<float>(S, c_old, coff, st);
       if (e != hipSuccess) return PF_E_HIP;
       hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, S.mass, 0);
       if (hipGetLastError() != hipSuccess) return PF_E_HIP;
@@ -568,19 +569,19 @@ int fused_mass_correct(FusedState &S, const KParams &k, void *c, hipStream_t st)
   if (!S.ledger_valid || !S.mass) return PF_OK;
   const int it = k.it;
   const uint64_t seed = k.seed;
-  hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c, st)
-                                   : mass_interior<float>(S, c, st);
+  hipError_t e = S.prec == PF_FP64 ? mass_interior<double>(S, c, 0.0, st)
+                                   : mass_interior<float>(S, c, k.c_off, st);
   if (e != hipSuccess) return PF_E_HIP;
   hipLaunchKernelGGL(k_mass_update, dim3(1), dim3(256), 0, st, S.mpart, S.mass, 1);
   if (hipGetLastError() != hipSuccess) return PF_E_HIP;
   if (S.prec == PF_FP64) {
     hipLaunchKernelGGL(k_mass_scale<double>, dim3(kMassBlocks), dim3(256), 0, st, (double *)c,
-                       S.nx, S.ny, S.nz, it, seed, S.mass);
+                       S.nx, S.ny, S.nz, it, seed, 0.0, S.mass);
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_solute_refresh<double>(k, (double *)c, st);
   } else {
     hipLaunchKernelGGL(k_mass_scale<float>, dim3(kMassBlocks), dim3(256), 0, st, (float *)c, S.nx,
-                       S.ny, S.nz, it, seed, S.mass);
+                       S.ny, S.nz, it, seed, k.c_off, S.mass);
     e = hipGetLastError();
     if (e == hipSuccess) e = launch_solute_refresh<float>(k, (float *)c, st);
   }

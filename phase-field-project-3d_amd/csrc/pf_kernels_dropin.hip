@@ -14,6 +14,7 @@
 
 #include "pf_common.h"
 #include "pf_kernels.h"
+#include "pf_mass.h"
 
 #pragma clang fp contract(off)
 
@@ -154,7 +155,9 @@ __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict
 #undef P
   const R one = (R)1;
   const R g_linha = (R)2 * PP * (one - PP) * (one - PP) - (R)2 * PP * PP * (one - PP);
-  const R CPP = co[c];
+  // the FP32-mixed form stores c - c_off
+  R CPP = co[c];
+  if constexpr (sizeof(R) == 4) CPP = (R)p.c_off + CPP;
   R h_linha, h;
   if (p.h_quintico == 1) {
     h_linha = (R)30 * PP * PP * (one - PP) * (one - PP);
@@ -227,8 +230,13 @@ __global__ void __launch_bounds__(256) k_solute_interior(KParams p, const R *__r
   const R one = (R)1;
   const R PP = po[c], PN = po[c + sy], PS = po[c - sy], PW = po[c - sx], PE = po[c + sx];
   const R PT = po[c + sz], PB = po[c - sz];
-  const R CPP = co[c], CPS = co[c - sy], CPN = co[c + sy], CPW = co[c - sx], CPE = co[c + sx];
-  const R CPT = co[c + sz], CPB = co[c - sz];
+  // the FP32-mixed form stores c - c_off
+  auto cat = [&](size_t q) -> R {
+    if constexpr (sizeof(R) == 8) return co[q];
+    else return (R)p.c_off + co[q];
+  };
+  const R CPP = cat(c), CPS = cat(c - sy), CPN = cat(c + sy), CPW = cat(c - sx), CPE = cat(c + sx);
+  const R CPT = cat(c + sz), CPB = cat(c - sz);
   R h, hE, hW, hN, hS, hT, hB;
   if (p.h_quintico == 1) {
 #define HQ(x) (((x) * (x) * (x)) * ((R)10 - ((R)15 * (x)) + ((R)6 * ((x) * (x)))))
@@ -263,10 +271,14 @@ __global__ void __launch_bounds__(256) k_solute_interior(KParams p, const R *__r
   const R IX = (R)(1.0 / (p.dx * p.dx)), IY = (R)(1.0 / (p.dy * p.dy)), IZ = (R)(1.0 / (p.dz * p.dz));
 #define MU(cl) rlog<R>((cl) / (one - (cl)))
   const R muP = MU(CLP);
-  cn[c] = (Ge * IX * (MU(CLE) - muP) - Gw * IX * (muP - MU(CLW)) + Gn * IY * (MU(CLN) - muP) -
-           Gs * IY * (muP - MU(CLS)) + Gt * IZ * (MU(CLT) - muP) - Gb * IZ * (muP - MU(CLB))) *
-              (R)p.dt +
-          CPP;
+  // the increment onto the stored value (= CPP for FP64; FP32: unbiased
+  // stochastic rounding, as in the fused sweeps)
+  const R inc = (Ge * IX * (MU(CLE) - muP) - Gw * IX * (muP - MU(CLW)) + Gn * IY * (MU(CLN) - muP) -
+                 Gs * IY * (muP - MU(CLS)) + Gt * IZ * (MU(CLT) - muP) - Gb * IZ * (muP - MU(CLB))) *
+                (R)p.dt;
+  uint32_t hc = 0;
+  if constexpr (sizeof(R) == 4) hc = cell_hash(cell_hash_col(i, j, p.it), k);
+  cn[c] = csum<R>(co[c], inc, hc);
 #undef MU
 }
 
@@ -327,7 +339,8 @@ __global__ void __launch_bounds__(256) k_diag_partial(KParams p, const R *__rest
       s2 += v * v;
       mx = v > mx ? v : mx;
       mn = v < mn ? v : mn;
-      sc += (double)cc[t];
+      if constexpr (sizeof(R) == 8) sc += (double)cc[t];
+      else sc += (double)cc[t] + p.c_off;
       const double w = w0 * v * v * (1.0 - v) * (1.0 - v);
       double g2 = 0.0;
       if (rin && i > 1 && i < nx) {

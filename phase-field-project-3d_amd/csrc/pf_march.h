@@ -51,11 +51,12 @@ struct Tile {
   static_assert(TX % 16 == 0 && NT % 64 == 0, "whole wavefronts, 128-byte row segments");
 };
 
-// Elements of the arithmetic type in a march_tile work-group's LDS arrays:
-// NSLOT phi planes, four node planes and, in sweep B, two solute planes.
-template <typename TL>
-constexpr size_t march_lds_words(bool sweep_b) {
-  return (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0));
+// Bytes of a march_tile work-group's LDS arrays (before the FP64 log
+// table): NSLOT phi planes and four node planes of the arithmetic type R
+// and, in sweep B, two solute node planes (1/gamma and q).
+template <typename R, typename TL>
+constexpr size_t march_arrays_bytes(bool sweep_b) {
+  return (size_t)(NSLOT * TL::PPL + 4 * TL::QPL + (sweep_b ? 2 * TL::QPL : 0)) * sizeof(R);
 }
 
 // log and 1/x in the arithmetic type.  FP64: the ~1 ulp table-driven
@@ -87,20 +88,17 @@ template <>
 __device__ __forceinline__ float rcpn_pos<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // The FP64 logarithm's table in LDS: kLogTabN 16-byte entries placed after
-// a kernel's other LDS arrays (`words` elements of R from the start), filled
-// by the work-group before its first barrier.  The FP32 form has none.
+// a kernel's other LDS arrays (`bytes` from the start), filled by the
+// work-group before its first barrier.  The FP32 form has none.
 template <typename R>
 constexpr size_t log_tab_bytes() {
   return sizeof(R) == 8 ? sizeof(LogEnt) * (size_t)kLogTabN : 0;
 }
+constexpr size_t log_tab_offset(size_t bytes) { return (bytes + 15) & ~(size_t)15; }
 template <typename R>
-constexpr size_t log_tab_offset(size_t words) {
-  return (words * sizeof(R) + 15) & ~(size_t)15;
-}
-template <typename R>
-__device__ __forceinline__ const LogEnt *log_tab_fill(unsigned char *smem, size_t words,
+__device__ __forceinline__ const LogEnt *log_tab_fill(unsigned char *smem, size_t bytes,
                                                       int tid, int nt) {
-  LogEnt *lt = reinterpret_cast<LogEnt *>(smem + log_tab_offset<R>(words));
+  LogEnt *lt = reinterpret_cast<LogEnt *>(smem + log_tab_offset(bytes));
   if constexpr (sizeof(R) == 8) {
     for (int t = tid; t < kLogTabN; t += nt) lt[t] = LogEnt{kLogTab[t][0], kLogTab[t][1]};
   }
@@ -120,6 +118,7 @@ struct SC {
   R M0dt, AAA;
   R Dliq2dt, Dsol2dt;  // 2*D*dt for the solute faces
   R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
+  R coff;              // offset of the stored c (KParams::c_off; 0 for FP64)
 };
 
 // Computed on the host for a single instance and on the device for batched
@@ -142,6 +141,7 @@ __host__ __device__ inline SC<R> make_sc(const KParams &p) {
   s.M0dt = (R)(p.M0 * p.dt); s.AAA = (R)p.AAA;
   s.Dliq2dt = (R)(2.0 * p.D_liq * p.dt); s.Dsol2dt = (R)(2.0 * p.D_sol * p.dt);
   s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
+  s.coff = (R)p.c_off;
   return s;
 }
 
@@ -183,43 +183,58 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
   }
 }
 
+// The full c of a stored value (FP32: stored as c - c_off).
+template <typename R>
+__device__ __forceinline__ R cfull(R stored, const SC<R> &s) {
+  if constexpr (sizeof(R) == 8) return stored;
+  else return s.coff + stored;
+}
+
 // Solute node quantities (conc_body.inc:61-118, 139-144): 1/gamma (gamma
 // without the centre's D_C) and q, the node's share of the chemical
 // potential difference taken across each face by dmu():
 //   FP64: q = mu = ln(CL/(1-CL)) = ln(c/(den-c)), dmu = mu_X - mu_P;
-//   FP32: q = CL, dmu = ln((CL_X (1-CL_P)) / (CL_P (1-CL_X))) (same quantity)
-//         from the exact difference CL_X - CL_P (see dmu<float>).
+//   FP32: q = CL - c_off = (dc + c_off (1-Ke) h) / den for the stored
+//         dc = c - c_off (since 1 - den = (1-Ke) h), dmu = ln((CL_X (1-CL_P))
+//         / (CL_P (1-CL_X))) (the same quantity) from the difference
+//         q_X - q_P = CL_X - CL_P (see dmu<float>).
 // In FP32 the difference of two ~-4 logarithms is noise of about one FP32
 // ulp of mu, which in near-uniform liquid is as large as the true
 // difference and would random-walk c by ~1 ulp per step; the FP32 form takes
-// the exact difference CL_X - CL_P first (c_X - c_P where den = 1).
+// the difference CL_X - CL_P first, which in the liquid (h = 0, den = 1)
+// is dc_X - dc_P: the increments of the near-uniform liquid keep the
+// precision of the stored offsets, far below an FP32 ulp of c.
 template <typename R>
-__device__ __forceinline__ void solute_node(R p, R cc, int quint, const SC<R> &s,
+__device__ __forceinline__ void solute_node(R p, R cst, int quint, const SC<R> &s,
                                             const LogEnt *lt, R &ig, R &q) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
-  const R CL = cc * rcpn<R>(den);
+  const R cc = cfull<R>(cst, s);
+  const R rden = rcpn<R>(den);
+  const R CL = cc * rden;
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
   ig = rcpn<R>(g);
   if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc), lt);
-  else q = CL;
+  else q = (cst + s.coff * ((R)1 - s.Ke) * h) * rden;
 }
 
 template <typename R>
-__device__ __forceinline__ R dmu(R qX, R qP);
+__device__ __forceinline__ R dmu(R qX, R qP, R coff);
 template <>
-__device__ __forceinline__ double dmu<double>(double qX, double qP) { return qX - qP; }
+__device__ __forceinline__ double dmu<double>(double qX, double qP, double) { return qX - qP; }
 template <>
-__device__ __forceinline__ float dmu<float>(float qX, float qP) {
+__device__ __forceinline__ float dmu<float>(float qX, float qP, float coff) {
   // ln r, r = CL_X (1-CL_P) / (CL_P (1-CL_X)), as 2 atanh(t) with
-  // t = (r-1)/(r+1) = (CL_X - CL_P) / ((CL_X + CL_P) - 2 CL_X CL_P).  The
-  // denominator is symmetric as computed and the numerator flips sign
-  // exactly, so t(X,P) = -t(P,X); the odd series and the log difference
-  // below keep dmu(X,P) = -dmu(P,X) bit for bit: a face's flux is the same
-  // from both sides (and carrying it down the column changes nothing).
-  const float sum = qX + qP, prod = qX * qP;  // each symmetric
+  // t = (r-1)/(r+1) = (CL_X - CL_P) / ((CL_X + CL_P) - 2 CL_X CL_P), where
+  // CL = c_off + q.  The denominator is symmetric as computed and the
+  // numerator flips sign exactly, so t(X,P) = -t(P,X); the odd series and
+  // the log difference below keep dmu(X,P) = -dmu(P,X) bit for bit: a
+  // face's flux is the same from both sides (and carrying it down the
+  // column changes nothing).
+  const float cx = coff + qX, cp = coff + qP;
+  const float sum = cx + cp, prod = cx * cp;  // each symmetric
   const float t = (qX - qP) * rcpn<float>(sum - 2.0f * prod);
   const float a = fabsf(t), t2 = t * t;
   // 2 atanh(t) = 2t (1 + t^2/3 + t^4/5 + ...): truncation < 1e-10 relative
@@ -238,8 +253,8 @@ __device__ __forceinline__ float dmu<float>(float qX, float qP) {
   // the grid, have none); which form a lane uses depends on |t| alone.
   float r = ser;
   if (__builtin_amdgcn_ballot_w64(a >= 0.125f) != 0) {
-    const float lg = (__builtin_amdgcn_logf(qX * (1.0f - qP)) -
-                      __builtin_amdgcn_logf(qP * (1.0f - qX))) * 0.693147180559945309f;
+    const float lg = (__builtin_amdgcn_logf(cx * (1.0f - cp)) -
+                      __builtin_amdgcn_logf(cp * (1.0f - cx))) * 0.693147180559945309f;
     r = a < 0.125f ? ser : lg;
   }
   return r;
@@ -255,12 +270,13 @@ template <typename R>
 __device__ __forceinline__ R solute_inc(const R *ig, const R *mu, int nc, int qx, R igc, R muc,
                                         R igP, R muP, R igM, R muM, bool fresh, R &fzu_prev,
                                         R D2dt, const SC<R> &s) {
-  const R fx = dmu<R>(mu[nc + 1], muc) * rcpn<R>(ig[nc + 1] + igc) +
-               dmu<R>(mu[nc - 1], muc) * rcpn<R>(ig[nc - 1] + igc);
-  const R fy = dmu<R>(mu[nc + qx], muc) * rcpn<R>(ig[nc + qx] + igc) +
-               dmu<R>(mu[nc - qx], muc) * rcpn<R>(ig[nc - qx] + igc);
-  const R fzu = dmu<R>(muP, muc) * rcpn<R>(igP + igc);
-  const R fzd = fresh ? dmu<R>(muM, muc) * rcpn<R>(igM + igc) : -fzu_prev;
+  const R co = s.coff;
+  const R fx = dmu<R>(mu[nc + 1], muc, co) * rcpn<R>(ig[nc + 1] + igc) +
+               dmu<R>(mu[nc - 1], muc, co) * rcpn<R>(ig[nc - 1] + igc);
+  const R fy = dmu<R>(mu[nc + qx], muc, co) * rcpn<R>(ig[nc + qx] + igc) +
+               dmu<R>(mu[nc - qx], muc, co) * rcpn<R>(ig[nc - qx] + igc);
+  const R fzu = dmu<R>(muP, muc, co) * rcpn<R>(igP + igc);
+  const R fzd = fresh ? dmu<R>(muM, muc, co) * rcpn<R>(igM + igc) : -fzu_prev;
   fzu_prev = fzu;
   const R fz = fzu + fzd;
   return D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
@@ -331,12 +347,17 @@ enum : int { kVarSmooth = 1, kVarTDiff = 2, kVarAll = 3 };
 
 // Occupancy hints for the kernels around march_tile (waves per SIMD the
 // register allocation must allow): without the optional paths, FP32 sweep A
-// and phase + T fit six (<= 80 VGPRs), the fused FP32 sweep B five (<= 96),
-// FP64 sweep A and phase + T three (<= 168); with them, FP32 fits four; FP64
-// sweep B takes what it needs.
+// and phase + T fit six (<= 80 VGPRs), FP64 sweep A and phase + T three
+// (<= 168); the fused FP32 sweep B (the solute with its ledger units and the
+// stochastically rounded store) four (<= 128; at five it spilled 16
+// registers), three with the smooth noise; with the optional paths the
+// other FP32 sweeps fit four; FP64 sweep B takes what it needs.
 template <typename R, int MODE, int VAR>
 constexpr int march_waves() {
-  if (sizeof(R) == 4) return VAR != 0 ? 4 : (MODE == kSweepB ? 5 : 6);
+  if (sizeof(R) == 4) {
+    if (MODE == kSweepB) return (VAR & kVarSmooth) ? 3 : 4;
+    return VAR != 0 ? 4 : 6;
+  }
   return MODE != kSweepB && VAR == 0 ? 3 : 1;
 }
 
@@ -358,7 +379,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   extern __shared__ __align__(16) unsigned char smem_raw[];
   R *sphi = reinterpret_cast<R *>(smem_raw);  // NSLOT * PPL
   R *snode = sphi + NSLOT * PPL;              // 4 * QPL
-  R *sgm = snode + 4 * QPL;                   // 2 * QPL (sweep B)
+  R *sig = snode + 4 * QPL;                   // QPL (sweep B): 1/gamma
+  R *smu = sig + QPL;                         // QPL (sweep B): q
 
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
@@ -374,7 +396,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   // visible to the whole work-group after the first plane's barrier, before
   // any logarithm
   const LogEnt *lt =
-      log_tab_fill<R>(smem_raw, march_lds_words<TL>(MODE == kSweepB), tid, NT);
+      log_tab_fill<R>(smem_raw, march_arrays_bytes<R, TL>(MODE == kSweepB), tid, NT);
 
   // ring node handled by this thread (tid < NRING): its node-frame coords
   int rlx = 0, rly = 0;
@@ -437,6 +459,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   R cZ = 0;
   R cPPcur = 0;  // c of the plane updated next (sweeps A and phase + T)
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
+  const uint32_t hcol = cell_hash_col(i, j, kp.it);  // csum's draw (FP32 solute)
+  (void)hcol;
   // solute increments of this column for the ledger, in ledger units
   const bool acc_on = SWEEP_B && macc_dst != nullptr;  // uniform
   long long macc = 0;
@@ -553,7 +577,8 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       h_of(PP, quint, h, hl);
       // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
       const R den = s.Ke * h + ((R)1 - h);
-      const R arg = s.omCsCl * (den - cPP) * rcpn<R>(den - s.Ke * cPP);
+      const R cfP = cfull<R>(cPP, s);
+      const R arg = s.omCsCl * (den - cfP) * rcpn<R>(den - s.Ke * cfP);
       const R GG = hl * s.RTV * dlog<R>(fabs(arg), lt) - s.W0 * gl;
       R ruidos = (R)0;
       if (noise) {
@@ -579,17 +604,20 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       }
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
-        const R *ig = sgm, *mu = sgm + QPL;
+        const R *ig = sig, *mu = smu;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
-        bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+        uint32_t hc = 0;
+        if constexpr (sizeof(R) == 4) hc = cell_hash(hcol, k);
+        const R cnew = csum<R>(cPP, inc, hc);
+        bstore(cn + pk, own ? ooff : kNoStore, cnew);
         // x faces of the row (concentracao_soluto.f90:116-124: a face cell of
         // a row inside [2,n-1] in y and z copies its inner neighbour's new
         // value); the face kernel after the sweep leaves them out
         constexpr uint32_t E = (uint32_t)sizeof(R);
         bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
-               cPP + inc);
+               cnew);
         if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
       }
       if constexpr (T_OUT) {
@@ -626,9 +654,9 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
     }
     if constexpr (SWEEP_B) {
       if (ST || q >= ks) {
-        sgm[oc] = igP; sgm[QPL + oc] = muP;
+        sig[oc] = igP; smu[oc] = muP;
         if (tid < NRING) {
-          sgm[rc] = rig; sgm[QPL + rc] = rmu;
+          sig[rc] = rig; smu[rc] = rmu;
         }
       }
     }
@@ -660,7 +688,14 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
 // LDS bytes of a march_tile work-group.
 template <typename R, typename TL>
 constexpr size_t march_lds_bytes(int mode) {
-  return log_tab_offset<R>(march_lds_words<TL>(mode == kSweepB)) + log_tab_bytes<R>();
+  return log_tab_offset(march_arrays_bytes<R, TL>(mode == kSweepB)) + log_tab_bytes<R>();
+}
+
+// LDS of a solute_tile work-group: three node planes of 1/gamma and three
+// of q (arithmetic type), then the FP64 log table.
+template <typename R, typename TL>
+constexpr size_t solute_arrays_bytes() {
+  return (size_t)(6 * TL::QPL) * sizeof(R);
 }
 
 // One work-group's tile and z-segment of the solute update alone
@@ -682,7 +717,9 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   constexpr int TX = TL::TX, NT = TL::NT, QX = TL::QX, QY = TL::QY, QPL = TL::QPL;
   constexpr int NRING = TL::NRING;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  R *sgm = reinterpret_cast<R *>(smem_raw);  // 3 x {1/gamma, mu} x QPL
+  // three node planes of 1/gamma, then three of q
+  R *sig = reinterpret_cast<R *>(smem_raw);
+  R *smu = sig + 3 * QPL;
 
   const int nx = g.nx, ny = g.ny;
   const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * TX + tx;
@@ -693,7 +730,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   const int ke = min(ks + g.seg - 1, g.k_hi);
   const size_t sz = (size_t)nx * ny;
   const int quint = kp.h_quintico;
-  const LogEnt *lt = log_tab_fill<R>(smem_raw, (size_t)(6 * QPL), tid, NT);
+  const LogEnt *lt = log_tab_fill<R>(smem_raw, solute_arrays_bytes<R, TL>(), tid, NT);
   if constexpr (sizeof(R) == 8) __syncthreads();  // the first node's log
   int rlx = 0, rly = 0;
   if (tid < NRING) {
@@ -708,6 +745,8 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       (uint32_t)((clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx) *
       (uint32_t)sizeof(R);
   const int oc = oly * QX + olx, rc = rly * QX + rlx;
+  const uint32_t hcol = cell_hash_col(i, j, kp.it);  // csum's draw (FP32)
+  (void)hcol;
 
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R pZ = 0, cZ = 0;  // phi and c of the own cell of plane k
@@ -735,15 +774,16 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
     constexpr bool ST = decltype(steady)::value;
     const R pcur = pq, ccur = cq, rpc = rp, rcc = rcv;
     fetch(min(q + 1, q_last), pq, cq, rp, rcv);
-    R *gw = sgm + (q % 3) * 2 * QPL;
+    R *gw = sig + (q % 3) * QPL;
+    R *qw = smu + (q % 3) * QPL;
     solute_node(pcur, ccur, quint, s, lt, igP, muP);
     gw[oc] = igP;
-    gw[QPL + oc] = muP;
+    qw[oc] = muP;
     if (tid < NRING) {
       R rig, rmu;
       solute_node(rpc, rcc, quint, s, lt, rig, rmu);
       gw[rc] = rig;
-      gw[QPL + rc] = rmu;
+      qw[rc] = rmu;
     }
     __syncthreads();
     if constexpr (ST) {
@@ -751,16 +791,18 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const size_t pk = (size_t)(k - 1) * sz;
       const R PP = pZ, cPP = cZ;
       const int nc = oc;
-      const R *ig = sgm + (k % 3) * 2 * QPL, *mu = ig + QPL;
+      const R *ig = sig + (k % 3) * QPL, *mu = smu + (k % 3) * QPL;
       const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
       const R igc = igZ, muc = muZ;
       const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
-      bstore(cn + pk, own ? ooff : kNoStore, cPP + inc);
+      uint32_t hc = 0;
+      if constexpr (sizeof(R) == 4) hc = cell_hash(hcol, k);
+      const R cnew = csum<R>(cPP, inc, hc);
+      bstore(cn + pk, own ? ooff : kNoStore, cnew);
       // x faces of the row, as in march_tile's sweep B
       constexpr uint32_t E = (uint32_t)sizeof(R);
-      bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
-             cPP + inc);
+      bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore, cnew);
       if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
@@ -771,9 +813,10 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   if (acc_on) mass_flush<NT>(macc, tid, macc_dst);
 }
 
+// (its log table after the arrays)
 template <typename TL, typename R>
 constexpr size_t solute_lds_bytes() {
-  return log_tab_offset<R>((size_t)(6 * TL::QPL)) + log_tab_bytes<R>();
+  return log_tab_offset(solute_arrays_bytes<R, TL>()) + log_tab_bytes<R>();
 }
 
 

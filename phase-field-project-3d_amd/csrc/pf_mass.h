@@ -86,16 +86,20 @@ __device__ __forceinline__ double block_sum256(double v, double *sh) {
   return sh[0];
 }
 
+// c is the stored field: c itself (FP64) or c - coff (FP32-mixed).
 template <typename R>
 __device__ __forceinline__ double mass_rows(const R *__restrict__ c, int nx, int ny, int nz,
-                                            int b) {
+                                            int b, double coff) {
   const int nrow = (ny - 2) * (nz - 2);
   const size_t sz = (size_t)nx * ny;
   double acc = 0.0;
   for (int r = b; r < nrow; r += kMassBlocks) {
     const int j = 2 + r % (ny - 2), k = 2 + r / (ny - 2);
     const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
-    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) acc += (double)c[row + i - 1];
+    for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) {
+      if constexpr (sizeof(R) == 8) acc += (double)c[row + i - 1];
+      else acc += (double)c[row + i - 1] + coff;
+    }
   }
   return acc;
 }
@@ -107,7 +111,7 @@ __device__ __forceinline__ double mass_fold(const double *__restrict__ part, dou
   return block_sum256(a, sh);
 }
 
-// Rescales the interior rows of block b by m.f.  FP32: the FP64 product is
+// Rescales the interior rows of block b by f.  FP32: the FP64 product is
 // rounded stochastically to one of its two neighbouring floats with the
 // probability that makes the expectation exact (a counter hash of cell and
 // step, salt 4, draws the choice), because the factor is typically
@@ -115,7 +119,7 @@ __device__ __forceinline__ double mass_fold(const double *__restrict__ part, dou
 // cell as it was.
 template <typename R>
 __device__ __forceinline__ void mass_scale_rows(R *__restrict__ c, int nx, int ny, int nz, int it,
-                                                uint64_t seed, double f, int b) {
+                                                uint64_t seed, double f, int b, double coff) {
   const int nrow = (ny - 2) * (nz - 2);
   const size_t sz = (size_t)nx * ny;
   for (int r = b; r < nrow; r += kMassBlocks) {
@@ -123,10 +127,11 @@ __device__ __forceinline__ void mass_scale_rows(R *__restrict__ c, int nx, int n
     const size_t row = (size_t)(j - 1) * nx + (size_t)(k - 1) * sz;
     for (int i = 2 + (int)threadIdx.x; i <= nx - 1; i += 256) {
       const size_t t = row + i - 1;
-      const double x = (double)c[t] * f;
       if constexpr (sizeof(R) == 8) {
-        c[t] = (R)x;
+        c[t] = (R)((double)c[t] * f);
       } else {
+        // the stored offset of the rescaled c
+        const double x = ((double)c[t] + coff) * f - coff;
         const float nearest = (float)x;
         float lo = nearest, hi = nearest;
         if ((double)nearest > x) lo = nextafterf(nearest, -__builtin_huge_valf());
@@ -140,6 +145,53 @@ __device__ __forceinline__ void mass_scale_rows(R *__restrict__ c, int nx, int n
       }
     }
   }
+}
+
+// The stored c after an increment, cst + inc.  FP64: rounded to nearest.
+// FP32: rounded stochastically to one of the two floats around the exact
+// sum, with the probability that makes the expected stored value exact
+// (TwoSum gives the rounding error e of the nearest sum s; the neighbour of s
+// on e's side is taken with probability |e| / gap), the draw from a 32-bit
+// counter hash of the cell and the step (h).  Increments far below an FP32
+// ulp of c -- those of the solid, whose diffusivity is 1e-4 of the
+// liquid's -- are then kept on average instead of all being rounded away,
+// which biased the mass change the FP32 form computes (by ~2e-4 of the FP64
+// change per step) and the interface with it.
+template <typename R>
+__device__ __forceinline__ R csum(R cst, R inc, uint32_t h);
+template <>
+__device__ __forceinline__ double csum<double>(double cst, double inc, uint32_t) {
+  return cst + inc;
+}
+template <>
+__device__ __forceinline__ float csum<float>(float cst, float inc, uint32_t h) {
+#pragma clang fp contract(off)
+  const float s = cst + inc;
+  const float bb = s - cst;
+  const float e = (cst - (s - bb)) + (inc - bb);  // cst + inc = s + e exactly
+  // the float next to s on e's side (s == 0 implies e == 0: both operands are
+  // multiples of the smallest denormal, so is their sum)
+  const int step = (e > 0.f) == (s > 0.f) ? 1 : -1;
+  const float nb = __int_as_float(__float_as_int(s) + step);
+  const float u = (float)(h >> 8) * 0x1p-24f;  // [0, 1)
+  return u * fabsf(nb - s) < fabsf(e) ? nb : s;  // false for e == 0 and for NaN
+}
+
+// 32-bit counter hash of a cell and step for csum: the "lowbias32"
+// finaliser (C. Wellons, hash-prospector) over an odd-multiplier mix of the
+// coordinates.  col = cell_hash_col(i, j, it) once per column, then
+// cell_hash(col, k) per plane.
+__device__ __forceinline__ uint32_t cell_hash_col(int i, int j, int it) {
+  return (uint32_t)i * 0x9E3779B1u ^ (uint32_t)j * 0x85EBCA77u ^ (uint32_t)it * 0x27D4EB2Fu;
+}
+__device__ __forceinline__ uint32_t cell_hash(uint32_t col, int k) {
+  uint32_t h = col ^ ((uint32_t)k * 0xC2B2AE3Du);
+  h ^= h >> 16;
+  h *= 0x7FEB352Du;
+  h ^= h >> 15;
+  h *= 0x846CA68Bu;
+  h ^= h >> 16;
+  return h;
 }
 
 // FP64 interior mass of a host field (the ledger's start when the stored

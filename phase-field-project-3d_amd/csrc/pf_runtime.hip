@@ -83,6 +83,10 @@ struct pf_ctx {
   pf::KTimer timer;
   bool use_fused = true;
   size_t bytes = 0;
+  // FP32-mixed: c is stored as c - c_off (the initial c of the top corner
+  // cell, a liquid value), which keeps the small solute increments of the
+  // near-uniform liquid that an FP32 c would round away; 0 for FP64
+  double c_off = 0.0;
 };
 
 namespace {
@@ -98,10 +102,14 @@ int check_ctx(const pf_ctx *ctx, const pf_params *p) {
   return PF_OK;
 }
 
-pf::KParams kparams(const pf_params &p) { return pf::kparams_from(p); }
+pf::KParams kparams(const pf_ctx *ctx, const pf_params &p) {
+  pf::KParams k = pf::kparams_from(p);
+  k.c_off = ctx->c_off;
+  return k;
+}
 
-// Host FP64 <-> device field (FP64 or FP32).
-int upload(pf_ctx *ctx, void *dst, const double *src) {
+// Host FP64 <-> device field (FP64, or FP32 stored as value - off).
+int upload(pf_ctx *ctx, void *dst, const double *src, double off = 0.0) {
   if (!src) {
     HIP_TRY(hipMemsetAsync(dst, 0, ctx->n * ctx->esz, ctx->stream));
     return PF_OK;
@@ -112,13 +120,13 @@ int upload(pf_ctx *ctx, void *dst, const double *src) {
     return PF_OK;
   }
   std::vector<float> tmp(ctx->n);
-  for (size_t q = 0; q < ctx->n; ++q) tmp[q] = (float)src[q];
+  for (size_t q = 0; q < ctx->n; ++q) tmp[q] = (float)(src[q] - off);
   HIP_TRY(hipMemcpyAsync(dst, tmp.data(), ctx->n * 4, hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return PF_OK;
 }
 
-int download(pf_ctx *ctx, double *dst, const void *src) {
+int download(pf_ctx *ctx, double *dst, const void *src, double off = 0.0) {
   if (!dst) return PF_OK;
   if (ctx->prec == PF_FP64) {
     HIP_TRY(hipMemcpyAsync(dst, src, ctx->n * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -128,7 +136,7 @@ int download(pf_ctx *ctx, double *dst, const void *src) {
   std::vector<float> tmp(ctx->n);
   HIP_TRY(hipMemcpyAsync(tmp.data(), src, ctx->n * 4, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  for (size_t q = 0; q < ctx->n; ++q) dst[q] = (double)tmp[q];
+  for (size_t q = 0; q < ctx->n; ++q) dst[q] = (double)tmp[q] + off;
   return PF_OK;
 }
 
@@ -272,7 +280,7 @@ int pf_diagnosticos_noalloc(const pf_params *p, const void *phi_new, const void 
   double *part = nullptr;
   int rc = diag_scratch(dev, &part);
   if (rc != PF_OK) return rc;
-  const pf::KParams k = kparams(*p);
+  const pf::KParams k = pf::kparams_from(*p);  // the caller's fields: full c
   const hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   if (p->precision == PF_FP64)
@@ -382,8 +390,10 @@ int pf_init(const pf_params *p, const double *phi_old, const double *phi_new, co
   ctx->diag_out = ctx->diag_part + 5 * ctx->diag_nblk;
   if (hipHostMalloc(&ctx->pinned, 8 * sizeof(double)) != hipSuccess)
     return cleanup(fail(PF_E_NOMEM, "hipHostMalloc"));
+  if (ctx->prec == PF_FP32_MIXED && c_old) ctx->c_off = c_old[ctx->n - 1];
   for (int f = 0; f < 7; ++f)
-    if ((rc = upload(ctx, *bufs[f], src[f])) != PF_OK) return cleanup(rc);
+    if ((rc = upload(ctx, *bufs[f], src[f], (f == 2 || f == 3) ? ctx->c_off : 0.0)) != PF_OK)
+      return cleanup(rc);
   if ((rc = pf::fused_init(ctx->fused, ctx->nx, ctx->ny, ctx->nz, ctx->prec)) != PF_OK)
     return cleanup(fail(rc, "fused path set-up failed"));
   // the solute-mass ledger starts from the FP64 mass of the caller's field,
@@ -418,14 +428,14 @@ int pf_temperatura(pf_ctx *ctx, const pf_params *p) {
   int rc = check_ctx(ctx, p);
   if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
   if (rc != PF_OK) return rc;
-  return do_temperatura(ctx, kparams(*p));
+  return do_temperatura(ctx, kparams(ctx, *p));
 }
 
 int pf_equacao_campo_fase(pf_ctx *ctx, const pf_params *p, double probe[3]) {
   int rc = check_ctx(ctx, p);
   if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
   if (rc != PF_OK) return rc;
-  if ((rc = do_phase(ctx, kparams(*p))) != PF_OK) return rc;
+  if ((rc = do_phase(ctx, kparams(ctx, *p))) != PF_OK) return rc;
   if (probe) {
     if ((rc = ensure_probe_cap(ctx, 2)) != PF_OK) return rc;
     if ((rc = launch_probe(ctx, ctx->phi.new_(), ctx->probe_dev)) != PF_OK) return rc;
@@ -450,7 +460,7 @@ int pf_concentracao_soluto(pf_ctx *ctx, const pf_params *p) {
   int rc = check_ctx(ctx, p);
   if (rc == PF_OK && !p) rc = fail(PF_E_ARG, "null params");
   if (rc != PF_OK) return rc;
-  return do_solute(ctx, kparams(*p));
+  return do_solute(ctx, kparams(ctx, *p));
 }
 
 int pf_end_step(pf_ctx *ctx) {
@@ -466,7 +476,7 @@ int pf_diagnosticos(pf_ctx *ctx, const pf_params *p, double out[5]) {
   int rc = check_ctx(ctx, p);
   if (rc == PF_OK && (!p || !out)) rc = fail(PF_E_ARG, "null argument");
   if (rc != PF_OK) return rc;
-  const pf::KParams k = kparams(*p);
+  const pf::KParams k = kparams(ctx, *p);
   hipError_t e;
   if (ctx->prec == PF_FP64)
     e = pf::launch_diag<double>(k, (const double *)ctx->phi.new_(), (const double *)ctx->c.new_(),
@@ -486,7 +496,7 @@ int pf_download(pf_ctx *ctx, double *phi_new, double *c_new, double *T_new) {
   int rc = check_ctx(ctx, nullptr);
   if (rc != PF_OK) return rc;
   if ((rc = download(ctx, phi_new, ctx->phi.new_())) != PF_OK) return rc;
-  if ((rc = download(ctx, c_new, ctx->c.new_())) != PF_OK) return rc;
+  if ((rc = download(ctx, c_new, ctx->c.new_(), ctx->c_off)) != PF_OK) return rc;
   return download(ctx, T_new, ctx->T.new_());
 }
 
@@ -498,7 +508,8 @@ int pf_download_all(pf_ctx *ctx, double *phi_old, double *phi_new, double *c_old
                         ctx->T.old_(),   ctx->T.new_(),   ctx->dpp};
   double *dst[7] = {phi_old, phi_new, c_old, c_new, T_old, T_new, dpp};
   for (int f = 0; f < 7; ++f)
-    if ((rc = download(ctx, dst[f], src[f])) != PF_OK) return rc;
+    if ((rc = download(ctx, dst[f], src[f], (f == 2 || f == 3) ? ctx->c_off : 0.0)) != PF_OK)
+      return rc;
   return PF_OK;
 }
 
@@ -515,8 +526,9 @@ struct CkptHeader {
   int32_t mass_every, ledger_valid, host_mass_ok, pad;
   pf::MassLedger ledger;
   double host_mass;
+  double c_off;  // the stored c is c - c_off (FP32-mixed)
 };
-const char kCkptMagic[8] = {'P', 'F', 'C', 'K', 'P', 'T', '0', '2'};
+const char kCkptMagic[8] = {'P', 'F', 'C', 'K', 'P', 'T', '0', '3'};
 }  // namespace
 
 int pf_checkpoint_save(pf_ctx *ctx, const pf_params *p, const pf_loop *L, const char *path) {
@@ -526,7 +538,7 @@ int pf_checkpoint_save(pf_ctx *ctx, const pf_params *p, const pf_loop *L, const 
   CkptHeader h;
   memset(&h, 0, sizeof h);
   memcpy(h.magic, kCkptMagic, 8);
-  h.version = 2;
+  h.version = 3;
   h.prec = ctx->prec;
   h.nx = ctx->nx; h.ny = ctx->ny; h.nz = ctx->nz;
   h.params_size = (int32_t)sizeof(pf_params);
@@ -537,6 +549,7 @@ int pf_checkpoint_save(pf_ctx *ctx, const pf_params *p, const pf_loop *L, const 
   h.ledger_valid = ctx->fused.ledger_valid ? 1 : 0;
   h.host_mass_ok = ctx->fused.host_mass_ok ? 1 : 0;
   h.host_mass = ctx->fused.host_mass;
+  h.c_off = ctx->c_off;
   if (h.ledger_valid && pf::fused_mass_state(ctx->fused, &h.ledger, ctx->stream) != PF_OK)
     return fail(PF_E_HIP, "reading the ledger failed");
   const void *src[7] = {ctx->phi.old_(), ctx->phi.new_(), ctx->c.old_(), ctx->c.new_(),
@@ -569,7 +582,7 @@ int pf_checkpoint_load(const char *path, pf_params *p, pf_loop *L, pf_ctx **out)
   pf_params pp;
   pf_loop ll;
   bool ok = fread(&h, sizeof h, 1, f) == 1;
-  if (ok && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 2 ||
+  if (ok && (memcmp(h.magic, kCkptMagic, 8) != 0 || h.version != 3 ||
              h.params_size != (int32_t)sizeof(pf_params) || h.loop_size != (int32_t)sizeof(pf_loop))) {
     fclose(f);
     return fail(PF_E_ARG, "%s is not a checkpoint of this library version", path);
@@ -608,9 +621,10 @@ int pf_checkpoint_load(const char *path, pf_params *p, pf_loop *L, pf_ctx **out)
     ctx->fused.ledger_valid = false;
     ctx->fused.host_mass = h.host_mass;
     ctx->fused.host_mass_ok = h.host_mass_ok != 0;
+    ctx->c_off = h.c_off;
     if (h.ledger_valid && h.mass_every > 0) {
       // allocate the ledger, then put the saved record back
-      if (pf::fused_mass_prepare(ctx->fused, ctx->c.old_(), ctx->stream) != PF_OK ||
+      if (pf::fused_mass_prepare(ctx->fused, ctx->c.old_(), ctx->c_off, ctx->stream) != PF_OK ||
           pf::fused_mass_restore(ctx->fused, h.ledger, ctx->stream) != PF_OK)
         rc = fail(PF_E_HIP, "restoring the solute-mass ledger failed");
     }
@@ -712,7 +726,7 @@ int pf_autotune(pf_ctx *ctx, const pf_params *p, int32_t reps, int32_t *ncand, d
   if (!ctx->fused.enabled) return fail(PF_E_STATE, "grid too small for the fused sweeps");
   HIP_TRY(hipSetDevice(ctx->device));
   pf::AutotuneResult res;
-  rc = pf::fused_autotune(ctx->fused, kparams(*p), ctx->phi.old_(), ctx->c.old_(),
+  rc = pf::fused_autotune(ctx->fused, kparams(ctx, *p), ctx->phi.old_(), ctx->c.old_(),
                           ctx->T.old_(), reps, ctx->stream, &res);
   if (rc == PF_E_NOMEM) return fail(rc, "no device memory for the autotune scratch fields");
   if (rc != PF_OK) return fail(rc, "autotune launch failed: %s",
@@ -791,7 +805,7 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
   const bool fused = ctx->use_fused && pf::fused_supported(ctx->fused, *p);
   pf::FusedState &fst = ctx->fused;
   if (fused) {
-    if ((rc = pf::fused_mass_prepare(fst, ctx->c.old_(), ctx->stream)) != PF_OK)
+    if ((rc = pf::fused_mass_prepare(fst, ctx->c.old_(), ctx->c_off, ctx->stream)) != PF_OK)
       return fail(rc, "solute-mass ledger set-up failed");
   } else {
     fst.ledger_valid = false;  // the unfused kernels do not keep the ledger
@@ -802,10 +816,10 @@ int pf_run_steps(pf_ctx *ctx, pf_params *p, pf_loop *L, int32_t nsteps, double *
     p->it += 1;
     const int it = p->it;
     L->time = (float)(p->dt + (double)L->time);  // principal.f90:96
-    const pf::KParams k_temp = kparams(*p);      // temperatura sees TP of step it-1
+    const pf::KParams k_temp = kparams(ctx, *p);      // temperatura sees TP of step it-1
     if (L->taxa_resfr > 0.0) p->TP = L->T0 - L->taxa_resfr * p->dt * (double)it;  // :104
     pf_equilibrium(p->TP, p->Tm, p->Ke, p->Me, &p->Cs_eq, &p->Cl_eq);
-    const pf::KParams k = kparams(*p);
+    const pf::KParams k = kparams(ctx, *p);
     if (probes) tp_seen.push_back(p->TP);
     double *pr = probes ? ctx->probe_dev + (size_t)2 * nv * s : nullptr;
     if (fused) {

@@ -1,10 +1,13 @@
 """GPU test of pf_diagnosticos_noalloc, the diagnostics entry point that
 allocates nothing: called through the C-ABI on device arrays the test
 allocates itself (HIP runtime through ctypes), its output equals
-pf_diagnosticos of a context holding the same fields, bit for bit -- on
-three calls in a row, each on a smaller grid than the one before (the
-fixed scratch array keeps the previous call's partial sums), FP64 and
-FP32-mixed."""
+pf_diagnosticos of a context holding the same fields -- on three calls in a
+row, each on a smaller grid than the one before (the fixed scratch array
+keeps the previous call's partial sums).  FP64: bit for bit.  FP32: the
+caller's arrays hold full c values, while a context stores c - c_off and
+sums (c - c_off) + c_off in FP64; the four phi-based outputs agree bit for
+bit and the solute total to 1e-8 relative (the FP32 rounding of each c,
+unbiased, summed over the grid)."""
 import ctypes as C
 
 import numpy as np
@@ -70,7 +73,11 @@ def test_noalloc_diagnostics_equal_context():
             for q in (dphi, dc, dout):
                 hip.hipFree(q)
         print(f"  {out} vs {ref}", flush=True)
-        assert np.array_equal(out, ref), (n, prec)
+        if prec == capi.PF_FP64:
+            assert np.array_equal(out, ref), (n, prec)
+        else:
+            assert np.array_equal(out[[0, 1, 2, 4]], ref[[0, 1, 2, 4]]), (n, prec)
+            assert abs(out[3] - ref[3]) <= 1e-8 * abs(ref[3]), (n, prec)
     # rejected before any device work: a grid below the minimum, null fields
     tiny = _dev(hip, np.zeros(8))
     try:
