@@ -330,6 +330,10 @@ int pf_batch_set_group_target(pf_batch *b, int32_t target_groups);
  * T_new reported for the instances is not kept until their next step).
  * Synchronous. */
 int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
+/* Sweep B of the batched steps as one fused kernel (on = 0) or split into
+ * a phase + T kernel and a solute kernel over the same tiles (on = 1, the
+ * default), as pf_set_sweep_split; both forms give the same bits. */
+int pf_batch_set_sweep_split(pf_batch *b, int32_t on);
 /* The instance's solute-mass correction period (as pf_set_mass_correction;
  * 0 = off); only before its first step (PF_E_STATE after). */
 int pf_batch_set_mass_correction(pf_batch *b, int32_t slot, int32_t every);

@@ -118,9 +118,12 @@ __global__ void k_batch_end(InstDev<R> *__restrict__ inst, int n) {
 
 // ---- batched sweeps and decorations ----------------------------------------
 
-template <typename R, bool SWEEP_B, typename TL, int VAR>
+// One batched sweep of kind MODE (kSweepA: a phase sweep; kSweepB: the last
+// phase sweep + solute + next T; kSweepPT: the last phase sweep + next T,
+// the solute then running in k_solute_batch over the same tiles).
+template <typename R, int MODE, typename TL, int VAR>
 __global__ void __launch_bounds__(TL::NT)
-    __attribute__((amdgpu_waves_per_eu(march_waves<R, SWEEP_B ? kSweepB : kSweepA, VAR>())))
+    __attribute__((amdgpu_waves_per_eu(march_waves<R, MODE, VAR>())))
     k_march_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
                   InstDev<R> *__restrict__ inst, int sweep, int ledger) {
   const int b = blockIdx.x;
@@ -130,15 +133,38 @@ __global__ void __launch_bounds__(TL::NT)
   const InstDev<R> &I = inst[d.inst];
   if (!I.active) return;
   const int pin = I.po ^ ((sweep - 1) & 1), pout = pin ^ 1;
-  if constexpr (SWEEP_B)
+  if constexpr (MODE == kSweepB)
     march_tile<R, kSweepB, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        I.c[I.co ^ 1], I.T[I.to], I.T[I.to ^ 1], nullptr, 1,
                                        ledger && I.mass_every > 0 ? &inst[d.inst].m.acc : nullptr,
                                        d.bx, d.by, d.sg);
+  else if constexpr (MODE == kSweepPT)
+    march_tile<R, kSweepPT, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
+                                        nullptr, I.T[I.to], I.T[I.to ^ 1], nullptr, 1, nullptr,
+                                        d.bx, d.by, d.sg);
   else
     march_tile<R, kSweepA, TL, VAR>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
                                        nullptr, nullptr, nullptr, nullptr, 0, nullptr, d.bx,
                                        d.by, d.sg);
+}
+
+// The solute update of the split sweep B over the same tile table: reads
+// the phi and c sweep B reads, writes c (solute_tile, the single-instance
+// split form's body).
+template <typename R, typename TL>
+__global__ void __launch_bounds__(TL::NT)
+    k_solute_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
+                   InstDev<R> *__restrict__ inst, int sweep, int ledger) {
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= ntiles) return;
+  const TileRef d = tiles[t];
+  const InstDev<R> &I = inst[d.inst];
+  if (!I.active) return;
+  const int pin = I.po ^ ((sweep - 1) & 1);
+  solute_tile<R, TL>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.c[I.co ^ 1],
+                     ledger && I.mass_every > 0 ? &inst[d.inst].m.acc : nullptr, d.bx, d.by,
+                     d.sg);
 }
 
 // which = 0: phase decoration of sweep `sweep` (shell of width 2);
@@ -165,55 +191,71 @@ __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ ch
   }
 }
 
+// sweep kinds of a batched launch: kSweepA, kSweepB, kSweepPT, and
+// kSweepSolute (k_solute_batch)
+constexpr int kSweepSolute = 3;
+
 template <typename R, typename TL, int VAR>
-void launch_batch_march_v(bool sweep_b, const TileRef *tiles, int ntiles, InstDev<R> *inst,
+void launch_batch_march_v(int mode, const TileRef *tiles, int ntiles, InstDev<R> *inst,
                           int sweep, int ledger, hipStream_t st) {
   const int per_xcd = (ntiles + 7) / 8;
-  const size_t lds = march_lds_bytes<R, TL>(sweep_b ? kSweepB : kSweepA);
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
-  if (sweep_b)
-    hipLaunchKernelGGL((k_march_batch<R, true, TL, VAR>), grd, blk, lds, st, tiles, ntiles,
+  const size_t lds_b = march_lds_bytes<R, TL>(kSweepB), lds_pt = march_lds_bytes<R, TL>(kSweepPT);
+  const size_t lds_a = march_lds_bytes<R, TL>(kSweepA), lds_s = solute_lds_bytes<TL, R>();
+  if (mode == kSweepB)
+    hipLaunchKernelGGL((k_march_batch<R, kSweepB, TL, VAR>), grd, blk, lds_b, st, tiles, ntiles,
                        per_xcd, inst, sweep, ledger);
+  else if (mode == kSweepPT)
+    hipLaunchKernelGGL((k_march_batch<R, kSweepPT, TL, VAR>), grd, blk, lds_pt, st, tiles, ntiles,
+                       per_xcd, inst, sweep, 0);
+  else if (mode == kSweepSolute)
+    hipLaunchKernelGGL((k_solute_batch<R, TL>), grd, blk, lds_s, st, tiles, ntiles, per_xcd, inst,
+                       sweep, ledger);
   else
-    hipLaunchKernelGGL((k_march_batch<R, false, TL, VAR & kVarSmooth>), grd, blk, lds, st, tiles,
-                       ntiles, per_xcd, inst, sweep, 0);
+    hipLaunchKernelGGL((k_march_batch<R, kSweepA, TL, VAR & kVarSmooth>), grd, blk, lds_a, st,
+                       tiles, ntiles, per_xcd, inst, sweep, 0);
 }
 
 // var: the optional paths (kVar*) some instance of the group needs (the
 // kernels without a path must not see an instance that needs it); ledger:
-// sweep B adds its solute increments to the instances' ledgers (off for the
-// autotuner's timing launches)
+// sweep B (or the split form's solute) adds its solute increments to the
+// instances' ledgers (off for the autotuner's timing launches)
 template <typename R, typename TL>
-hipError_t launch_batch_march(bool sweep_b, int var, const TileRef *tiles, int ntiles,
+hipError_t launch_batch_march(int mode, int var, const TileRef *tiles, int ntiles,
                               InstDev<R> *inst, int sweep, int ledger, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
   switch (var) {
-    case 0: launch_batch_march_v<R, TL, 0>(sweep_b, tiles, ntiles, inst, sweep, ledger, st); break;
+    case 0: launch_batch_march_v<R, TL, 0>(mode, tiles, ntiles, inst, sweep, ledger, st); break;
     case kVarSmooth:
-      launch_batch_march_v<R, TL, kVarSmooth>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
+      launch_batch_march_v<R, TL, kVarSmooth>(mode, tiles, ntiles, inst, sweep, ledger, st);
       break;
     case kVarTDiff:
-      launch_batch_march_v<R, TL, kVarTDiff>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
+      launch_batch_march_v<R, TL, kVarTDiff>(mode, tiles, ntiles, inst, sweep, ledger, st);
       break;
     default:
-      launch_batch_march_v<R, TL, kVarAll>(sweep_b, tiles, ntiles, inst, sweep, ledger, st);
+      launch_batch_march_v<R, TL, kVarAll>(mode, tiles, ntiles, inst, sweep, ledger, st);
   }
   return hipGetLastError();
 }
 
+// The launches of one group: sweep A, or sweep B fused or split (the phase
+// + T kernel, then the solute kernel over the same tiles).
 template <typename R>
-hipError_t launch_batch_shape(int shape, bool sweep_b, int var, const TileRef *tiles, int ntiles,
-                              InstDev<R> *inst, int sweep, int ledger, hipStream_t st) {
-  switch (shape) {
-    case 1:
-      return launch_batch_march<R, Tile32x8>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
-    case 2:
-      return launch_batch_march<R, Tile32x16>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
-    case 3:
-      return launch_batch_march<R, Tile16x16>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
-    default:
-      return launch_batch_march<R, Tile64x8>(sweep_b, var, tiles, ntiles, inst, sweep, ledger, st);
-  }
+hipError_t launch_batch_shape(int shape, bool sweep_b, bool split, int var, const TileRef *tiles,
+                              int ntiles, InstDev<R> *inst, int sweep, int ledger,
+                              hipStream_t st) {
+  auto one = [&](int mode) -> hipError_t {
+    switch (shape) {
+      case 1: return launch_batch_march<R, Tile32x8>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
+      case 2: return launch_batch_march<R, Tile32x16>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
+      case 3: return launch_batch_march<R, Tile16x16>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
+      default: return launch_batch_march<R, Tile64x8>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
+    }
+  };
+  if (!sweep_b) return one(kSweepA);
+  if (!split) return one(kSweepB);
+  const hipError_t e = one(kSweepPT);
+  return e != hipSuccess ? e : one(kSweepSolute);
 }
 
 // ---- solute-mass correction of the instances due this step (pf_mass.h) -----
@@ -345,6 +387,9 @@ struct pf_batch {
   // over the whole batch (each instance gets its share by cell count)
   int cls_shape[pf::kNClasses];
   int group_target = 16384;
+  // sweep B split into the phase + T kernel and the solute kernel (as the
+  // single-instance split form; same bits)
+  bool split_b = true;
   size_t total_cells = 0;
   // tables (rebuilt when instances change)
   bool dirty = true;
@@ -591,10 +636,12 @@ int enqueue_step(pf_batch *b) {
       if (g.sweep_b != sweep_b) continue;
       hipError_t e;
       if (g.prec == PF_FP64)
-        e = pf::launch_batch_shape<double>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
+        e = pf::launch_batch_shape<double>(g.shape, sweep_b, b->split_b, g.var,
+                                           b->tiles_dev + g.off, g.n,
                                            (pf::InstDev<double> *)b->inst_dev[0], sweep, 1, st);
       else
-        e = pf::launch_batch_shape<float>(g.shape, sweep_b, g.var, b->tiles_dev + g.off, g.n,
+        e = pf::launch_batch_shape<float>(g.shape, sweep_b, b->split_b, g.var,
+                                          b->tiles_dev + g.off, g.n,
                                           (pf::InstDev<float> *)b->inst_dev[1], sweep, 1, st);
       if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
     }
@@ -1158,11 +1205,13 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
           if (e != hipSuccess) break;
           // timing launches: the ledgers are left alone
           if (g.prec == PF_FP64)
-            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
+            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, b->split_b, g.var,
+                                               b->tiles_dev + g.off, g.n,
                                                (pf::InstDev<double> *)b->inst_dev[0], 1, 0,
                                                b->stream);
           else
-            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, g.var, b->tiles_dev + g.off, g.n,
+            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, b->split_b, g.var,
+                                              b->tiles_dev + g.off, g.n,
                                               (pf::InstDev<float> *)b->inst_dev[1], 1, 0,
                                               b->stream);
         }
@@ -1188,6 +1237,15 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
   (void)hipEventDestroy(e1);
   b->dirty = true;
   return rc;
+}
+
+int pf_batch_set_sweep_split(pf_batch *b, int32_t on) {
+  int rc = check_batch(b);
+  if (rc != PF_OK) return rc;
+  HIP_OK(hipStreamSynchronize(b->stream));
+  b->split_b = on != 0;
+  b->dirty = true;  // recapture the steps
+  return PF_OK;
 }
 
 int pf_batch_set_mass_correction(pf_batch *b, int32_t slot, int32_t every) {

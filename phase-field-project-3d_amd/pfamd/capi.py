@@ -80,7 +80,7 @@ EXPORTS = (
     "pf_batch_download", "pf_batch_diagnosticos", "pf_batch_bytes", "pf_batch_size_class",
     "pf_batch_set_tiling", "pf_batch_get_tiling", "pf_batch_set_group_target",
     "pf_batch_autotune", "pf_batch_set_mass_correction", "pf_batch_mass_ledger",
-    "pf_diagnosticos_noalloc",
+    "pf_diagnosticos_noalloc", "pf_batch_set_sweep_split",
 )
 TILE_SHAPES = ((64, 8), (32, 8), (32, 16), (16, 16))
 PF_KT_NAMES = ("sweep_A", "sweep_B", "boundary", "temperatura", "dropin", "mass", "solute")
@@ -146,6 +146,7 @@ def load(path: str = LIB_PATH):
         "pf_batch_set_group_target": ([vp, i32], C.c_int),
         "pf_batch_autotune": ([vp, i32, i32], C.c_int),
         "pf_batch_set_mass_correction": ([vp, i32, i32], C.c_int),
+        "pf_batch_set_sweep_split": ([vp, i32], C.c_int),
         "pf_batch_mass_ledger": ([vp, i32, vp], C.c_int),
         "pf_diagnosticos_noalloc": ([P, vp, vp, vp, vp], C.c_int),
         "pf_mass_ledger": ([vp, vp], C.c_int),
@@ -441,6 +442,9 @@ class Batch:
 
     def set_mass_correction(self, slot: int, every: int):
         check(self.lib.pf_batch_set_mass_correction(self.h, int(slot), int(every)), self.lib)
+
+    def set_sweep_split(self, on: bool):
+        check(self.lib.pf_batch_set_sweep_split(self.h, 1 if on else 0), self.lib)
 
     def mass_ledger(self, slot: int):
         out = np.zeros(3)

@@ -171,3 +171,20 @@ def test_batch_fp32_mass_correction_equals_single():
     sim.close()
     _same(got_off, ref_off, "correction off")
     assert not np.array_equal(got_off[1], got["fp32_seed_small"][0][1])
+
+
+def test_batch_split_and_fused_sweep_b_agree():
+    """Sweep B of the batched steps fused or split into the phase + T and the
+    solute kernels: the same bits, FP64 and FP32 (with a mass correction)."""
+    print("[batch] split vs fused sweep B", flush=True)
+    outs = []
+    for split in (True, False):
+        with capi.Batch() as b:
+            b.set_sweep_split(split)
+            b.set_graph_steps(6)
+            slots = {nm: _add(b, nm, 105) for nm in ("fp64_seed_noise", "fp32_seed_small")}
+            b.run(105)
+            outs.append({nm: (_from_batch(b, sl, nm), b.mass_ledger(sl)) for nm, sl in slots.items()})
+    for nm in outs[0]:
+        _same(outs[0][nm][0], outs[1][nm][0], nm)
+        assert outs[0][nm][1] == outs[1][nm][1], nm
