@@ -5,14 +5,18 @@ A request is a Problem (grid, cooling rate, seeds, step count, precision)
 with an arrival time.  The server loop, one tick at a time:
   1. admits arrived requests in arrival order while the batch's HBM budget
      has room for their fields and the cells in flight stay under
-     `max_active_cells` (a request that does not fit waits, and so do the ones
-     behind it: no overtaking).  The cap matters for latency: every admitted
-     run advances by the same number of steps per tick, so beyond the cell
-     count that saturates the GPU more runs in flight only stretch each of
-     them;
-  2. advances every admitted run by up to `tick_steps` steps (one batched
+     `max_active_cells`; a request that does not fit waits, and a later one
+     that fits may overtake it, at most `overtake_limit` times in a row (then
+     the head blocks admission until it fits).  The cap matters for latency:
+     every admitted run advances by the same number of steps per tick, so
+     beyond the cell count that saturates the GPU more runs in flight only
+     stretch each of them;
+  2. enqueues up to `tick_steps` steps of every admitted run (one batched
      launch per sweep for all of them; runs that finish mid-tick stop at their
-     own step count on the device);
+     own step count on the device) -- the tick ends at the first completion
+     when `end_at_completion`, else it runs its full length -- and, while the
+     GPU works, builds the initial fields of requests that have arrived (host
+     work that would otherwise leave the GPU idle at admission);
   3. retires the runs that completed: downloads phi/c/T and the diagnostics,
      frees their memory, and records the completion time.
 Requests that arrive while others are on the GPU are therefore admitted as
@@ -26,7 +30,6 @@ sleeps when it has nothing to run and the next arrival is in the future).
 from __future__ import annotations
 
 import time
-from collections import deque
 from dataclasses import dataclass
 from typing import Callable, Dict, List, Optional
 
@@ -64,6 +67,8 @@ class ServeReport:
     busy_ticks: int = 0
     peak_bytes: int = 0
     peak_instances: int = 0
+    gpu_wait_s: float = 0.0  # host time spent waiting for ticks to finish
+    overtakes: int = 0
 
     def latencies(self):
         return np.array([c.latency for c in self.completions])
@@ -78,7 +83,10 @@ class ServeReport:
                 "max_s": float(lat.max()),
                 "makespan_s": self.makespan,
                 "peak_instances": self.peak_instances,
-                "peak_bytes": self.peak_bytes}
+                "peak_bytes": self.peak_bytes,
+                "ticks": self.busy_ticks,
+                "gpu_wait_s": self.gpu_wait_s,
+                "overtakes": self.overtakes}
 
 
 def request_stream(n: int = 32, rate: float = 2.0, seed: int = 11, steps_lo: int = 1000,
@@ -104,38 +112,51 @@ def instance_bytes(problem: Problem) -> int:
 
 def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int = 200,
                   graph_steps: int = 16, keep: bool = False, autotune: bool = True,
-                  max_active_cells: Optional[int] = None, lib=None,
+                  max_active_cells: Optional[int] = None, end_at_completion: bool = True,
+                  overtake_limit: int = 8, lib=None,
                   log: Optional[Callable[[str], None]] = None) -> ServeReport:
     lib = lib or capi.load()
     b = capi.Batch(hbm_budget, lib=lib)
     try:
         b.set_graph_steps(graph_steps)
         _, budget = b.bytes()
-        queue = deque(sorted(requests, key=lambda r: r.arrival))
+        queue = sorted(requests, key=lambda r: r.arrival)
         active: Dict[int, tuple] = {}  # slot -> (request, admitted time)
+        prepared: Dict[int, tuple] = {}  # rid -> (setup, initial fields) built ahead
         tuned = set()
         done: List[Completion] = []
         rep = ServeReport(done, 0.0)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
+        head_skips = 0
         while queue or active:
-            # 1. admission by free HBM, in arrival order (inputs are built on
-            # the host at admission, inside the measured latency)
-            while queue and queue[0].arrival <= now():
-                r = queue[0]
+            # 1. admission by free HBM in arrival order, overtaking bounded
+            q = 0
+            while q < len(queue) and queue[q].arrival <= now():
+                r = queue[q]
                 used, _ = b.bytes()
-                if used + instance_bytes(r.problem) > budget:
-                    if not active:
+                fits = used + instance_bytes(r.problem) <= budget
+                in_flight = sum(x.problem.cells for x, _ in active.values())
+                if fits and active and max_active_cells and \
+                        in_flight + r.problem.cells > max_active_cells:
+                    fits = False
+                if not fits:
+                    if not active and used + instance_bytes(r.problem) > budget:
                         raise RuntimeError(f"request {r.rid} needs more than the HBM budget")
-                    break
-                in_flight = sum(q.problem.cells for q, _ in active.values())
-                if active and max_active_cells and in_flight + r.problem.cells > max_active_cells:
-                    break
-                s, f = r.problem.setup(lib)
+                    if q == 0 and head_skips >= overtake_limit:
+                        break  # the head has been overtaken enough: wait for it
+                    q += 1
+                    continue
+                s, f = prepared.pop(r.rid, None) or r.problem.setup(lib)
                 slot = b.add(s.p, s.L, r.problem.steps, f[0], f[2], f[4])
                 del f
                 active[slot] = (r, now())
-                queue.popleft()
+                queue.pop(q)
+                if q > 0:
+                    head_skips += 1
+                    rep.overtakes += 1
+                else:
+                    head_skips = 0
                 if autotune:
                     cls = b.size_class(r.problem.precision, r.problem.deck.Nx, r.problem.deck.Ny)
                     if cls not in tuned:  # first instance of a size class: tune it
@@ -149,10 +170,19 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
                 if wait > 0:
                     time.sleep(wait)
                 continue
-            # 2. advance every admitted run; stop the tick at the first completion
-            k = min(tick_steps, min(b.status(sl)[1] for sl in active))
+            # 2. advance every admitted run (asynchronously) and build the
+            # inputs of arrived requests while the GPU works
+            left = [b.status(sl)[1] for sl in active]
+            k = min(tick_steps, min(left) if end_at_completion else max(left))
             b.run(k)
+            for r in queue:
+                if r.arrival > now():
+                    break
+                if r.rid not in prepared:
+                    prepared[r.rid] = r.problem.setup(lib)
+            tw = time.perf_counter()
             b.synchronize()
+            rep.gpu_wait_s += time.perf_counter() - tw
             rep.busy_ticks += 1
             # 3. retire the completed runs
             for sl in [sl for sl in active if b.status(sl)[1] == 0]:
