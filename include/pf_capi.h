@@ -99,6 +99,10 @@ const char *pf_version(void);
 const char *pf_last_error(void);
 /* Number of HIP devices visible (0 on a host without a GPU). */
 int pf_device_count(int32_t *n);
+/* The HIP device a context was created on (the current device at pf_init),
+ * for the driver's timing block (principal.f90:380-383 prints the offload
+ * device id). */
+int pf_ctx_device(const pf_ctx *ctx, int32_t *dev);
 
 /* ---- host scalar physics (bit-exact with the reference's host code) ------- */
 /* source/variaveis.f90:73-92 (+ a seed term that is 0 for seed=0). */
@@ -110,6 +114,11 @@ double pf_ruido_suave(int32_t i, int32_t j, int32_t k, int32_t it, uint64_t seed
  * loop increments taken (0-based index of the accepted try) or -1 when the
  * loop ran out without meeting the tolerance.  E0/W0 hold the last values. */
 int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E0);
+/* source/W0_E0.f90:19: the value the reference prints for every try,
+ * |E0 - EPSLON_Try| (Inf for the first try, whose W0 is 0).  Writes the
+ * first min(cap, tries) values to diff and returns the number of tries up to
+ * and including the accepted one (pf_w0_e0's index + 1), or -1 as pf_w0_e0. */
+int32_t pf_w0_e0_tries(double lambda, float ajuste, float sigma, double *diff, int32_t cap);
 /* source/auxiliar.f90:1-49: thin-interface mobility M0(TP). */
 double pf_auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm,
                       double E0, double W0, float Sigma, float Beta, float Ke,

@@ -26,8 +26,10 @@ double pf_ruido_suave(int32_t i, int32_t j, int32_t k, int32_t it, uint64_t seed
 
 // source/W0_E0.f90:10-28.  The DO variable is REAL(8) with a REAL(4) step: it
 // starts at 0 and is incremented by (double)1e-7f after each pass (trip count
-// fixed up front), and the first try with |E0 - try| < 1e-5 is kept.
-int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E0) {
+// fixed up front), and the first try with |E0 - try| < 1e-5 is kept.  Each
+// try's |E0 - try| goes to diff[n] while n < cap (the reference prints it).
+static int32_t w0_e0_search(double lambda, float ajuste, float sigma, double *W0, double *E0,
+                            double *diff, int32_t cap) {
   const double step = (double)0.0000001f;
   const long long trips = (long long)((1.0 - 0.0 + step) / step);
   const float as = ajuste * sigma;  // REAL(4) product
@@ -36,6 +38,7 @@ int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E
     const double a = (1.5556349186104 / lambda) * t;
     w = a * a;
     e = (double)as * 4.24264068711929 / sqrt(w);
+    if (diff && n < cap) diff[n] = fabs(e - t);
     if (fabs(e - t) < 1e-05) {
       if (W0) *W0 = w;
       if (E0) *E0 = e;
@@ -46,6 +49,15 @@ int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E
   if (W0) *W0 = w;
   if (E0) *E0 = e;
   return -1;
+}
+
+int32_t pf_w0_e0(double lambda, float ajuste, float sigma, double *W0, double *E0) {
+  return w0_e0_search(lambda, ajuste, sigma, W0, E0, nullptr, 0);
+}
+
+int32_t pf_w0_e0_tries(double lambda, float ajuste, float sigma, double *diff, int32_t cap) {
+  const int32_t idx = w0_e0_search(lambda, ajuste, sigma, nullptr, nullptr, diff, cap);
+  return idx < 0 ? -1 : idx + 1;
 }
 
 // source/auxiliar.f90:14-42 (pf::auxiliar_m0 in pf_common.h).

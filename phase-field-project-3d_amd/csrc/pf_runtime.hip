@@ -346,6 +346,12 @@ int pf_device_count(int32_t *n) {
   return PF_OK;
 }
 
+int pf_ctx_device(const pf_ctx *ctx, int32_t *dev) {
+  if (!ctx || !dev) return fail(PF_E_ARG, "null context or output");
+  *dev = ctx->device;
+  return PF_OK;
+}
+
 int pf_init(const pf_params *p, const double *phi_old, const double *phi_new, const double *c_old,
             const double *c_new, const double *T_old, const double *T_new, const double *dpp,
             pf_ctx **out) {

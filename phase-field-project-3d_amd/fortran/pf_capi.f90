@@ -46,6 +46,19 @@ module pf_capi
       real(c_float), value :: ajuste, sigma
       real(c_double), intent(out) :: W0, E0
     end function
+    integer(c_int) function pf_ctx_device(ctx, dev) bind(C, name='pf_ctx_device')
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int32_t), intent(out) :: dev
+    end function
+    integer(c_int32_t) function pf_w0_e0_tries(lambda, ajuste, sigma, diff, cap) &
+        bind(C, name='pf_w0_e0_tries')
+      import :: c_int32_t, c_double, c_float
+      real(c_double), value :: lambda
+      real(c_float), value :: ajuste, sigma
+      real(c_double), intent(out) :: diff(*)
+      integer(c_int32_t), value :: cap
+    end function
     real(c_double) function pf_auxiliar_m0(TP, Cl_eq, Cs_eq, R, Vm, E0, W0, Sigma, Beta, Ke, Me, &
                                            D_liq, ajuste) bind(C, name='pf_auxiliar_m0')
       import :: c_double, c_float

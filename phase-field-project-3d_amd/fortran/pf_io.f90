@@ -16,11 +16,11 @@ contains
     integer :: u, ios
     integer :: opt_precision, opt_n_seeds
     integer(c_int64_t) :: opt_noise_seed, opt_seed_rng
-    logical :: opt_fused, opt_snapshots, have
+    logical :: opt_fused, opt_snapshots, opt_w0e0_trace, have
     integer :: opt_it_stop
     double precision :: opt_D
     namelist /pf3d/ opt_precision, opt_noise_seed, opt_n_seeds, opt_seed_rng, opt_fused, &
-                    opt_snapshots, opt_it_stop, opt_D
+                    opt_snapshots, opt_it_stop, opt_D, opt_w0e0_trace
 
     open(newunit=u, file='entrada.dat', status='old', action='read', iostat=ios)
     if (ios /= 0) then
@@ -64,7 +64,7 @@ contains
     ! options of this build: pf3d.nml next to the deck (optional)
     opt_precision = precision; opt_noise_seed = noise_seed; opt_n_seeds = n_seeds
     opt_seed_rng = seed_rng; opt_fused = fused; opt_snapshots = write_snapshots
-    opt_it_stop = it_stop; opt_D = D_thermal
+    opt_it_stop = it_stop; opt_D = D_thermal; opt_w0e0_trace = w0e0_trace
     inquire(file='pf3d.nml', exist=have)
     if (have) then
       open(newunit=u, file='pf3d.nml', status='old', action='read')
@@ -77,23 +77,23 @@ contains
     end if
     precision = opt_precision; noise_seed = opt_noise_seed; n_seeds = opt_n_seeds
     seed_rng = opt_seed_rng; fused = opt_fused; write_snapshots = opt_snapshots
-    it_stop = opt_it_stop; D_thermal = opt_D
+    it_stop = opt_it_stop; D_thermal = opt_D; w0e0_trace = opt_w0e0_trace
 
-    write(*, '(/,A,/)') '### Dados de Entrada ###'
-    write(*, '(A,I5,A,I5,A,I5)') 'Nx x Ny x Nz = ', Nx, ' x ', Ny, ' x ', Nz
-    write(*, '(A,F14.2,A,F14.2,A)') 'Tm = ', Tm, ' K   T0 = ', T0, ' K'
-    write(*, '(A,F14.2,A,F10.6,A,F10.2,A)') 'TP = ', TP_deck, ' K  (Cl_eq = ', &
-          (Tm - TP_deck) / Me, ', TL(C0) = ', Tm - Me * C0, ' K)'
-    write(*, '(A,I0,A,I0,A,I0,A,I0)') 'tipo_inicial = ', tipo_inicial, '  n_varreduras = ', &
-          n_varreduras, '  h_quintico = ', h_quintico, '  ruido_tipo = ', ruido_tipo
-    write(*, '(A,I0,A,L1,A,I0)') 'precision = ', precision, '  fused = ', fused, &
-          '  noise seed = ', noise_seed
+    if (taxa_resfr > 0.d0) TP_deck = T0  ! entrada.f90:83: cooling starts at T0
+    call echo_deck()
 
     ! derived scalars (entrada.f90:159-185)
     Lambda = Thin_Interface * dx
     DELTA_T = pf_time_step(dx, D_liq)
     AAA = pf_noise_scale(AAA_deck, DELTA_T, dx, D_liq, int(ruido_tipo, c_int32_t))
-    write(*, '(A,F10.5)') 'AAA efetivo = ', AAA
+    if (ruido_tipo == 0) then
+      write(*, "('AAA efetivo (escalado com dt) = ',F10.5)") AAA
+    else
+      write(*, "('AAA efetivo (escalado com sqrt(dt)) = ',F10.5)") AAA
+    end if
+    write(*, "('h(phi) quintico = ',I0,'   tipo de ruido = ',I0)") h_quintico, ruido_tipo
+    write(*, '(A,I0,A,L1,A,I0,A,L1)') 'pf3d: precision = ', precision, '  fused = ', fused, &
+          '  noise seed = ', noise_seed, '  W0_E0 trace = ', w0e0_trace
 
     prm%nx = Nx; prm%ny = Ny; prm%nz = Nz; prm%it = 0
     prm%n_varreduras = n_varreduras; prm%h_quintico = h_quintico; prm%ruido_tipo = ruido_tipo
@@ -101,7 +101,6 @@ contains
     prm%dx = dx; prm%dy = dy; prm%dz = dz; prm%dt = DELTA_T; prm%D = D_thermal
     prm%delta_H = delta_H; prm%cp_s = Cp_s
     prm%TP = TP_deck
-    if (taxa_resfr > 0.d0) prm%TP = T0
     prm%Tm = Tm; prm%Me = Me; prm%Vm = Vm; prm%R = R
     prm%M0 = M0_deck; prm%delta_eps = DELTA_EPSLON; prm%AAA = AAA
     prm%D_liq = D_liq; prm%D_sol = D_sol; prm%Ke = Ke
@@ -118,6 +117,75 @@ contains
                   conc_soluto_old, conc_soluto_new, TOLD, TNEW, DPP), 'initial fields')
     if (tipo_inicial == 4) call ler_restart()
   end subroutine entrada
+
+  ! The input echo of entrada.f90:92-148: the deck's values as read (E0,
+  ! w0, DELTA_t, AAA from the deck, before W0_E0 and the rescales), in the
+  ! reference's formats, so that logs compare line by line.  Modes 5 and 6
+  ! (bottom seeds) are this build's and have lines of their own.
+  subroutine echo_deck()
+    write(*, "(/,'### Dados de Entrada ###',/)")
+    write(*, "('Tm = ',F14.2,' K')") Tm
+    write(*, "('T0 = ',F14.2,' K')") T0
+    write(*, "('delta_H = ',E14.6,' J/m^3')") delta_H
+    write(*, "('E0 = ',E14.6,' -')") E0_deck
+    write(*, "('w0 = ',E14.6,' -')") W0_deck
+    write(*, "('M0 = ',E14.6,' -')") M0_deck
+    write(*, "('D_s = ',E14.6,' m^2/s')") D_s
+    write(*, "('D_l = ',E14.6,' m^2/s')") D_l
+    write(*, "('Cp_s = ',E14.6,' J/kg.K')") Cp_s
+    write(*, "('Cp_l = ',E14.6,' J/kg.K')") Cp_l
+    write(*, "('Nx = ',I5,' -')") Nx
+    write(*, "('Ny = ',I5,' -')") Ny
+    write(*, "('Nz = ',I5,' -')") Nz
+    write(*, "('dy = ',E14.6,' m')") dy
+    write(*, "('dx = ',E14.6,' m')") dx
+    write(*, "('dz = ',E14.6,' m')") dz
+    write(*, "('small = ',E14.6,' -')") small
+    write(*, "('ANI = ',E14.6,' -')") ANI
+    write(*, "('it_max = ',I10,' -')") it_max
+    write(*, "('ANI = ',E14.6,' -')") ANI
+    write(*, "('AAA = ',F14.4,' -')") AAA_deck
+    write(*, "('Theta0_inic = ',F14.4,' -')") Teta0
+    write(*, "('DELTA_EPSLON = ',F14.4,' -')") DELTA_EPSLON
+    write(*, "('DELTA_t = ',E14.4,' s')") DELTA_T_deck
+    write(*, "('D_sol = ',E14.4,' m^2/s')") D_sol
+    write(*, "('D_liq = ',E14.4,' m^2/s')") D_liq
+    write(*, "('Ke = ',F14.4,' -')") Ke
+    write(*, "('Me = ',F14.4,' K/mol')") Me
+    write(*, "('Mv = ',E14.4,' m^3/mol')") Vm
+    write(*, "('R = ',F14.5,' J/mol K')") R
+    write(*, "('C0 = ',F14.5,' molar fraction')") C0
+    write(*, "('Thin_Interface = ',F14.3,' molar fraction')") Thin_Interface
+    write(*, "('Sigma = ',F14.3,' J/m^2')") Sigma
+    write(*, "('ajuste',F14.3,' -')") ajuste
+    write(*, "('stoptime',F14.3,' s')") stoptime
+    write(*, "('WriteFileD_timeCounter',I8,' -')") WriteFileD_timeCounter
+    write(*, "('TP = ',F14.2,' K  (Cl_eq = ',F10.6,', TL(C0) = ',F10.2,' K)')") &
+          TP_deck, (Tm - TP_deck) / Me, Tm - Me * C0
+    if (taxa_resfr > 0.d0) write(*, "('Resfriamento continuo: TP = T0 - ',F8.3," // &
+                                 "' K/s * t  (T0 = ',F8.2,' K)')") taxa_resfr, T0
+    write(*, "('Varreduras da fase por passo = ',I0)") n_varreduras
+    select case (tipo_inicial)
+    case (4)
+      write(*, "('Condicao inicial: continuacao a partir de restart.dat')")
+    case (3)
+      write(*, "('Condicao inicial: frente plana na base (k<=4), crescimento em +z')")
+    case (2)
+      write(*, "('Condicao inicial: 1/8 de semente no canto (3,3,3), raio = ',F6.1,' celulas')") &
+            R_semente
+    case (1)
+      write(*, "('Condicao inicial: semente esferica central, raio = ',F6.1,' celulas')") R_semente
+    case (5)
+      write(*, "('Condicao inicial: semente esferica no centro da base, raio = ',F6.1," // &
+               "' celulas')") R_semente
+    case (6)
+      write(*, "('Condicao inicial: ',I0,' sementes na base, raio = ',F6.1,' celulas')") &
+            n_seeds, R_semente
+    case default
+      write(*, "('Condicao inicial: frente plana (j<=4)')")
+    end select
+    write(*, "(/,'### Fim Leitura de Entrada de Dados ###',/)")
+  end subroutine echo_deck
 
   ! tipo_inicial = 4: continue from restart.dat, a Tecplot POINT snapshot of
   ! this grid (4 header records, then one record "x y z phi T c" per cell, x
@@ -154,8 +222,8 @@ contains
     end do
     conc_soluto_new = conc_soluto_old
     deallocate(phi_f, c_f)
-    write(*, '(A,I0,A,I0,A,I0,A,I0,A)') 'restart.dat read (', Nx, ' x ', Ny, ' x ', Nz, &
-          '), shift = ', nshift, ' layers'
+    write(*, "('Condicao inicial: restart.dat lido (',I0,' x ',I0,' x ',I0," // &
+             "'), deslocamento = ',I0,' camadas')") Nx, Ny, Nz, nshift
   end subroutine ler_restart
 
   subroutine restart_fail(msg)

@@ -49,13 +49,26 @@ subroutine diagnosticos(L2, pmax, pmin, sc, fe)
 end subroutine diagnosticos
 
 subroutine W0_E0()
-  ! The search itself runs in the library (bit-exact); only its final line
-  ! is printed (the reference also prints every one of its ~5,500 tries).
+  ! The search runs in the library (bit-exact); with w0e0_trace (the
+  ! default) every try's |E0 - try| is printed as W0_E0.f90:19 does, then
+  ! the final line of W0_E0.f90:30.
   use pf_state
   implicit none
-  integer(c_int32_t) :: idx
+  integer(c_int32_t) :: idx, n, q
   real(c_double) :: w, e
+  real(c_double), allocatable :: diff(:)
   idx = pf_w0_e0(Lambda, ajuste, Sigma, w, e)
+  if (w0e0_trace) then
+    n = idx + 1
+    if (idx < 0) n = 10000001
+    allocate(diff(n))
+    n = pf_w0_e0_tries(Lambda, ajuste, Sigma, diff, n)
+    if (n < 0) n = size(diff)
+    do q = 1, n
+      write(*, *) "E0-EPSLON_Try==>", diff(q)
+    end do
+    deallocate(diff)
+  end if
   prm%W0 = w
   prm%E0 = e
   write(*, *) 'W0==>', prm%W0, '     ', 'E0==>', prm%E0, '     ', 'Dt==>', DELTA_T

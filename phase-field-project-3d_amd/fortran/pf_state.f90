@@ -24,6 +24,7 @@ module pf_state
   logical :: write_snapshots = .true.
   integer :: it_stop = 0            ! > 0: stop after this many steps
   double precision :: D_thermal = 0.d0  ! the reference's D (never assigned there: 0)
+  logical :: w0e0_trace = .true.    ! print every W0_E0 try, as W0_E0.f90:19 does
 
   ! derived scalars and loop state
   double precision :: DELTA_T, Lambda, AAA

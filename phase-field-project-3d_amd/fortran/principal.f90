@@ -127,9 +127,11 @@ contains
     call pf_check(pf_download(ctx, phase_field_new, conc_soluto_new, TNEW), 'pf_download')
   end subroutine download_fields
 
+  ! The event banner of principal.f90:166-175 (formats 2000-2050).
   subroutine event_header()
     write(*, '(A)') '====================================================='
     write(*, '(A)') '=============  ESCREVENDO RESULTADOS  ==============='
+    write(*, '(A)') '====================================================='
     write(*, '(A)') '====================================================='
     write(*, '(A,I8,A)') 'IT==> ', it, ' iteracoes'
     write(*, '(A,d12.4,A)') 'Tempo total simulado ==> ', lp%time, ' s'
@@ -139,38 +141,70 @@ contains
     write(*, '(A)') '====================================================='
   end subroutine event_header
 
-  ! Wall-clock block for the console and performance_log.txt, plus the lattice
-  ! update rate (MLUPS) of the interval.
+  ! The wall-clock block of principal.f90:308-408, in its formats: the log
+  ! file gets the event banner and the diagnostics line, then the console and
+  ! the log get the timing block.  Progress is time/stoptime, or it/it_max
+  ! when stoptime is 0.  The host side of this driver is one thread.
   subroutine print_timing()
     integer(8) :: clk
-    double precision :: el, dl, avg, mlups
-    integer :: u, ku, units(2), nunits, ios
+    double precision :: el, dl, avg_step, avg_output, pct, eta
+    integer :: u, ku, units(2), nunits, ios, ndev32
+    integer(c_int32_t) :: ndev, devid
     call system_clock(clk)
     el = dble(clk - clk0) / dble(clk_rate)
     dl = dble(clk - clk_prev) / dble(clk_rate)
     n_outputs = n_outputs + 1
-    avg = el / dble(max(it, 1))
-    mlups = 0.d0
-    if (dl > 0.d0) mlups = dble(Nx) * dble(Ny) * dble(Nz) * dble(it - it_prev_output) / dl / 1.d6
+    avg_step = el / dble(max(it, 1))
+    avg_output = el / dble(max(n_outputs, 1))
+    if (stoptime > 0.d0) then
+      pct = 100.d0 * dble(lp%time) / stoptime
+    else
+      pct = 100.d0 * dble(it) / dble(max(it_max, 1))
+    end if
+    eta = 0.d0
+    if (pct > 0.d0) eta = el * (100.d0 - pct) / pct
+    ndev = 0
+    if (pf_device_count(ndev) /= 0) ndev = 0
+    ndev32 = ndev
+    devid = 0
+    if (pf_ctx_device(ctx, devid) /= 0) devid = -1
     open(iu_log, file=perf_log_file, status='old', position='append', action='write', iostat=ios)
     nunits = 1; units(1) = 6
     if (ios == 0) then
       nunits = 2; units(2) = iu_log
+      write(iu_log, '(A)') '====================================================='
+      write(iu_log, '(A)') '=============  ESCREVENDO RESULTADOS  ==============='
+      write(iu_log, '(A)') '====================================================='
+      write(iu_log, '(A)') '====================================================='
       write(iu_log, '(A,I8,A)') 'IT==> ', it, ' iteracoes'
+      write(iu_log, '(A,D12.4,A)') 'Tempo total simulado ==> ', lp%time, ' s'
+      write(iu_log, '(A,D12.4,A)') 'Tempo de parada (stoptime) ==> ', stoptime, ' s'
+      write(iu_log, '(A,D12.4,A)') 'Intervalo de escrita em arquivo  ==> ', &
+            WriteFileD_timeCounter * DELTA_T, ' s'
+      write(iu_log, '(A)') '====================================================='
       write(iu_log, fmt_diag) L2_phi, phi_max, phi_min, solute_total, free_energy
     end if
     do ku = 1, nunits
       u = units(ku)
       write(u, '(A)') '--------------------------------------------------------'
       write(u, '(A,I0)') 'Output Number          : ', n_outputs
-      write(u, '(A,F0.6)') 'Simulation Time        : ', lp%time
+      write(u, '(A,F0.4)') 'Simulation Time        : ', lp%time
       write(u, '(A,I0)') 'Iteration              : ', it
+      write(u, '(A)') ''
       write(u, '(A)') 'Execution Device       : GPU (HIP, gfx950)'
+      write(u, '(A,I0)') 'GPU Device ID          : ', devid
+      write(u, '(A,I0)') 'Available Devices      : ', ndev32
+      write(u, '(A,I0)') 'OpenMP Threads         : ', 1
+      write(u, '(A)') ''
       write(u, '(A,F0.2,A)') 'Elapsed Time           : ', el, ' s'
       write(u, '(A,F0.2,A)') 'Time Since Last Output : ', dl, ' s'
-      write(u, '(A,F0.6,A)') 'Average Time/Step      : ', avg, ' s'
+      write(u, '(A)') ''
+      write(u, '(A,F0.5,A)') 'Average Time/Step      : ', avg_step, ' s'
+      write(u, '(A,F0.2,A)') 'Average Time/Output    : ', avg_output, ' s'
       write(u, '(A,I0)') 'Steps This Interval    : ', it - it_prev_output
-      write(u, '(A,F0.1)') 'MLUPS This Interval    : ', mlups
+      write(u, '(A,I0)') 'Total Outputs Written  : ', n_outputs
+      write(u, '(A,F0.2,A)') 'Completed              : ', pct, ' %'
+      write(u, '(A,F0.1,A)') 'Estimated Remaining    : ', eta, ' s'
       write(u, '(A)') '--------------------------------------------------------'
     end do
     if (ios == 0) close(iu_log)

@@ -66,8 +66,8 @@ _lib = None
 
 # Every symbol declared in include/pf_capi.h (tests check the library exports them).
 EXPORTS = (
-    "pf_version", "pf_last_error", "pf_device_count", "pf_pseudo_random", "pf_ruido_suave",
-    "pf_w0_e0", "pf_auxiliar_m0", "pf_equilibrium", "pf_time_step", "pf_noise_scale",
+    "pf_version", "pf_last_error", "pf_device_count", "pf_ctx_device", "pf_pseudo_random", "pf_ruido_suave",
+    "pf_w0_e0", "pf_w0_e0_tries", "pf_auxiliar_m0", "pf_equilibrium", "pf_time_step", "pf_noise_scale",
     "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
@@ -100,9 +100,11 @@ def load(path: str = LIB_PATH):
         "pf_version": ([], C.c_char_p),
         "pf_last_error": ([], C.c_char_p),
         "pf_device_count": ([C.POINTER(i32)], C.c_int),
+        "pf_ctx_device": ([C.c_void_p, C.POINTER(i32)], C.c_int),
         "pf_pseudo_random": ([i32, i32, i32, i32, i32, u64], f64),
         "pf_ruido_suave": ([i32, i32, i32, i32, u64], f64),
         "pf_w0_e0": ([f64, f32, f32, C.POINTER(f64), C.POINTER(f64)], i32),
+        "pf_w0_e0_tries": ([f64, f32, f32, C.POINTER(f64), i32], i32),
         "pf_auxiliar_m0": ([f64, f64, f64, f64, f64, f64, f64, f32, f32, f32, f64, f64, f32], f64),
         "pf_equilibrium": ([f64, f64, f32, f64, C.POINTER(f64), C.POINTER(f64)], None),
         "pf_time_step": ([f64, f64], f64),
@@ -192,6 +194,16 @@ def w0_e0(lam, ajuste, sigma, lib=None):
     W0, E0 = C.c_double(), C.c_double()
     idx = lib.pf_w0_e0(lam, ajuste, sigma, C.byref(W0), C.byref(E0))
     return W0.value, E0.value, idx
+
+
+def w0_e0_tries(lam, ajuste, sigma, lib=None):
+    """|E0 - try| of every try of the W0/E0 search, up to the accepted one."""
+    lib = lib or load()
+    _, _, idx = w0_e0(lam, ajuste, sigma, lib)
+    n = idx + 1 if idx >= 0 else 10000001
+    out = np.zeros(n)
+    got = lib.pf_w0_e0_tries(lam, ajuste, sigma, out.ctypes.data_as(C.POINTER(C.c_double)), n)
+    return out[:got] if got > 0 else out
 
 
 def equilibrium(TP, Tm, Ke, Me, lib=None):

@@ -11,8 +11,10 @@ script drives both on small decks and stores inputs and outputs as .npz:
   unit_*.npz         one call of temperatura / equacao_campo_fase /
                      concentracao_soluto / diagnosticos on random fields
   run_*.npz          the time loop (principal.f90:93-226) for a few steps
-  principal_*.npz    the unmodified principal.f90's Tecplot output events
-                     (principal_restart: continued from a restart.dat)
+  principal_*.npz    the unmodified principal.f90's Tecplot output events,
+                     console transcript and performance_log.txt
+                     (principal_restart: continued from a restart.dat;
+                     principal_stop: ended by stoptime)
 
 Run here (needs /root/reference and flang):  python tests/golden/make_golden.py
 The fixtures are data only; no reference source is stored.
@@ -245,7 +247,19 @@ PRINCIPAL_DECKS = {
     # the run6 deck's own settings (planar base front, cubic h, positive noise)
     "planar": dict(Nx=14, Ny=12, Nz=18, tipo_inicial=3, R_semente=0.0, h_quintico=0,
                    ruido_tipo=0, AAA=0.03, taxa_resfr=19.84, n_varreduras=2, TP=905.0),
+    # the original front (tipo_inicial = 0: phi = 1 for j <= 4), quintic h,
+    # smooth zero-mean noise, fixed TP, one sweep
+    "front_j": dict(Nx=12, Ny=14, Nz=10, tipo_inicial=0, R_semente=0.0, h_quintico=1,
+                    ruido_tipo=2, AAA=0.03, taxa_resfr=0.0, n_varreduras=1, TP=905.0),
 }
+
+# E0-EPSLON_Try lines (W0_E0.f90:19, one per try, ~5,500 of them) kept in the
+# fixtures: the first and last five and every 100th.
+W0E0_TRY = "E0-EPSLON_Try==>"
+
+
+def _try_sample(n):
+    return sorted(set(range(min(5, n))) | set(range(0, n, 100)) | set(range(max(0, n - 5), n)))
 
 
 def _run_principal(d, files=None):
@@ -276,6 +290,18 @@ def _run_principal(d, files=None):
             arrays[f"e{q}_header"] = np.array(lines[:4])
             arrays[f"e{q}_xyz_phi_tp_c"] = np.array(
                 [[float(x) for x in ln.split()] for ln in lines[4:] if ln.strip()])
+        # the console transcript and performance_log.txt, for the line-by-line
+        # comparison (the try lines apart: their count and a sample)
+        out = r.stdout.splitlines()
+        tries = [ln for ln in out if ln.lstrip().startswith(W0E0_TRY)]
+        idx = _try_sample(len(tries))
+        arrays["stdout_lines"] = np.array([ln for ln in out
+                                           if not ln.lstrip().startswith(W0E0_TRY)])
+        arrays["w0e0_tries"] = np.array(len(tries))
+        arrays["w0e0_sample_index"] = np.array(idx, dtype=np.int64)
+        arrays["w0e0_sample_lines"] = np.array([tries[q] for q in idx])
+        with open(os.path.join(wd, "performance_log.txt")) as fh:
+            arrays["perf_log_lines"] = np.array(fh.read().splitlines())
         diag = [ln for ln in r.stdout.splitlines() if ln.startswith("L2(phi)")]
         arrays["diag"] = np.array([[fortran_float(t) for t in ln.replace("==>", " ").split()[1::2]]
                                    for ln in diag])
@@ -289,10 +315,24 @@ def _run_principal(d, files=None):
 
 def make_principal(name="sphere"):
     """The unmodified principal.f90 on a small deck: its output events
-    write Tecplot POINT files with 8 significant digits."""
+    (it = 5, 9) write Tecplot POINT files with 8 significant digits."""
     d = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **PRINCIPAL_DECKS[name])
     arrays, _ = _run_principal(d)
+    arrays["event_its"] = np.array([5, 9])
     save(f"principal_{name}.npz", **arrays)
+
+
+def make_principal_stop():
+    """The planar deck stopped by stoptime (principal.f90:231-255): 6.5 time
+    steps, so the run ends at it = 7 with the termination banner, a final
+    snapshot (E_R < N) and the final timing block; one regular event at 5."""
+    base = dict(PRINCIPAL_DECKS["planar"])
+    d = Deck(stoptime=0.0, it_max=50, WriteFileD_timeCounter=4, **base)
+    dt = d.dx * d.dx / (7.5 * d.D_liq)
+    d.stoptime = 6.5 * dt
+    arrays, _ = _run_principal(d)
+    arrays["event_its"] = np.array([5, 7])
+    save("principal_stop.npz", **arrays)
 
 
 def make_restart():
@@ -306,19 +346,23 @@ def make_restart():
     d1 = Deck(stoptime=0.0, it_max=9, WriteFileD_timeCounter=4, **base)
     arrays, _ = _run_principal(d1, {"restart.dat": texts[1]})
     arrays["restart_dat"] = np.array(texts[1])
+    arrays["event_its"] = np.array([5, 9])
     save("principal_restart.npz", **arrays)
 
 
-def main():
+def main(only=None):
+    """All fixtures, or only the principal_* ones (`make_golden.py principal`)."""
     build_ref()
-    make_random_kat()
-    make_scalars_kat()
-    make_unit()
-    make_runs()
+    if only != "principal":
+        make_random_kat()
+        make_scalars_kat()
+        make_unit()
+        make_runs()
     for name in PRINCIPAL_DECKS:
         make_principal(name)
+    make_principal_stop()
     make_restart()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -56,6 +56,23 @@ def test_scalars_bit_exact_with_reference(lib):
         assert M0 == r["M0"]
 
 
+@pytest.mark.parametrize("name", ["principal_planar.npz", "principal_front_j.npz"])
+def test_w0_e0_try_trace_matches_reference_transcript(lib, name):
+    """W0_E0.f90:19 prints |E0 - EPSLON_Try| for every try: the unmodified
+    principal.f90's count of such lines and a sample of them (list-directed,
+    shortest round-trip digits, so the parse is exact) against
+    pf_w0_e0_tries."""
+    z = load_golden(name)
+    d = deck_from_json(z["deck"]).resolved()
+    diff = capi.w0_e0_tries(d.Thin_Interface * d.dx, d.ajuste, d.Sigma, lib)
+    assert len(diff) == int(z["w0e0_tries"])
+    for q, ln in zip(z["w0e0_sample_index"], z["w0e0_sample_lines"]):
+        label, val = str(ln).split()
+        assert label == "E0-EPSLON_Try==>"
+        assert diff[int(q)] == float(val), (q, ln)
+    assert np.isinf(diff[0]) and diff[-1] < 1e-5 <= diff[-2]
+
+
 @pytest.mark.parametrize("tipo,R", [(0, 0.0), (1, 3.0), (2, 4.0), (3, 0.0), (5, 4.0), (6, 3.0)])
 def test_initial_fields_match_oracle(lib, oracle_lib, tipo, R):
     nx, ny, nz = 17, 15, 12

@@ -142,21 +142,23 @@ def test_time_loop_bitwise(oracle_lib, name):
         np.testing.assert_allclose(diag, ref, rtol=1e-14, equal_nan=True)
 
 
-@pytest.mark.parametrize("name", ["principal_sphere.npz", "principal_planar.npz"])
+@pytest.mark.parametrize("name", ["principal_sphere.npz", "principal_planar.npz",
+                                  "principal_front_j.npz", "principal_stop.npz"])
 def test_unmodified_principal_output_event(oracle_lib, name):
-    """The unmodified principal.f90: its output events (it = N+1, 2N+1) write
-    phi, TP and c with 8 significant digits; the oracle's loop must agree to
-    that precision (NaN included) -- this pins the loop order (TP lag, deck M0
-    at step 1, REAL(4) time) against the real driver, not only our harness."""
+    """The unmodified principal.f90: its output events (it = N+1, 2N+1, or
+    the stoptime step) write phi, TP and c with 8 significant digits; the
+    oracle's loop must agree to that precision (NaN included) -- this pins
+    the loop order (TP lag, deck M0 at step 1, REAL(4) time) against the real
+    driver, not only our harness.  principal_front_j starts from the
+    original front (tipo_inicial = 0), so it pins that
+    initial condition too (entrada.f90:237-260)."""
     O = oracle_lib
     z = load_golden(name)
     deck = deck_from_json(z["deck"])
-    d = deck.resolved()
     p, L = params_from_deck(deck, O)
     F = initial_from_deck(deck, O)
-    N = d.WriteFileD_timeCounter
     done = 0
-    for q, it_ev in enumerate((N + 1, 2 * N + 1)):
+    for q, it_ev in enumerate(int(x) for x in z["event_its"]):
         O.run_steps(p, L, it_ev - done, F)
         done = it_ev
         data = z[f"e{q}_xyz_phi_tp_c"]
@@ -169,3 +171,71 @@ def test_unmodified_principal_output_event(oracle_lib, name):
         assert fname.strip().startswith("saida_")
         assert fname.split("_time_")[0].split("saida_")[1].strip() == str(it_ev)
         assert abs(float(fname.split("_time_")[1].strip()[:-5]) - float(L.time)) < 1e-7
+
+
+def _thermal_case(O, nx=6, ny=5, nz=7, D=0.37):
+    deck = deck_from_json(load_golden("principal_planar.npz")["deck"])
+    deck.Nx, deck.Ny, deck.Nz = nx, ny, nz
+    p, _ = params_from_deck(deck, O)
+    p.dx, p.dy, p.dz = 1.0e-7, 1.7e-7, 2.9e-7  # distinct, so a swapped axis shows
+    p.D = D
+    F = O.initial_fields(nx, ny, nz, 0, 0.0, 900.0, 0.02, 0.14)
+    F.dpp[:] = 0.0  # no latent-heat source: only the D term and TP remain
+    return p, F
+
+
+def _temperatura(O, p, F, T):
+    G = F.copy()
+    G.T_old = np.ascontiguousarray(T.reshape(-1, order="F"))
+    O.temperatura(p, G)
+    return G.T_new.reshape((p.nx, p.ny, p.nz), order="F")
+
+
+def test_thermal_term_spike_closed_form(oracle_lib):
+    """temp_body.inc with D != 0 (the reference never sets D, so no fixture
+    holds this branch).  With T_old = TP everywhere but one cell m, where it
+    is TP + s, and no source: each face neighbour of m along x, y, z gets
+    TP + D dt s / dx^2, / dy^2, / dz^2; m itself gets TP (the stencil's centre
+    is TP, not T_old(m)); every other interior cell gets TP; the boundary
+    keeps T_old."""
+    O = oracle_lib
+    p, F = _thermal_case(O)
+    TP, s = p.TP, 3.0
+    T = np.full((p.nx, p.ny, p.nz), TP)
+    m = (2, 2, 3)
+    T[m] += s
+    got = _temperatura(O, p, F, T)
+    want = np.full_like(T, TP)
+    for ax, h in ((0, p.dx), (1, p.dy), (2, p.dz)):
+        for sg in (-1, 1):
+            q = list(m)
+            q[ax] += sg
+            want[tuple(q)] = TP + p.D * p.dt * s / h ** 2
+    inner = (slice(1, -1),) * 3
+    edge = np.ones_like(T, dtype=bool)
+    edge[inner] = False
+    np.testing.assert_allclose(got[inner], want[inner], rtol=1e-15, atol=0)
+    assert np.array_equal(got[edge], T[edge])
+
+
+def test_thermal_term_brute_force_superposition(oracle_lib):
+    """The D term is linear in T_old - TP: on a tiny grid, a random field's
+    update equals TP plus the sum, over every cell, of that cell's spike
+    response (one oracle call per cell) scaled by its deviation."""
+    O = oracle_lib
+    p, F = _thermal_case(O, nx=5, ny=4, nz=6)
+    TP = p.TP
+    rng = np.random.default_rng(5)
+    dev = rng.normal(0.0, 2.0, (p.nx, p.ny, p.nz))
+    got = _temperatura(O, p, F, TP + dev)
+    acc = np.zeros_like(dev)
+    for m in np.ndindex(dev.shape):
+        T = np.full_like(dev, TP)
+        T[m] += 1.0
+        acc += (_temperatura(O, p, F, T) - TP) * dev[m]
+    inner = (slice(1, -1),) * 3
+    # interior: superposition of the spike responses (boundary spikes act
+    # only through their interior neighbours)
+    resp = np.zeros_like(dev)
+    resp[inner] = acc[inner]
+    np.testing.assert_allclose(got[inner] - TP, resp[inner], rtol=1e-9, atol=1e-9 * TP)
