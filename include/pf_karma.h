@@ -1,9 +1,10 @@
 /* C-ABI of the quantitative Karma model (karma/karma3d.f90) on the MI355X.
  *
  * The reference is a stand-alone program with no subroutine interface: its
- * time loop (karma/karma3d.f90:146-296) and seed initialisation
- * (karma/karma3d.f90:131-144) are what these entry points replace, driven
- * by the parameters of its namelist (karma/karma3d.f90:71-86).  Fields are
+ * time loop (karma/karma3d.f90:159-312) and seed initialisation
+ * (karma/karma3d.f90:130-144) are what these entry points replace, driven
+ * by the parameters of its namelist (karma/karma3d.f90:75-89).  The
+ * program around them (namelist, tip, output files) is bin/karma3d.  Fields are
  * FP64 and stay resident in HBM between calls; status codes are those of
  * pf_capi.h (PF_OK, PF_E_ARG, PF_E_HIP, PF_E_NOMEM) with pf_last_error(). */
 #ifndef PF_KARMA_H
@@ -17,9 +18,9 @@ extern "C" {
 
 typedef struct pf_karma pf_karma;
 
-/* karma.nml (karma/karma3d.f90:71-86); the derived D = a2 lambda and
+/* karma.nml (karma/karma3d.f90:75-89); the derived D = a2 lambda and
  * Omega = (c_l0 - C0)/((1-k) c_l0), c_l0 = (Tm - TP)/mliq, are formed as
- * karma/karma3d.f90:111-114 forms them. */
+ * karma/karma3d.f90:110-112 forms them. */
 typedef struct {
   int32_t n;      /* cells per direction (>= 2) */
   int32_t pad_;
@@ -28,9 +29,9 @@ typedef struct {
 } pf_karma_params;
 
 /* Allocates the fields on the current device and sets the 1/8 seed at the
- * corner (karma/karma3d.f90:131-144); step counter 0. */
+ * corner (karma/karma3d.f90:130-144); step counter 0. */
 int pf_karma_create(const pf_karma_params *p, pf_karma **out);
-/* nsteps steps of karma/karma3d.f90:146-296: mirror ghosts, phase equation,
+/* nsteps steps of karma/karma3d.f90:159-291: mirror ghosts, phase equation,
  * solute equation with the anti-trapping current, clamp and advance.
  * Asynchronous on the model's stream. */
 int pf_karma_run(pf_karma *s, int32_t nsteps);

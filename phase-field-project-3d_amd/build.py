@@ -28,6 +28,7 @@ ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 HIP_SOURCES = ["pf_kernels_dropin.hip", "pf_fused.hip", "pf_batch.hip", "pf_runtime.hip", "pf_karma.hip",
                "pf_host.cpp"]
 FORTRAN_SOURCES = ["pf_capi.f90", "pf_state.f90", "pf_kernels.f90", "pf_io.f90", "principal.f90"]
+KARMA_SOURCES = ["pf_capi.f90", "karma_capi.f90", "karma3d.f90"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
@@ -77,20 +78,26 @@ def build_lib(verbose=False, extra=()):
 
 
 def build_fortran(verbose=False):
-    """The Fortran driver (principal + the drop-in subroutines), linked
+    """The Fortran programs -- bin/pf3d (principal + the drop-in
+    subroutines) and bin/karma3d (the Karma model's program) -- linked
     against libpfamd.so with an rpath to the in-tree lib/."""
-    srcs = [os.path.join(FSRC, f) for f in FORTRAN_SOURCES]
-    if not all(os.path.exists(s) for s in srcs) or not os.path.exists(FLANG):
+    if not os.path.exists(FLANG):
         return None
     os.makedirs(BIN_DIR, exist_ok=True)
-    mod = os.path.join(OBJ_DIR, "mod")
-    os.makedirs(mod, exist_ok=True)
-    exe = os.path.join(BIN_DIR, "pf3d")
-    _run([FLANG, "-O2", "-module-dir", mod, "-I", mod, *srcs, "-o", exe + ".tmp",
-          "-L", LIB_DIR, "-lpfamd", f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN/../lib"],
-         verbose)
-    os.replace(exe + ".tmp", exe)
-    return exe
+    exes = []
+    for name, files in (("pf3d", FORTRAN_SOURCES), ("karma3d", KARMA_SOURCES)):
+        srcs = [os.path.join(FSRC, f) for f in files]
+        if not all(os.path.exists(s) for s in srcs):
+            continue
+        mod = os.path.join(OBJ_DIR, "mod_" + name)
+        os.makedirs(mod, exist_ok=True)
+        exe = os.path.join(BIN_DIR, name)
+        _run([FLANG, "-O2", "-module-dir", mod, "-I", mod, *srcs, "-o", exe + ".tmp",
+              "-L", LIB_DIR, "-lpfamd", f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN/../lib"],
+             verbose)
+        os.replace(exe + ".tmp", exe)
+        exes.append(exe)
+    return exes[0] if exes else None
 
 
 def build(verbose=False, fortran=True):

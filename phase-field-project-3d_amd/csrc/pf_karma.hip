@@ -21,8 +21,8 @@
 namespace {
 
 constexpr int kBX = 32, kBY = 8;
-constexpr double kA1 = 0.8839, kA2 = 0.6267;          // karma3d.f90:86
-constexpr double kAt = 0.35355339059327373;          // 1/(2 sqrt 2), karma3d.f90:87
+constexpr double kA1 = 0.8839, kA2 = 0.6267;          // karma3d.f90:91
+constexpr double kAt = 0.35355339059327373;          // 1/(2 sqrt 2), karma3d.f90:92
 constexpr int kMaxN = 1200;                          // (n+2)^3 < 2^31: 32-bit indices
 
 struct KStep {
@@ -33,7 +33,7 @@ struct KStep {
 
 __device__ __forceinline__ int clampc(int v, int n) { return v < 1 ? 1 : (v > n ? n : v); }
 
-// espelho (karma3d.f90:313-333) for one or two fields: the x, y, z passes
+// espelho (karma3d.f90:318-339) for one or two fields: the x, y, z passes
 // in that order compose to ghost(i,j,k) = f(clamp(i), clamp(j), clamp(k)),
 // so every ghost cell reads interior cells only and the axes need no order
 // (cells on ghost edges are written with the same value by each axis).
@@ -71,7 +71,7 @@ __device__ __forceinline__ double fluxn(double g1, double g2, double g3, double 
 // Face gradient between cell c and its neighbour c + o (o = +-stride of the
 // face's axis): the normal difference and the two tangential centred
 // differences averaged over the face's cells, along e1 then e2 (the other
-// two axes in x, y, z order) (karma3d.f90:158-175, 200-260).
+// two axes in x, y, z order) (karma3d.f90:170-187, 215-268).
 __device__ __forceinline__ void face(const double *__restrict__ P, int c, int o, bool plus, int e1,
                                      int e2, double idx, double &gn, double &t1, double &t2) {
 #pragma clang fp contract(off)
@@ -95,8 +95,8 @@ __device__ __forceinline__ double ruido(int i, int j, int k, int it) {
   return 2.0 * (double)(h & m63) / (double)m63 - 1.0;
 }
 
-// Phase equation (karma3d.f90:153-194): dphi and the clamped phi of the next
-// step (karma3d.f90:272-279; the unclamped value is not read elsewhere).
+// Phase equation (karma3d.f90:164-207): dphi and the clamped phi of the next
+// step (karma3d.f90:280-289; the unclamped value is not read elsewhere).
 __global__ void __launch_bounds__(kBX *kBY)
     k_karma_phase(KStep g, const double *__restrict__ P, const double *__restrict__ U,
                   double *__restrict__ dphi, double *__restrict__ Pn) {
@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(kBX *kBY)
   Pn[c] = v < -1.0 ? -1.0 : (v > 1.0 ? 1.0 : v);
 }
 
-// One face's solute flux (karma3d.f90:200-260): diffusion D q (U_X - U_P)
+// One face's solute flux (karma3d.f90:215-268): diffusion D q (U_X - U_P)
 // plus the anti-trapping current a_t cf n.
 __device__ __forceinline__ double uflux(const double *__restrict__ P, const double *__restrict__ U,
                                         const double *__restrict__ dphi, int c, int o, bool plus,
@@ -166,7 +166,7 @@ __device__ __forceinline__ double uflux(const double *__restrict__ P, const doub
   return F;
 }
 
-// Solute equation (karma3d.f90:196-268): U of the next step.
+// Solute equation (karma3d.f90:211-278): U of the next step.
 __global__ void __launch_bounds__(kBX *kBY)
     k_karma_solute(KStep g, const double *__restrict__ P, const double *__restrict__ U,
                    const double *__restrict__ dphi, double *__restrict__ Un) {
@@ -238,7 +238,7 @@ extern "C" int pf_karma_create(const pf_karma_params *p, pf_karma **out) {
   s->p = *p;
   s->n = p->n; s->s = p->n + 2; s->ss = s->s * s->s;
   s->cells = (size_t)s->ss * s->s;
-  // karma3d.f90:111-113
+  // karma3d.f90:110-112
   s->D = kA2 * p->lambda;
   const double cl0 = (p->Tm - p->TP) / p->mliq;
   s->Omega = (cl0 - p->C0) / ((1.0 - p->k) * cl0);
@@ -258,7 +258,7 @@ extern "C" int pf_karma_create(const pf_karma_params *p, pf_karma **out) {
     karma_free(s);
     return pf::fail_msg(PF_E_NOMEM, "pf_karma_create: %zu bytes per field", bytes);
   }
-  // seed (karma3d.f90:131-144), ghosts included, on the host
+  // seed (karma3d.f90:130-144), ghosts included, on the host
   std::vector<double> ph(s->cells), u(s->cells, -s->Omega);
   const int n = s->n;
   for (int k = 0; k <= n + 1; ++k)

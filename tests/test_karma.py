@@ -4,10 +4,13 @@ CPU: the oracle (oracle/karma_oracle.c) against the reference program
 compiled from karma/karma3d.f90 (tests/golden/karma_n16.npz: its Tecplot
 output at steps 20 and 40 of a 16^3 run with noise, 8 significant digits),
 the mirror ghosts against the literal x, y, z passes, the noise against the
-main model's hash, and the library's exports.  GPU: the HIP step against the
-oracle."""
+main model's hash, the namelist reader and tip rule, and the library's
+exports.  GPU: the HIP step against the oracle, and the program bin/karma3d
+against the reference program's console transcript and files (the regular
+run and the tip_stop exit)."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -16,9 +19,10 @@ from conftest import ROOT
 
 import oracle as O
 from pfamd import capi
-from pfamd.karma import EXPORTS, Karma, KarmaDeck
+from pfamd.karma import EXPORTS, Karma, KarmaDeck, tips
 
 GOLD = os.path.join(ROOT, "tests", "golden", "karma_n16.npz")
+EXE = os.path.join(ROOT, "phase-field-project-3d_amd", "bin", "karma3d")
 
 
 def _nml():
@@ -128,3 +132,99 @@ def test_karma_gpu_matches_oracle(n, steps, Fnoise):
 def test_karma_gpu_errors():
     with pytest.raises(capi.PfError):
         Karma(KarmaDeck(N=1))
+
+
+def test_namelist_reader():
+    """KarmaDeck.from_nml reads the group as the Fortran namelist does:
+    entries in any order and case, comments, commas, d exponents; absent
+    entries keep the program's defaults; unknown names are errors."""
+    d = KarmaDeck.from_nml("! c\n&KARMA  n = 16, LAMBDA=3.5 ! x\n dl = 2.5d-9\n nout=7 /\n")
+    assert (d.N, d.lambda_, d.Dl, d.nout) == (16, 3.5, 2.5e-9, 7)
+    assert d.eps4 == KarmaDeck().eps4 and d.Fnoise == 0.0
+    assert KarmaDeck.from_nml(KarmaDeck.shipped().to_nml()) == KarmaDeck.shipped()
+    z = np.load(GOLD)
+    g = KarmaDeck.from_nml("&karma\n" + "\n".join(f"{k} = {v}" for k, v in z["nml"]) + "\n/")
+    assert (g.N, g.Fnoise, g.tip_stop, g.R0, g.nsteps, g.nout) == (16, 0.05, 2.0, 6.0, 40, 20)
+    with pytest.raises(ValueError):
+        KarmaDeck.from_nml("&karma\n bogus = 1\n/")
+    # the scales the program prints for the golden run
+    sc = g.scales()
+    assert f"{sc['D']:7.3f}".strip() == "2.507"
+    assert f"{sc['tau0']:10.3E}".strip() == "3.876E-07"
+
+
+def test_tip_rule():
+    n = 9
+    phi = -np.ones((n, n, n))
+    assert tips(phi) == (0, 0)
+    phi[0, 0, :5] = 0.5           # x axis: cells i = 1..5 solid
+    phi[0, 0, 7] = 1e-3           # an isolated positive cell further out counts
+    for q in range(3):
+        phi[q, q, q] = 1.0        # diagonal: i = 1..3
+    assert tips(phi) == (8, 3)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="program not built")
+def test_program_reports_missing_device(tmp_path):
+    from conftest import gpu_available
+    if gpu_available():
+        pytest.skip("a GPU is present")
+    (tmp_path / "karma.nml").write_text("&karma\n N = 8\n nsteps = 2\n/\n")
+    r = subprocess.run([EXE], cwd=str(tmp_path), stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode != 0 and "pf_karma_create failed" in r.stdout
+
+
+def _program(tmp_path, nml_items):
+    (tmp_path / "karma.nml").write_text(
+        "&karma\n" + "".join(f" {k} = {v}\n" for k, v in nml_items) + "/\n")
+    (tmp_path / "output").mkdir()
+    return subprocess.run([EXE], cwd=str(tmp_path), stdout=subprocess.PIPE,
+                          stderr=subprocess.STDOUT, text=True, timeout=300)
+
+
+_BANNERS = ("### CPU OpenMP", "### GPU offload ENABLED", "### gfx950 HIP device")
+
+
+def _console(lines):
+    """Transcript lines compared: the device banner dropped and the wall
+    clock field of the progress line cut off."""
+    return [str(ln).split("  wall=")[0] for ln in lines if not str(ln).startswith(_BANNERS)]
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(EXE), reason="program not built")
+def test_program_matches_reference_program(tmp_path):
+    """bin/karma3d on the golden's namelist: the console transcript line by
+    line (wall clock and device banner masked), the file names and header
+    records exactly, and the fields to the files' 8 significant digits."""
+    z = np.load(GOLD)
+    print("[karma program] 16^3, 40 steps", flush=True)
+    r = _program(tmp_path, z["nml"])
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert _console(r.stdout.splitlines()) == _console(z["stdout_lines"])
+    names = sorted(os.listdir(tmp_path / "output"))
+    assert names == [str(x) for x in z["filenames"]]
+    n = 16
+    for q, fn in enumerate(names):
+        it = int(fn[len("karma_"):-len(".dat")])
+        lines = (tmp_path / "output" / fn).read_text().splitlines()
+        assert lines[:4] == [str(x) for x in z["headers"][q]]
+        rows = np.array([[float(x) for x in ln.split()] for ln in lines[4:]])
+        assert rows.shape == (n ** 3, 6)
+        np.testing.assert_allclose(rows[:, 3].reshape(n, n, n), z[f"phase_{it}"], rtol=6e-8,
+                                   atol=1e-300)
+        np.testing.assert_allclose(rows[:, 5].reshape(n, n, n), z[f"conc_{it}"], rtol=6e-8)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(EXE), reason="program not built")
+def test_program_tip_stop_exit(tmp_path):
+    """tip_stop = 0.3: the x-axis tip (i = 8 > 0.3 * 16) ends the run at the
+    first output step, with the reference's message and one file."""
+    z = np.load(GOLD)
+    print("[karma program] tip_stop exit", flush=True)
+    r = _program(tmp_path, z["stop_nml"])
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert _console(r.stdout.splitlines()) == _console(z["stop_stdout_lines"])
+    assert sorted(os.listdir(tmp_path / "output")) == [str(x) for x in z["stop_filenames"]]
