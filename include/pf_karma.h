@@ -35,6 +35,11 @@ int pf_karma_create(const pf_karma_params *p, pf_karma **out);
  * solute equation with the anti-trapping current, clamp and advance.
  * Asynchronous on the model's stream. */
 int pf_karma_run(pf_karma *s, int32_t nsteps);
+/* Kernel form of the following steps: on (the default) the z-march kernels
+ * (a work-group keeps three planes of its tile in LDS and marches along z),
+ * off one thread per cell reading its neighbours through the caches.  Both
+ * give the same bits. */
+int pf_karma_set_zmarch(pf_karma *s, int32_t on);
 /* Synchronous: the cells 1..n of phi and U (n^3 doubles each, x fastest,
  * as the reference's arrays phi(1:N,1:N,1:N), U(1:N,1:N,1:N)); either may be
  * null.  *it (nullable) = steps taken. */

@@ -1,9 +1,9 @@
 // The quantitative Karma model (karma/karma3d.f90) on gfx950: phi and U in
 // HBM as padded (n+2)^3 FP64 cubes (cells 1..n, mirror ghosts 0 and n+1),
-// four launches per step on the model's own stream:
-//   ghosts of phi and U -> phase equation (dphi, clamped phi of the next
-//   step) -> ghosts of dphi -> solute equation with the anti-trapping current
-//   (U of the next step), then the two buffer pairs swap.
+// two launches per step on the model's own stream:
+//   phase equation (dphi and the clamped phi of the next step, ghosts
+//   included) -> solute equation with the anti-trapping current (U of the
+//   next step, ghosts included), then the two buffer pairs swap.
 // One thread per cell, x fastest (32x8 threads per work-group).  Every
 // expression keeps the reference's operation order with FP contraction off
 // (division and square root correctly rounded), so each cell rounds as a
@@ -31,29 +31,22 @@ struct KStep {
   int it;
 };
 
-__device__ __forceinline__ int clampc(int v, int n) { return v < 1 ? 1 : (v > n ? n : v); }
-
-// espelho (karma3d.f90:318-339) for one or two fields: the x, y, z passes
-// in that order compose to ghost(i,j,k) = f(clamp(i), clamp(j), clamp(k)),
-// so every ghost cell reads interior cells only and the axes need no order
-// (cells on ghost edges are written with the same value by each axis).
-__global__ void __launch_bounds__(kBX *kBY)
-    k_karma_mirror(KStep g, double *__restrict__ f1, double *__restrict__ f2) {
-  const int u = blockIdx.x * kBX + threadIdx.x, v = blockIdx.y * kBY + threadIdx.y;
-  const int n = g.n;
-  if (u > n + 1 || v > n + 1) return;
-  const int ax = blockIdx.z;
-  for (int side = 0; side < 2; ++side) {
-    const int w = side ? n + 1 : 0;
-    int i, j, k;
-    if (ax == 0) { i = w; j = u; k = v; }
-    else if (ax == 1) { i = u; j = w; k = v; }
-    else { i = u; j = v; k = w; }
-    const int dst = i + g.s * j + g.ss * k;
-    const int src = clampc(i, n) + g.s * clampc(j, n) + g.ss * clampc(k, n);
-    f1[dst] = f1[src];
-    if (f2) f2[dst] = f2[src];
-  }
+// espelho (karma3d.f90:318-339): its x, y, z passes compose to
+// ghost(i,j,k) = f(clamp(i), clamp(j), clamp(k)), so the thread that writes a
+// boundary cell's new value also writes it to every ghost position that
+// clamps to that cell (up to 7 for a corner).  The fields of the next step
+// therefore arrive with their ghosts set and no separate mirror pass runs;
+// no kernel reads the buffer it writes.
+__device__ __forceinline__ void put_ghosts(double *__restrict__ f, const KStep &g, int i, int j,
+                                           int k, int c, double v) {
+  const int di = i == 1 ? -1 : (i == g.n ? 1 : 0);
+  const int dj = j == 1 ? -g.s : (j == g.n ? g.s : 0);
+  const int dk = k == 1 ? -g.ss : (k == g.n ? g.ss : 0);
+  if ((di | dj | dk) == 0) return;
+  for (int a = 0; a <= (di != 0); ++a)
+    for (int b = 0; b <= (dj != 0); ++b)
+      for (int e = 0; e <= (dk != 0); ++e)
+        if (a + b + e) f[c + a * di + b * dj + e * dk] = v;
 }
 
 // fluxn (karma3d.f90:36-47)
@@ -66,19 +59,6 @@ __device__ __forceinline__ double fluxn(double g1, double g2, double g3, double 
   const double s4 = q1 * q1 + q2 * q2 + q3 * q3;
   const double as = a0 + 4.0 * eps4 * s4 / (g2s * g2s);
   return as * (as * g1 + 16.0 * eps4 * (q1 * g1 - g1 * s4 / g2s) / g2s);
-}
-
-// Face gradient between cell c and its neighbour c + o (o = +-stride of the
-// face's axis): the normal difference and the two tangential centred
-// differences averaged over the face's cells, along e1 then e2 (the other
-// two axes in x, y, z order) (karma3d.f90:170-187, 215-268).
-__device__ __forceinline__ void face(const double *__restrict__ P, int c, int o, bool plus, int e1,
-                                     int e2, double idx, double &gn, double &t1, double &t2) {
-#pragma clang fp contract(off)
-  const double pc = P[c], pn = P[c + o];
-  gn = plus ? (pn - pc) * idx : (pc - pn) * idx;
-  t1 = 0.25 * idx * (P[c + e1] + P[c + o + e1] - P[c - e1] - P[c + o - e1]);
-  t2 = 0.25 * idx * (P[c + e2] + P[c + o + e2] - P[c - e2] - P[c + o - e2]);
 }
 
 // ruido (karma3d.f90:50-64)
@@ -95,35 +75,70 @@ __device__ __forceinline__ double ruido(int i, int j, int k, int it) {
   return 2.0 * (double)(h & m63) / (double)m63 - 1.0;
 }
 
-// Phase equation (karma3d.f90:164-207): dphi and the clamped phi of the next
-// step (karma3d.f90:280-289; the unclamped value is not read elsewhere).
-__global__ void __launch_bounds__(kBX *kBY)
-    k_karma_phase(KStep g, const double *__restrict__ P, const double *__restrict__ U,
-                  double *__restrict__ dphi, double *__restrict__ Pn) {
+// A field around one cell: f(dx, dy, dz) is its value at the neighbour
+// (i+dx, j+dy, k+dz), |dx|, |dy|, |dz| <= 1 (the offsets are compile-time
+// constants once inlined).  GlobalNb reads the padded cube in HBM; PlaneNb
+// reads three LDS planes k-1, k, k+1 of a tile with a one-cell halo.
+struct GlobalNb {
+  const double *__restrict__ f;
+  int c, s, ss;
+  __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
+    return f[c + dx + s * dy + ss * dz];
+  }
+};
+
+constexpr int kTX = 32, kTY = 8;                      // z-march tile (cells)
+constexpr int kLX = kTX + 2, kLY = kTY + 2, kLP = kLX * kLY;  // with the halo ring
+
+struct PlaneNb {
+  const double *lo, *mid, *hi;  // planes k-1, k, k+1
+  int q;                        // the cell's index in a plane
+  __device__ __forceinline__ double operator()(int dx, int dy, int dz) const {
+    const double *p = dz < 0 ? lo : (dz > 0 ? hi : mid);
+    return p[q + dx + kLX * dy];
+  }
+};
+
+// Face gradient on the face between the cell and its neighbour along axis
+// AX (PLUS: +1, else -1): the normal difference and the two tangential
+// centred differences averaged over the face's two cells, along the other
+// two axes in x, y, z order (karma3d.f90:170-187, 215-268).
+template <int AX, bool PLUS, class F>
+__device__ __forceinline__ void face(const F &P, double idx, double &gn, double &t1, double &t2) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * kBX + threadIdx.x + 1, j = blockIdx.y * kBY + threadIdx.y + 1;
-  const int k = blockIdx.z + 1;
-  if (i > g.n || j > g.n) return;
-  const int sx = 1, sy = g.s, sz = g.ss;
-  const int c = i + sy * j + sz * k;
-  const double idx = g.idx, pp = P[c];
+  constexpr int sg = PLUS ? 1 : -1;
+  constexpr int A1 = AX == 0 ? 1 : 0, A2 = AX == 2 ? 1 : 2;
+  constexpr int nx = (AX == 0) * sg, ny = (AX == 1) * sg, nz = (AX == 2) * sg;
+  constexpr int x1 = A1 == 0, y1 = A1 == 1, z1 = A1 == 2;
+  constexpr int x2 = A2 == 0, y2 = A2 == 1, z2 = A2 == 2;
+  const double pc = P(0, 0, 0), pn = P(nx, ny, nz);
+  gn = PLUS ? (pn - pc) * idx : (pc - pn) * idx;
+  t1 = 0.25 * idx *
+       (P(x1, y1, z1) + P(nx + x1, ny + y1, nz + z1) - P(-x1, -y1, -z1) - P(nx - x1, ny - y1, nz - z1));
+  t2 = 0.25 * idx *
+       (P(x2, y2, z2) + P(nx + x2, ny + y2, nz + z2) - P(-x2, -y2, -z2) - P(nx - x2, ny - y2, nz - z2));
+}
+
+template <int AX, bool PLUS, class F>
+__device__ __forceinline__ double flux_phi(const F &P, const KStep &g) {
   double gn, t1, t2;
-  face(P, c, sx, true, sy, sz, idx, gn, t1, t2);
-  const double Jxp = fluxn(gn, t1, t2, g.eps4);
-  face(P, c, -sx, false, sy, sz, idx, gn, t1, t2);
-  const double Jxm = fluxn(gn, t1, t2, g.eps4);
-  face(P, c, sy, true, sx, sz, idx, gn, t1, t2);
-  const double Jyp = fluxn(gn, t1, t2, g.eps4);
-  face(P, c, -sy, false, sx, sz, idx, gn, t1, t2);
-  const double Jym = fluxn(gn, t1, t2, g.eps4);
-  face(P, c, sz, true, sx, sy, idx, gn, t1, t2);
-  const double Jzp = fluxn(gn, t1, t2, g.eps4);
-  face(P, c, -sz, false, sx, sy, idx, gn, t1, t2);
-  const double Jzm = fluxn(gn, t1, t2, g.eps4);
+  face<AX, PLUS>(P, g.idx, gn, t1, t2);
+  return fluxn(gn, t1, t2, g.eps4);
+}
+
+// Phase equation at one cell (karma3d.f90:164-207): dphi.
+template <class F>
+__device__ __forceinline__ double phase_cell(const F &P, double uc, int i, int j, int k,
+                                             const KStep &g) {
+#pragma clang fp contract(off)
+  const double idx = g.idx, pp = P(0, 0, 0);
+  const double Jxp = flux_phi<0, true>(P, g), Jxm = flux_phi<0, false>(P, g);
+  const double Jyp = flux_phi<1, true>(P, g), Jym = flux_phi<1, false>(P, g);
+  const double Jzp = flux_phi<2, true>(P, g), Jzm = flux_phi<2, false>(P, g);
   const double divJ = (Jxp - Jxm + Jyp - Jym + Jzp - Jzm) * idx;
-  const double g1 = 0.5 * idx * (P[c + sx] - P[c - sx]);
-  const double g2 = 0.5 * idx * (P[c + sy] - P[c - sy]);
-  const double g3 = 0.5 * idx * (P[c + sz] - P[c - sz]);
+  const double g1 = 0.5 * idx * (P(1, 0, 0) - P(-1, 0, 0));
+  const double g2 = 0.5 * idx * (P(0, 1, 0) - P(0, -1, 0));
+  const double g3 = 0.5 * idx * (P(0, 0, 1) - P(0, 0, -1));
   const double gq = g1 * g1 + g2 * g2 + g3 * g3;
   const double a0 = 1.0 - 3.0 * g.eps4;
   double as2;
@@ -136,58 +151,164 @@ __global__ void __launch_bounds__(kBX *kBY)
     as2 = a0 * a0;
   }
   const double om = 1.0 - pp * pp;
-  double dp = (divJ + pp - pp * pp * pp - g.lambda * (om * om) * U[c]) / as2;
+  double dp = (divJ + pp - pp * pp * pp - g.lambda * (om * om) * uc) / as2;
   if (g.Fnoise != 0.0) dp += g.Fnoise * ruido(i, j, k, g.it) * om / g.sqdt;
-  dphi[c] = dp;
-  const double v = pp + g.dt * dp;
-  Pn[c] = v < -1.0 ? -1.0 : (v > 1.0 ? 1.0 : v);
+  return dp;
 }
 
 // One face's solute flux (karma3d.f90:215-268): diffusion D q (U_X - U_P)
 // plus the anti-trapping current a_t cf n.
-__device__ __forceinline__ double uflux(const double *__restrict__ P, const double *__restrict__ U,
-                                        const double *__restrict__ dphi, int c, int o, bool plus,
-                                        int ax, int e1, int e2, double pp, double uc, double cc,
-                                        const KStep &g) {
+template <int AX, bool PLUS, class F>
+__device__ __forceinline__ double uflux(const F &P, const F &U, const F &Dp, double pp, double uc,
+                                        double cc, const KStep &g) {
 #pragma clang fp contract(off)
+  constexpr int sg = PLUS ? 1 : -1;
+  constexpr int nx = (AX == 0) * sg, ny = (AX == 1) * sg, nz = (AX == 2) * sg;
   double gn, t1, t2;
-  face(P, c, o, plus, e1, e2, g.idx, gn, t1, t2);
+  face<AX, PLUS>(P, g.idx, gn, t1, t2);
   // |grad phi| over x, y, z in that order
   double gx, gy, gz;
-  if (ax == 0) { gx = gn; gy = t1; gz = t2; }
-  else if (ax == 1) { gx = t1; gy = gn; gz = t2; }
+  if (AX == 0) { gx = gn; gy = t1; gz = t2; }
+  else if (AX == 1) { gx = t1; gy = gn; gz = t2; }
   else { gx = t1; gy = t2; gz = gn; }
   const double gm = sqrt(gx * gx + gy * gy + gz * gz);
-  const int nb = c + o;
-  const double qf = 0.25 * (2.0 - pp - P[nb]);
-  const double cf = 0.5 * (cc + (1.0 + g.k1 * U[nb]) * dphi[nb]);
-  double F = plus ? g.D * qf * (U[nb] - uc) * g.idx : g.D * qf * (uc - U[nb]) * g.idx;
-  if (gm > 1e-10) F = F + kAt * cf * gn / gm;
-  return F;
+  const double un = U(nx, ny, nz);
+  const double qf = 0.25 * (2.0 - pp - P(nx, ny, nz));
+  const double cf = 0.5 * (cc + (1.0 + g.k1 * un) * Dp(nx, ny, nz));
+  double Fl = PLUS ? g.D * qf * (un - uc) * g.idx : g.D * qf * (uc - un) * g.idx;
+  if (gm > 1e-10) Fl = Fl + kAt * cf * gn / gm;
+  return Fl;
 }
 
-// Solute equation (karma3d.f90:211-278): U of the next step.
-__global__ void __launch_bounds__(kBX *kBY)
-    k_karma_solute(KStep g, const double *__restrict__ P, const double *__restrict__ U,
-                   const double *__restrict__ dphi, double *__restrict__ Un) {
+// Solute equation at one cell (karma3d.f90:211-278): U of the next step.
+template <class F>
+__device__ __forceinline__ double solute_cell(const F &P, const F &U, const F &Dp, const KStep &g) {
 #pragma clang fp contract(off)
+  const double pp = P(0, 0, 0), uc = U(0, 0, 0), dc = Dp(0, 0, 0);
+  const double cc = (1.0 + g.k1 * uc) * dc;
+  const double Fxp = uflux<0, true>(P, U, Dp, pp, uc, cc, g);
+  const double Fxm = uflux<0, false>(P, U, Dp, pp, uc, cc, g);
+  const double Fyp = uflux<1, true>(P, U, Dp, pp, uc, cc, g);
+  const double Fym = uflux<1, false>(P, U, Dp, pp, uc, cc, g);
+  const double Fzp = uflux<2, true>(P, U, Dp, pp, uc, cc, g);
+  const double Fzm = uflux<2, false>(P, U, Dp, pp, uc, cc, g);
+  const double Au = 0.5 * (1.0 + g.kp) - 0.5 * g.k1 * pp;
+  const double rhs = (Fxp - Fxm + Fyp - Fym + Fzp - Fzm) * g.idx + 0.5 * (1.0 + g.k1 * uc) * dc;
+  return uc + g.dt * rhs / Au;
+}
+
+// Stores a new phi (clamped, karma3d.f90:280-289) and dphi of cell c.
+__device__ __forceinline__ void put_phase(const KStep &g, int i, int j, int k, int c, double pp,
+                                          double dp, double *__restrict__ dphi,
+                                          double *__restrict__ Pn) {
+#pragma clang fp contract(off)
+  const double v = pp + g.dt * dp;
+  const double vc = v < -1.0 ? -1.0 : (v > 1.0 ? 1.0 : v);
+  dphi[c] = dp;
+  Pn[c] = vc;
+  put_ghosts(dphi, g, i, j, k, c, dp);
+  put_ghosts(Pn, g, i, j, k, c, vc);
+}
+
+// One thread per cell, neighbours read from HBM through the caches.
+__global__ void __launch_bounds__(kBX *kBY)
+    k_karma_phase(KStep g, const double *__restrict__ P, const double *__restrict__ U,
+                  double *__restrict__ dphi, double *__restrict__ Pn) {
   const int i = blockIdx.x * kBX + threadIdx.x + 1, j = blockIdx.y * kBY + threadIdx.y + 1;
   const int k = blockIdx.z + 1;
   if (i > g.n || j > g.n) return;
-  const int sx = 1, sy = g.s, sz = g.ss;
-  const int c = i + sy * j + sz * k;
-  const double pp = P[c], uc = U[c];
-  const double cc = (1.0 + g.k1 * uc) * dphi[c];
-  const double Fxp = uflux(P, U, dphi, c, sx, true, 0, sy, sz, pp, uc, cc, g);
-  const double Fxm = uflux(P, U, dphi, c, -sx, false, 0, sy, sz, pp, uc, cc, g);
-  const double Fyp = uflux(P, U, dphi, c, sy, true, 1, sx, sz, pp, uc, cc, g);
-  const double Fym = uflux(P, U, dphi, c, -sy, false, 1, sx, sz, pp, uc, cc, g);
-  const double Fzp = uflux(P, U, dphi, c, sz, true, 2, sx, sy, pp, uc, cc, g);
-  const double Fzm = uflux(P, U, dphi, c, -sz, false, 2, sx, sy, pp, uc, cc, g);
-  const double Au = 0.5 * (1.0 + g.kp) - 0.5 * g.k1 * pp;
-  const double rhs =
-      (Fxp - Fxm + Fyp - Fym + Fzp - Fzm) * g.idx + 0.5 * (1.0 + g.k1 * uc) * dphi[c];
-  Un[c] = uc + g.dt * rhs / Au;
+  const int c = i + g.s * j + g.ss * k;
+  const GlobalNb Pv{P, c, g.s, g.ss};
+  put_phase(g, i, j, k, c, P[c], phase_cell(Pv, U[c], i, j, k, g), dphi, Pn);
+}
+
+__global__ void __launch_bounds__(kBX *kBY)
+    k_karma_solute(KStep g, const double *__restrict__ P, const double *__restrict__ U,
+                   const double *__restrict__ dphi, double *__restrict__ Un) {
+  const int i = blockIdx.x * kBX + threadIdx.x + 1, j = blockIdx.y * kBY + threadIdx.y + 1;
+  const int k = blockIdx.z + 1;
+  if (i > g.n || j > g.n) return;
+  const int c = i + g.s * j + g.ss * k;
+  const double un = solute_cell(GlobalNb{P, c, g.s, g.ss}, GlobalNb{U, c, g.s, g.ss},
+                                GlobalNb{dphi, c, g.s, g.ss}, g);
+  Un[c] = un;
+  put_ghosts(Un, g, i, j, k, c, un);
+}
+
+// z-march forms: a work-group owns a kTX x kTY column of cells over kKC
+// planes and keeps the planes k-1, k, k+1 of its inputs (tile plus a
+// one-cell halo ring, ghosts included) in LDS, loading one new plane per
+// step of the march, so every input value is read from HBM about once.
+constexpr int kKC = 8;
+
+// Loads padded plane kk of the tile's halo region of up to three fields.
+__device__ __forceinline__ void load_plane(const KStep &g, int i0, int j0, int kk,
+                                           const double *__restrict__ a, double *sa,
+                                           const double *__restrict__ b, double *sb,
+                                           const double *__restrict__ d, double *sd) {
+  const int t = threadIdx.x + kTX * threadIdx.y;
+  for (int q = t; q < kLP; q += kTX * kTY) {
+    const int i = i0 + q % kLX, j = j0 + q / kLX;  // padded coordinates
+    if (i <= g.n + 1 && j <= g.n + 1) {
+      const int c = i + g.s * j + g.ss * kk;
+      sa[q] = a[c];
+      if (b) sb[q] = b[c];
+      if (d) sd[q] = d[c];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kTX *kTY)
+    k_karma_phase_zm(KStep g, const double *__restrict__ P, const double *__restrict__ U,
+                     double *__restrict__ dphi, double *__restrict__ Pn) {
+  __shared__ double sP[3][kLP];
+  const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;  // padded origin of the halo region
+  const int k0 = blockIdx.z * kKC + 1, k1 = min(k0 + kKC - 1, g.n);
+  const int i = i0 + threadIdx.x + 1, j = j0 + threadIdx.y + 1;
+  const bool own = i <= g.n && j <= g.n;
+  const int q = threadIdx.x + 1 + kLX * (threadIdx.y + 1);
+  load_plane(g, i0, j0, k0 - 1, P, sP[0], nullptr, nullptr, nullptr, nullptr);
+  load_plane(g, i0, j0, k0, P, sP[1], nullptr, nullptr, nullptr, nullptr);
+  int lo = 0, mid = 1, hi = 2;
+  for (int k = k0; k <= k1; ++k) {
+    load_plane(g, i0, j0, k + 1, P, sP[hi], nullptr, nullptr, nullptr, nullptr);
+    __syncthreads();
+    if (own) {
+      const int c = i + g.s * j + g.ss * k;
+      const PlaneNb Pv{sP[lo], sP[mid], sP[hi], q};
+      put_phase(g, i, j, k, c, Pv(0, 0, 0), phase_cell(Pv, U[c], i, j, k, g), dphi, Pn);
+    }
+    __syncthreads();  // plane lo is overwritten next
+    const int t = lo; lo = mid; mid = hi; hi = t;
+  }
+}
+
+__global__ void __launch_bounds__(kTX *kTY)
+    k_karma_solute_zm(KStep g, const double *__restrict__ P, const double *__restrict__ U,
+                      const double *__restrict__ dphi, double *__restrict__ Un) {
+  __shared__ double sP[3][kLP], sU[3][kLP], sD[3][kLP];
+  const int i0 = blockIdx.x * kTX, j0 = blockIdx.y * kTY;
+  const int k0 = blockIdx.z * kKC + 1, k1 = min(k0 + kKC - 1, g.n);
+  const int i = i0 + threadIdx.x + 1, j = j0 + threadIdx.y + 1;
+  const bool own = i <= g.n && j <= g.n;
+  const int q = threadIdx.x + 1 + kLX * (threadIdx.y + 1);
+  load_plane(g, i0, j0, k0 - 1, P, sP[0], U, sU[0], dphi, sD[0]);
+  load_plane(g, i0, j0, k0, P, sP[1], U, sU[1], dphi, sD[1]);
+  int lo = 0, mid = 1, hi = 2;
+  for (int k = k0; k <= k1; ++k) {
+    load_plane(g, i0, j0, k + 1, P, sP[hi], U, sU[hi], dphi, sD[hi]);
+    __syncthreads();
+    if (own) {
+      const int c = i + g.s * j + g.ss * k;
+      const double un = solute_cell(PlaneNb{sP[lo], sP[mid], sP[hi], q},
+                                    PlaneNb{sU[lo], sU[mid], sU[hi], q},
+                                    PlaneNb{sD[lo], sD[mid], sD[hi], q}, g);
+      Un[c] = un;
+      put_ghosts(Un, g, i, j, k, c, un);
+    }
+    __syncthreads();
+    const int t = lo; lo = mid; mid = hi; hi = t;
+  }
 }
 
 }  // namespace
@@ -199,6 +320,7 @@ struct pf_karma {
   double D = 0, Omega = 0;
   double *phi[2] = {nullptr, nullptr}, *U[2] = {nullptr, nullptr}, *dphi = nullptr;
   int cur = 0, it = 0;
+  int zmarch = 1;  // z-march kernels (1) or one thread per cell (0)
   hipStream_t st = nullptr;
 };
 
@@ -258,16 +380,23 @@ extern "C" int pf_karma_create(const pf_karma_params *p, pf_karma **out) {
     karma_free(s);
     return pf::fail_msg(PF_E_NOMEM, "pf_karma_create: %zu bytes per field", bytes);
   }
-  // seed (karma3d.f90:130-144), ghosts included, on the host
+  // seed (karma3d.f90:130-144) on the cells 1..n, then the mirror ghosts
+  // the first step's espelho sets (U is uniform, its ghosts included)
   std::vector<double> ph(s->cells), u(s->cells, -s->Omega);
   const int n = s->n;
-  for (int k = 0; k <= n + 1; ++k)
-    for (int j = 0; j <= n + 1; ++j)
-      for (int i = 0; i <= n + 1; ++i) {
+  auto at = [&](int i, int j, int k) { return (size_t)i + (size_t)s->s * j + (size_t)s->ss * k; };
+  auto cl = [n](int v) { return v < 1 ? 1 : (v > n ? n : v); };
+  for (int k = 1; k <= n; ++k)
+    for (int j = 1; j <= n; ++j)
+      for (int i = 1; i <= n; ++i) {
         const double x = (i - 0.5) * p->dx, y = (j - 0.5) * p->dx, z = (k - 0.5) * p->dx;
         const double r = std::sqrt(x * x + y * y + z * z);
-        ph[(size_t)i + (size_t)s->s * j + (size_t)s->ss * k] = -std::tanh((r - p->R0) / std::sqrt(2.0));
+        ph[at(i, j, k)] = -std::tanh((r - p->R0) / std::sqrt(2.0));
       }
+  for (int k = 0; k <= n + 1; ++k)
+    for (int j = 0; j <= n + 1; ++j)
+      for (int i = 0; i <= n + 1; ++i)
+        if (i != cl(i) || j != cl(j) || k != cl(k)) ph[at(i, j, k)] = ph[at(cl(i), cl(j), cl(k))];
   if (hipMemcpy(s->phi[0], ph.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(s->U[0], u.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(s->dphi, 0, bytes) != hipSuccess) {
@@ -282,21 +411,35 @@ extern "C" int pf_karma_run(pf_karma *s, int32_t nsteps) {
   if (!s || nsteps < 0) return pf::fail_msg(PF_E_ARG, "pf_karma_run: bad argument");
   const int n = s->n;
   const dim3 blk(kBX, kBY, 1);
-  const dim3 gm((n + 2 + kBX - 1) / kBX, (n + 2 + kBY - 1) / kBY, 3);
   const dim3 gc((n + kBX - 1) / kBX, (n + kBY - 1) / kBY, n);
+  const dim3 bz(kTX, kTY, 1);
+  const dim3 gz((n + kTX - 1) / kTX, (n + kTY - 1) / kTY, (n + kKC - 1) / kKC);
   for (int t = 0; t < nsteps; ++t) {
     s->it += 1;
     const KStep g = kstep(s);
     const int a = s->cur, b = 1 - s->cur;
-    hipLaunchKernelGGL(k_karma_mirror, gm, blk, 0, s->st, g, s->phi[a], s->U[a]);
-    hipLaunchKernelGGL(k_karma_phase, gc, blk, 0, s->st, g, s->phi[a], s->U[a], s->dphi, s->phi[b]);
-    hipLaunchKernelGGL(k_karma_mirror, gm, blk, 0, s->st, g, s->dphi, (double *)nullptr);
-    hipLaunchKernelGGL(k_karma_solute, gc, blk, 0, s->st, g, s->phi[a], s->U[a], s->dphi, s->U[b]);
+    if (s->zmarch) {
+      hipLaunchKernelGGL(k_karma_phase_zm, gz, bz, 0, s->st, g, s->phi[a], s->U[a], s->dphi,
+                         s->phi[b]);
+      hipLaunchKernelGGL(k_karma_solute_zm, gz, bz, 0, s->st, g, s->phi[a], s->U[a], s->dphi,
+                         s->U[b]);
+    } else {
+      hipLaunchKernelGGL(k_karma_phase, gc, blk, 0, s->st, g, s->phi[a], s->U[a], s->dphi,
+                         s->phi[b]);
+      hipLaunchKernelGGL(k_karma_solute, gc, blk, 0, s->st, g, s->phi[a], s->U[a], s->dphi,
+                         s->U[b]);
+    }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess)
       return pf::fail_msg(PF_E_HIP, "pf_karma_run: launch failed: %s", hipGetErrorString(e));
     s->cur = b;
   }
+  return PF_OK;
+}
+
+extern "C" int pf_karma_set_zmarch(pf_karma *s, int32_t on) {
+  if (!s) return pf::fail_msg(PF_E_ARG, "pf_karma_set_zmarch: null context");
+  s->zmarch = on ? 1 : 0;
   return PF_OK;
 }
 

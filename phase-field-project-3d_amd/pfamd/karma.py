@@ -15,7 +15,7 @@ import numpy as np
 from . import capi
 
 EXPORTS = ("pf_karma_create", "pf_karma_run", "pf_karma_download", "pf_karma_synchronize",
-           "pf_karma_destroy")
+           "pf_karma_destroy", "pf_karma_set_zmarch")
 
 
 class KarmaParams(C.Structure):
@@ -132,6 +132,7 @@ def _lib():
         lib.pf_karma_download.argtypes = [vp, vp, vp, C.POINTER(i32)]
         lib.pf_karma_synchronize.argtypes = [vp]
         lib.pf_karma_destroy.argtypes = [vp]
+        lib.pf_karma_set_zmarch.argtypes = [vp, i32]
         for nm in EXPORTS:
             getattr(lib, nm).restype = C.c_int
         lib._karma_sig = True
@@ -151,6 +152,9 @@ class Karma:
 
     def synchronize(self):
         capi.check(self.lib.pf_karma_synchronize(self.h), self.lib)
+
+    def set_zmarch(self, on: bool):
+        capi.check(self.lib.pf_karma_set_zmarch(self.h, int(bool(on))), self.lib)
 
     def fields(self):
         """(phi, U, it): the cells 1..N as (N, N, N) arrays indexed [k, j, i]."""

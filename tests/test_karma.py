@@ -106,15 +106,20 @@ def test_library_exports_karma_symbols():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,steps,Fnoise", [(16, 40, 0.05), (23, 60, 0.0), (40, 30, 0.02)])
-def test_karma_gpu_matches_oracle(n, steps, Fnoise):
+@pytest.mark.parametrize("zmarch", [True, False])
+@pytest.mark.parametrize("n,steps,Fnoise", [(16, 40, 0.05), (23, 60, 0.0), (40, 30, 0.02),
+                                            (37, 20, 0.01)])
+def test_karma_gpu_matches_oracle(n, steps, Fnoise, zmarch):
     """Same operation order with FP contraction off on both sides, correctly
     rounded division and square root on the device, the seed set on the host:
-    the GPU fields equal the oracle's bit for bit."""
-    print(f"[karma] n={n} steps={steps} Fnoise={Fnoise}", flush=True)
+    the GPU fields equal the oracle's bit for bit, in both kernel forms (the
+    z-march tiles are 32 x 8 x 8: n = 23, 37, 40 leave partial tiles on every
+    axis)."""
+    print(f"[karma] n={n} steps={steps} Fnoise={Fnoise} zmarch={zmarch}", flush=True)
     R0 = min(8.0, 0.45 * n * 0.8)
     out, _ = _oracle_fields(n, steps, Fnoise, R0, record=(steps,))
     m = Karma(KarmaDeck(N=n, Fnoise=Fnoise, R0=R0))
+    m.set_zmarch(zmarch)
     try:
         m.run(steps // 2)
         m.run(steps - steps // 2)
