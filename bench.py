@@ -342,12 +342,19 @@ def serving_study(n=24, rate=4.0, peak_mlups=None):
     return out
 
 
-def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
-    """The quantitative Karma model (karma/karma3d.f90) at its own default
-    size, 200^3 with the namelist defaults: GPU steps timed after a warm-up,
-    and the reference program compiled from its source (oracle/_ref) on a
-    bounded sample (96^3, 200 steps, this host's cores), as it times itself
-    (its loop wall clock, printed before its output file is written)."""
+def karma_study(n=200, warm=20, steps=500, cpu_steps=40, cpu=True):
+    """The quantitative Karma model (karma/karma3d.f90) on the karma.nml the
+    reference ships (200^3, lambda 4, eps4 0.02, dx 0.8, dt 0.008, R0 8,
+    Fnoise 0.01): GPU steps timed after a warm-up, and the reference program
+    compiled from its source (oracle/_ref) on the same 200^3 grid for a
+    bounded number of steps on this host's physical cores, as it times
+    itself (its loop wall clock, printed before its output file, whose
+    writing is not waited for: the process is ended once the line is out).
+
+    Its step is FP64-VALU bound (correctly rounded divisions and square
+    roots per face, every expression in the reference's order): the roofline
+    reported is the FP64 issue rate, from the instruction count per cell of
+    the two kernels."""
     import re
     import subprocess
     import tempfile
@@ -355,9 +362,11 @@ def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
 
     import numpy as np
 
-    from pfamd.karma import Karma, KarmaDeck
+    from pfamd.karma import Karma, KarmaDeck, tips
 
-    m = Karma(KarmaDeck(N=n))
+    deck = KarmaDeck.shipped()
+    deck.N = n
+    m = Karma(deck)
     try:
         m.run(warm)
         m.synchronize()
@@ -369,36 +378,57 @@ def karma_study(n=200, warm=20, steps=500, cpu_n=96, cpu_steps=200):
     finally:
         m.close()
     mlups = n ** 3 * steps / wall / 1e6
-    # per cell-step: phase kernel reads phi, U and writes dphi, phi'; the
-    # solute kernel reads phi, U, dphi and writes U' (ghost copies aside)
-    out = {"config": f"{n}^3, karma.nml defaults (lambda 4, eps4 0.02, dx 0.8, dt 0.008, R0 8), "
-                     f"{warm} warm-up + {steps} timed steps, FP64",
+    # FP64 VALU instructions per cell-step of the two z-march kernels (their
+    # loops' static counts; rcp/rsq are quarter-rate: 3 extra issue slots)
+    valu = {"phase": 481 + 3 * 21, "solute": 362 + 3 * (7 + 6)}
+    per_cell = sum(valu.values())
+    valu_peak = 256 * 4 * 16 * 2.4e9  # lane-instructions/s (FP64 at 16 lanes/SIMD/clk)
+    out = {"config": f"{n}^3, karma.nml as shipped (lambda 4, eps4 0.02, dx 0.8, dt 0.008, R0 8, "
+                     f"Fnoise 0.01), {warm} warm-up + {steps} timed steps, FP64, z-march kernels",
            "mlups": round(mlups, 1), "ms_per_step": round(wall / steps * 1e3, 4),
-           "alg_bytes_per_cell_step": 64,
-           "achieved_gbs": round(mlups * 1e6 * 64 / 1e9, 1),
-           "frac_of_hbm_peak": round(mlups * 1e6 * 64 / 8.0e12, 4),
-           "tip_cells_on_x_axis": int(np.max(np.nonzero(phi[0, 0, :] > 0)[0], initial=-1) + 1),
+           "roofline": {"bound": "valu_fp64",
+                        "fp64_issue_slots_per_cell_step": per_cell,
+                        "achieved_tinstr_s": round(mlups * 1e6 * per_cell / 1e12, 2),
+                        "peak_tinstr_s": round(valu_peak / 1e12, 2),
+                        "frac": round(mlups * 1e6 * per_cell / valu_peak, 3),
+                        "hbm_alg_bytes_per_cell_step": 64,
+                        "hbm_frac": round(mlups * 1e6 * 64 / 8.0e12, 4)},
+           "tip_cells": dict(zip(("x_axis", "diagonal"), tips(phi))),
            "reference_published_mlups": 200.0,
            "reference_published_note": "200^3 for 45k steps in ~30 min on the reference's GPU"}
     log(f"[bench] karma {n}^3: {out['mlups']} MLUPS, {out['ms_per_step']} ms/step")
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_karma3d")
-    if os.path.exists(exe):
+    if cpu and os.path.exists(exe):
         threads = host_cpu()["physical_cores"]
         with tempfile.TemporaryDirectory() as d:
+            # one progress line, at the last step
+            cpu_deck = KarmaDeck.shipped()
+            cpu_deck.N, cpu_deck.nsteps, cpu_deck.nout, cpu_deck.tip_stop = (n, cpu_steps,
+                                                                             cpu_steps, 2.0)
             with open(os.path.join(d, "karma.nml"), "w") as f:
-                f.write(f"&karma\n N = {cpu_n}\n nsteps = {cpu_steps}\n nout = {cpu_steps}\n"
-                        " tip_stop = 2.0\n/\n")
+                f.write(cpu_deck.to_nml())
             os.makedirs(os.path.join(d, "output"))
             env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                        OMP_PLACES="cores", OMP_TARGET_OFFLOAD="DISABLED")
-            r = subprocess.run([exe], cwd=d, env=env, capture_output=True, text=True, timeout=300)
-        w = re.findall(r"wall=\s*([0-9.]+)\s*s", r.stdout)
-        if r.returncode == 0 and w and float(w[-1]) > 0:
-            cpu = cpu_n ** 3 * cpu_steps / float(w[-1]) / 1e6
+            pr = subprocess.Popen([exe], cwd=d, env=env, stdout=subprocess.PIPE, text=True)
+            wline = None
+            t_end = time.time() + 300
+            for line in pr.stdout:
+                if "wall=" in line:
+                    wline = line
+                    break
+                if time.time() > t_end:
+                    break
+            pr.kill()
+            pr.wait()
+        w = re.findall(r"wall=\s*([0-9.]+)\s*s", wline or "")
+        if w and float(w[-1]) > 0:
+            cpu = n ** 3 * cpu_steps / float(w[-1]) / 1e6
             out["cpu_reference"] = {"value": round(cpu, 2), "unit": "MLUPS", "cores": threads,
                                     "kind": "reference",
-                                    "sample": f"{cpu_n}^3, {cpu_steps} steps, its own loop wall "
-                                              f"clock {w[-1]} s (0.1 s resolution)"}
+                                    "sample": f"{n}^3 (the same grid and namelist), {cpu_steps} "
+                                              f"steps, its own loop wall clock {w[-1]} s "
+                                              "(0.1 s resolution)"}
             out["gpu_vs_cpu"] = round(mlups / cpu, 1)
             log(f"[bench] karma CPU reference: {cpu:.1f} MLUPS on {threads} threads")
     return out
@@ -615,7 +645,7 @@ def main(argv=None):
         out["serving"] = serving_study(peak_mlups=peaks if len(peaks) == 2 else None)
     if not a.no_karma:
         log("[bench] Karma model")
-        out["karma"] = karma_study()
+        out["karma"] = karma_study(cpu=not a.no_cpu_baseline)
     if not a.no_cpu_baseline:
         log("[bench] CPU baseline (reference OpenMP path)")
         cb = cpu_baseline()
