@@ -592,9 +592,9 @@ def main(argv=None):
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
                                      "1.0, factors from a copy of known size)",
                      "traffic_kernel": traffic_key(r64, rl["kernel"], "fp64"),
-                     "limiter": "VALU issue and latency: 187-265 VALU instructions per cell "
-                                "per FP64 kernel at 63-74% issue, 3 waves/SIMD (FP64 register "
-                                "demand)",
+                     "limiter": "VALU issue and latency: 258 (sweep A) and 267 (phase + T) "
+                                "VALU instructions per cell at 60-63% issue, 3 waves/SIMD at "
+                                "155-162 VGPRs (FP64 register demand; a fourth wave spills)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,
