@@ -176,7 +176,12 @@ __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict
   const R amp = (p.ruido_tipo == PF_RUIDO_CELL_POSITIVE) ? (R)ran * (R)p.AAA
                                                          : ((R)2 * (R)ran - one) * (R)p.AAA;
   const R ruidos = amp * (R)16 * (PP * PP) * ((one - PP) * (one - PP));
-  const R PPNEW = PP + ((R)p.M0 * (R)p.dt) * (AA + GG) + ruidos;
+  R PPNEW;
+  if constexpr (sizeof(R) == 4)  // stochastically rounded store (csum, pf_mass.h)
+    PPNEW = csum<R>(PP, ((R)p.M0 * (R)p.dt) * (AA + GG) + ruidos,
+                    cell_hash(cell_hash_col(i, j, p.it) ^ kSaltPhiFirst, k));
+  else
+    PPNEW = PP + ((R)p.M0 * (R)p.dt) * (AA + GG) + ruidos;
   dpp = PPNEW - PP;
   const R cl = PPNEW > one ? one : PPNEW;  // dmax1(0,dmin1(1,x)), NaN kept
   pnew = cl < (R)0 ? (R)0 : cl;

@@ -459,7 +459,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   R cZ = 0;
   R cPPcur = 0;  // c of the plane updated next (sweeps A and phase + T)
   uint64_t hbase = hash_base(i, j, ks, kp.it, 1, kp.seed);
-  const uint32_t hcol = cell_hash_col(i, j, kp.it);  // csum's draw (FP32 solute)
+  const uint32_t hcol = cell_hash_col(i, j, kp.it);  // csum's draws (FP32 stores)
   (void)hcol;
   // solute increments of this column for the ledger, in ledger units
   const bool acc_on = SWEEP_B && macc_dst != nullptr;  // uniform
@@ -586,7 +586,14 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
                                                               : ((R)2 * (R)ran - (R)1) * s.AAA;
         ruidos = amp * (R)16 * (PP * PP) * (om * om);
       }
-      const R pstar = PP + s.M0dt * (AA + GG) + ruidos;
+      R pstar;
+      if constexpr (sizeof(R) == 4) {
+        // stochastically rounded store of phi (csum in pf_mass.h)
+        const R incp = s.M0dt * (AA + GG) + ruidos;
+        pstar = csum<R>(PP, incp, cell_hash(hcol ^ (T_OUT ? kSaltPhiLast : kSaltPhiFirst), k));
+      } else {
+        pstar = PP + s.M0dt * (AA + GG) + ruidos;
+      }
       const R cl = pstar > (R)1 ? (R)1 : pstar;
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;

@@ -147,16 +147,19 @@ __device__ __forceinline__ void mass_scale_rows(R *__restrict__ c, int nx, int n
   }
 }
 
-// The stored c after an increment, cst + inc.  FP64: rounded to nearest.
-// FP32: rounded stochastically to one of the two floats around the exact
-// sum, with the probability that makes the expected stored value exact
-// (TwoSum gives the rounding error e of the nearest sum s; the neighbour of s
-// on e's side is taken with probability |e| / gap), the draw from a 32-bit
-// counter hash of the cell and the step (h).  Increments far below an FP32
-// ulp of c -- those of the solid, whose diffusivity is 1e-4 of the
-// liquid's -- are then kept on average instead of all being rounded away,
-// which biased the mass change the FP32 form computes (by ~2e-4 of the FP64
-// change per step) and the interface with it.
+// A stored value after an increment, cst + inc (c after the solute update,
+// phi after a phase sweep).  FP64: rounded to nearest.  FP32: rounded
+// stochastically to one of the two floats around the exact sum, with the
+// probability that makes the expected stored value exact (TwoSum gives the
+// rounding error e of the nearest sum s; the neighbour of s on e's side is
+// taken with probability |e| / gap), the draw from a 32-bit counter hash of
+// the cell and the step (h).  Increments far below an FP32 ulp are then kept
+// on average instead of all being rounded away: those of c in the solid,
+// whose diffusivity is 1e-4 of the liquid's, and those of phi as it
+// approaches 1 in the solid, where round-to-nearest stalled phi below its
+// FP64 trajectory and biased the solute flux across the walls (the FP32
+// form's interior mass drifted 4.3e-7 from the FP64 run's over 5,000 steps
+// of a growth deck with c alone rounded so, 5.2e-8 with phi too).
 template <typename R>
 __device__ __forceinline__ R csum(R cst, R inc, uint32_t h);
 template <>
@@ -193,6 +196,13 @@ __device__ __forceinline__ uint32_t cell_hash(uint32_t col, int k) {
   h ^= h >> 16;
   return h;
 }
+
+// Salts that keep the draws of one step's FP32 stores apart: the solute
+// store draws from cell_hash(col, k), the phase stores from
+// cell_hash(col ^ salt, k) -- kSaltPhiFirst for the sweeps before the last
+// one of a step, kSaltPhiLast for the last (the drop-in kernel, called once
+// per sweep, uses kSaltPhiFirst for every sweep).
+constexpr uint32_t kSaltPhiFirst = 0x1B56C4E9u, kSaltPhiLast = 0x68E31DA4u;
 
 // FP64 interior mass of a host field (the ledger's start when the stored
 // field is FP32: its rounding would otherwise enter the ledger).  Defined in
