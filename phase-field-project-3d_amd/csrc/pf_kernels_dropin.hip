@@ -15,6 +15,7 @@
 #include "pf_common.h"
 #include "pf_kernels.h"
 #include "pf_mass.h"
+#include "pf_math.h"
 
 #pragma clang fp contract(off)
 
@@ -167,11 +168,25 @@ __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict
     h = PP * PP * ((R)3 - (R)2 * PP);
   }
   const R Ke = (R)p.Ke;
-  const R CLP = CPP / (Ke * h + (one - h));
-  const R CSP = Ke * CPP / (Ke * h + (one - h));
-  const R GG = (-((R)p.W0 * g_linha)) +
-               ((R)p.R * (R)p.TP / (R)p.Vm) * h_linha *
-                   rlog<R>(fabs(((one - (R)p.Cs_eq) * (one - CLP)) / ((one - (R)p.Cl_eq) * (one - CSP))));
+  R lnr;
+  if constexpr (sizeof(R) == 4) {
+    // the ratio minus 1 formed without cancellation, as the fused FP32
+    // sweeps do (ratio_ln_f32 in pf_march.h): (den A - c B) / (den - Ke c)
+    // with A = K - 1, B = K - Ke, K = (1-Cs)/(1-Cl), and den A - c B =
+    // (A - c_off B) - h (1 - Ke) A - dc B for the stored dc = c - c_off (the
+    // constants in FP64, then rounded)
+    const double K = (1.0 - p.Cs_eq) / (1.0 - p.Cl_eq);
+    const double A = K - 1.0, B = K - p.Ke;
+    const R den = Ke * h + (one - h);
+    const R x = ((R)(A - p.c_off * B) - h * (R)((1.0 - p.Ke) * A) - co[c] * (R)B) /
+                (den - Ke * CPP);
+    lnr = ln1p_abs_f32(x);
+  } else {
+    const R CLP = CPP / (Ke * h + (one - h));
+    const R CSP = Ke * CPP / (Ke * h + (one - h));
+    lnr = rlog<R>(fabs(((one - (R)p.Cs_eq) * (one - CLP)) / ((one - (R)p.Cl_eq) * (one - CSP))));
+  }
+  const R GG = (-((R)p.W0 * g_linha)) + ((R)p.R * (R)p.TP / (R)p.Vm) * h_linha * lnr;
   const float ran = ran_real4(i, j, k, p.it, p.ruido_tipo, p.seed);
   const R amp = (p.ruido_tipo == PF_RUIDO_CELL_POSITIVE) ? (R)ran * (R)p.AAA
                                                          : ((R)2 * (R)ran - one) * (R)p.AAA;
@@ -179,7 +194,7 @@ __device__ __forceinline__ void phase_body(const KParams &p, const R *__restrict
   R PPNEW;
   if constexpr (sizeof(R) == 4)  // stochastically rounded store (csum, pf_mass.h)
     PPNEW = csum<R>(PP, ((R)p.M0 * (R)p.dt) * (AA + GG) + ruidos,
-                    cell_hash(cell_hash_col(i, j, p.it) ^ kSaltPhiFirst, k));
+                    cell_hash_col(i, j, p.it) ^ kSaltPhiFirst, k);
   else
     PPNEW = PP + ((R)p.M0 * (R)p.dt) * (AA + GG) + ruidos;
   dpp = PPNEW - PP;
@@ -281,9 +296,7 @@ __global__ void __launch_bounds__(256) k_solute_interior(KParams p, const R *__r
   const R inc = (Ge * IX * (MU(CLE) - muP) - Gw * IX * (muP - MU(CLW)) + Gn * IY * (MU(CLN) - muP) -
                  Gs * IY * (muP - MU(CLS)) + Gt * IZ * (MU(CLT) - muP) - Gb * IZ * (muP - MU(CLB))) *
                 (R)p.dt;
-  uint32_t hc = 0;
-  if constexpr (sizeof(R) == 4) hc = cell_hash(cell_hash_col(i, j, p.it), k);
-  cn[c] = csum<R>(co[c], inc, hc);
+  cn[c] = csum<R>(co[c], inc, cell_hash_col(i, j, p.it), k);
 #undef MU
 }
 

@@ -119,6 +119,9 @@ struct SC {
   R Dliq2dt, Dsol2dt;  // 2*D*dt for the solute faces
   R fonte, TP, Ddt;    // delta_H/cp, TP, D*dt
   R coff;              // offset of the stored c (KParams::c_off; 0 for FP64)
+  // the free-energy ratio K (1-CL)/(1-Ke CL) minus 1 (FP32 form, see
+  // ratio_ln_f32): A = K - 1, B = K - Ke, K1 = A - c_off B, gAh = (1 - Ke) A
+  R gAh, gB, gK1;
 };
 
 // Computed on the host for a single instance and on the device for batched
@@ -142,6 +145,13 @@ __host__ __device__ inline SC<R> make_sc(const KParams &p) {
   s.Dliq2dt = (R)(2.0 * p.D_liq * p.dt); s.Dsol2dt = (R)(2.0 * p.D_sol * p.dt);
   s.fonte = (R)(p.delta_H / p.cp_s); s.TP = (R)p.TP; s.Ddt = (R)(p.D * p.dt);
   s.coff = (R)p.c_off;
+  {
+    const double K = (1.0 - p.Cs_eq) / (1.0 - p.Cl_eq);
+    const double A = K - 1.0, B = K - p.Ke;
+    s.gAh = (R)((1.0 - p.Ke) * A);
+    s.gB = (R)B;
+    s.gK1 = (R)(A - p.c_off * B);
+  }
   return s;
 }
 
@@ -181,6 +191,24 @@ __device__ __forceinline__ void h_of(R p, int quint, R &h, R &hl) {
     h = p * p * ((R)3 - (R)2 * p);
     hl = (R)6 * p * ((R)1 - p);
   }
+}
+
+// ln of the free-energy ratio K (den - c)/(den - Ke c) of the phase
+// equation (phase_body.inc:159-170, with CL = c/den folded in), FP32 form.
+// Near equilibrium the ratio is within a few 1e-3 of 1, and forming it in
+// FP32 before the logarithm leaves an absolute error of ~1 ulp of 1
+// (6e-8) in ln, i.e. ~50 J/m^3 of driving force per cell, which pushed phi
+// off the FP64 trajectory by ~1e-5 within 1,000 steps and, through the
+// reference's non-conservative solute faces, the solute mass by up to 1e-6.
+// Instead the ratio minus 1 is formed directly,
+//   (den A - c B) / (den - Ke c),  A = K - 1, B = K - Ke,
+// with den A - c B = K1 - h (1 - Ke) A - dc B for the stored dc = c - c_off
+// (den - 1 = -h (1 - Ke); K1 = A - c_off B), whose terms are all small, and
+// its logarithm is ln|1 + x| (pf_math.h; the reference takes ln of |ratio|).
+__device__ __forceinline__ float ratio_ln_f32(float h, float den, float dc, float cfull,
+                                              const SC<float> &s) {
+  const float x = (s.gK1 - h * s.gAh - dc * s.gB) * rcpn<float>(den - s.Ke * cfull);
+  return ln1p_abs_f32(x);
 }
 
 // The full c of a stored value (FP32: stored as c - c_off).
@@ -578,8 +606,14 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       // (1-Cs)(1-CLP)/((1-Cl)(1-CSP)) with CLP = c/den, CSP = Ke CLP
       const R den = s.Ke * h + ((R)1 - h);
       const R cfP = cfull<R>(cPP, s);
-      const R arg = s.omCsCl * (den - cfP) * rcpn<R>(den - s.Ke * cfP);
-      const R GG = hl * s.RTV * dlog<R>(fabs(arg), lt) - s.W0 * gl;
+      R lnr;
+      if constexpr (sizeof(R) == 4) {
+        lnr = ratio_ln_f32(h, den, cPP, cfP, s);
+      } else {
+        const R arg = s.omCsCl * (den - cfP) * rcpn<R>(den - s.Ke * cfP);
+        lnr = dlog<R>(fabs(arg), lt);
+      }
+      const R GG = hl * s.RTV * lnr - s.W0 * gl;
       R ruidos = (R)0;
       if (noise) {
         const R amp = kp.ruido_tipo == PF_RUIDO_CELL_POSITIVE ? (R)ran * s.AAA
@@ -590,7 +624,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       if constexpr (sizeof(R) == 4) {
         // stochastically rounded store of phi (csum in pf_mass.h)
         const R incp = s.M0dt * (AA + GG) + ruidos;
-        pstar = csum<R>(PP, incp, cell_hash(hcol ^ (T_OUT ? kSaltPhiLast : kSaltPhiFirst), k));
+        pstar = csum<R>(PP, incp, hcol ^ (T_OUT ? kSaltPhiLast : kSaltPhiFirst), k);
       } else {
         pstar = PP + s.M0dt * (AA + GG) + ruidos;
       }
@@ -615,9 +649,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
         const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
-        uint32_t hc = 0;
-        if constexpr (sizeof(R) == 4) hc = cell_hash(hcol, k);
-        const R cnew = csum<R>(cPP, inc, hc);
+        const R cnew = csum<R>(cPP, inc, hcol, k);
         bstore(cn + pk, own ? ooff : kNoStore, cnew);
         // x faces of the row (concentracao_soluto.f90:116-124: a face cell of
         // a row inside [2,n-1] in y and z copies its inner neighbour's new
@@ -803,9 +835,7 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const R igc = igZ, muc = muZ;
       const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
-      uint32_t hc = 0;
-      if constexpr (sizeof(R) == 4) hc = cell_hash(hcol, k);
-      const R cnew = csum<R>(cPP, inc, hc);
+      const R cnew = csum<R>(cPP, inc, hcol, k);
       bstore(cn + pk, own ? ooff : kNoStore, cnew);
       // x faces of the row, as in march_tile's sweep B
       constexpr uint32_t E = (uint32_t)sizeof(R);

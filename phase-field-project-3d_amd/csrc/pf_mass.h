@@ -147,39 +147,6 @@ __device__ __forceinline__ void mass_scale_rows(R *__restrict__ c, int nx, int n
   }
 }
 
-// A stored value after an increment, cst + inc (c after the solute update,
-// phi after a phase sweep).  FP64: rounded to nearest.  FP32: rounded
-// stochastically to one of the two floats around the exact sum, with the
-// probability that makes the expected stored value exact (TwoSum gives the
-// rounding error e of the nearest sum s; the neighbour of s on e's side is
-// taken with probability |e| / gap), the draw from a 32-bit counter hash of
-// the cell and the step (h).  Increments far below an FP32 ulp are then kept
-// on average instead of all being rounded away: those of c in the solid,
-// whose diffusivity is 1e-4 of the liquid's, and those of phi as it
-// approaches 1 in the solid, where round-to-nearest stalled phi below its
-// FP64 trajectory and biased the solute flux across the walls (the FP32
-// form's interior mass drifted 4.3e-7 from the FP64 run's over 5,000 steps
-// of a growth deck with c alone rounded so, 5.2e-8 with phi too).
-template <typename R>
-__device__ __forceinline__ R csum(R cst, R inc, uint32_t h);
-template <>
-__device__ __forceinline__ double csum<double>(double cst, double inc, uint32_t) {
-  return cst + inc;
-}
-template <>
-__device__ __forceinline__ float csum<float>(float cst, float inc, uint32_t h) {
-#pragma clang fp contract(off)
-  const float s = cst + inc;
-  const float bb = s - cst;
-  const float e = (cst - (s - bb)) + (inc - bb);  // cst + inc = s + e exactly
-  // the float next to s on e's side (s == 0 implies e == 0: both operands are
-  // multiples of the smallest denormal, so is their sum)
-  const int step = (e > 0.f) == (s > 0.f) ? 1 : -1;
-  const float nb = __int_as_float(__float_as_int(s) + step);
-  const float u = (float)(h >> 8) * 0x1p-24f;  // [0, 1)
-  return u * fabsf(nb - s) < fabsf(e) ? nb : s;  // false for e == 0 and for NaN
-}
-
 // 32-bit counter hash of a cell and step for csum: the "lowbias32"
 // finaliser (C. Wellons, hash-prospector) over an odd-multiplier mix of the
 // coordinates.  col = cell_hash_col(i, j, it) once per column, then
@@ -195,6 +162,45 @@ __device__ __forceinline__ uint32_t cell_hash(uint32_t col, int k) {
   h *= 0x846CA68Bu;
   h ^= h >> 16;
   return h;
+}
+
+// A stored value after an increment, cst + inc (c after the solute update,
+// phi after a phase sweep).  FP64: rounded to nearest.  FP32: rounded
+// stochastically to one of the two floats around the exact sum, with the
+// probability that makes the expected stored value exact (TwoSum gives the
+// rounding error e of the nearest sum s; the neighbour of s on e's side is
+// taken with probability |e| / gap), the draw from a 32-bit counter hash of
+// the cell and the step.  Increments far below an FP32 ulp are then kept
+// on average instead of all being rounded away: those of c in the solid,
+// whose diffusivity is 1e-4 of the liquid's, and those of phi as it
+// approaches 1 in the solid, where round-to-nearest stalled phi below its
+// FP64 trajectory and biased the solute flux across the walls (the FP32
+// form's interior mass drifted 4.3e-7 from the FP64 run's over 5,000 steps
+// of a growth deck with c alone rounded so, 5.2e-8 with phi too).
+template <typename R>
+__device__ __forceinline__ R csum(R cst, R inc, uint32_t col, int k);
+template <>
+__device__ __forceinline__ double csum<double>(double cst, double inc, uint32_t, int) {
+  return cst + inc;
+}
+// (col, k) name the draw, cell_hash(col, k) below.  It is hashed only in
+// waves where some lane's sum is inexact (e != 0): a lane with e == 0 keeps
+// s whatever the draw, so the result depends on the lane's own values
+// alone, and waves of exact sums -- zero increments in uniform regions --
+// skip the hash.
+template <>
+__device__ __forceinline__ float csum<float>(float cst, float inc, uint32_t col, int k) {
+#pragma clang fp contract(off)
+  const float s = cst + inc;
+  const float bb = s - cst;
+  const float e = (cst - (s - bb)) + (inc - bb);  // cst + inc = s + e exactly
+  if (__builtin_amdgcn_ballot_w64(e != 0.f) == 0) return s;
+  // the float next to s on e's side (s == 0 implies e == 0: both operands are
+  // multiples of the smallest denormal, so is their sum)
+  const int step = (e > 0.f) == (s > 0.f) ? 1 : -1;
+  const float nb = __int_as_float(__float_as_int(s) + step);
+  const float u = (float)(cell_hash(col, k) >> 8) * 0x1p-24f;  // [0, 1)
+  return u * fabsf(nb - s) < fabsf(e) ? nb : s;  // false for e == 0 and for NaN
 }
 
 // Salts that keep the draws of one step's FP32 stores apart: the solute

@@ -95,4 +95,22 @@ __host__ __device__ __forceinline__ double log_tab(double x, const LogEnt *tab) 
   return res;
 }
 
+// ln|1 + x| in FP32 for an x formed without cancellation: a 6-term series
+// x (1 - x (1/2 - x (1/3 - x (1/4 - x (1/5 - x/6))))) below |x| = 1/16
+// (truncation < 1e-8 relative), the hardware logarithm of |1 + x| above --
+// evaluated only by waves that hold such an x (most waves hold none); which
+// form a lane uses depends on x alone.
+__device__ __forceinline__ float ln1p_abs_f32(float x) {
+  float r = x * fmaf(-x, fmaf(-x, fmaf(-x, fmaf(-x, fmaf(-x, 1.0f / 6.0f, 0.2f), 0.25f),
+                                      1.0f / 3.0f),
+                              0.5f),
+                     1.0f);
+  const bool big = !(fabsf(x) < 0.0625f);  // NaN included
+  if (__builtin_amdgcn_ballot_w64(big) != 0) {
+    const float lg = __logf(fabsf(1.0f + x));
+    r = big ? lg : r;
+  }
+  return r;
+}
+
 }  // namespace pf

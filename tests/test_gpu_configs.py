@@ -17,9 +17,9 @@ the GPU work) or against the single-instance form, bit for bit.
       steps, drop-in and fused against the oracle, the same bounds
       (measured phi 1.4e-17 drop-in, 5.6e-16 fused).
   C3  the FP32-mixed form on C1's grid against the FP64 form at 5,000 steps:
-      phi max-abs <= 1e-3; the solute mass of the cooling
-      deck within 1e-8 of the FP64 run (the ledger starts from the FP64
-      field); the growth deck's gap is printed and bounded by 1e-6.
+      phi max-abs <= 1e-3 and the interior solute mass within 1e-8 of the
+      FP64 run, on both decks (measured on the MI355X: cooling 4.2e-11,
+      growth 3.8e-9; 7.3e-11 and 3.1e-9 at 20,000 steps).
   C4  the 64-instance ensemble (edges 32-160, FP32-mixed, seed 7), 300
       steps batched (three mass corrections): six instances bit for bit
       equal to pf_run_steps on their own.
@@ -192,7 +192,7 @@ def test_c3_fp32_vs_fp64_5000_steps(name):
           flush=True)
     assert ep <= 1e-3
     assert abs(m32 - led[0]) <= 1e-10 * abs(led[0])
-    assert abs(gap) <= (1e-8 if name == "cooling" else 1e-6)
+    assert abs(gap) <= 1e-8
 
 
 def _single(pr, steps):
