@@ -388,8 +388,10 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
                   int reps, hipStream_t st, AutotuneResult *res) {
   const int nx = S.nx, ny = S.ny, nz = S.nz;
   const size_t n = (size_t)nx * ny * nz;
-  R *out = nullptr;  // scratch outputs: phi, c, T (the context's fields are not touched)
-  if (hipMalloc((void **)&out, 3 * n * sizeof(R)) != hipSuccess) return PF_E_NOMEM;
+  // scratch outputs: phi, c, T and the second sweep's phi (the context's
+  // fields are not touched)
+  R *out = nullptr;
+  if (hipMalloc((void **)&out, 4 * n * sizeof(R)) != hipSuccess) return PF_E_NOMEM;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = PF_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PF_E_HIP;
@@ -445,30 +447,90 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
     }
   }
   // the runners-up are often within timing noise of the first: time the
-  // three fastest of each kind again, three times as long, and keep the
+  // three fastest of each kind again, three times as long, and keep the two
   // fastest of those
+  std::vector<Cand> top2[4];
   for (int kind = 0; kind < 4 && rc == PF_OK; ++kind) {
     std::vector<Cand> top;
     for (const Cand &c : all)
       if (c.kind == kind) top.push_back(c);
     std::sort(top.begin(), top.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
     if (top.size() > 3) top.resize(3);
-    for (const Cand &c : top) {
-      float ms = 0.f;
-      if (time_cand(kind, c.tile, c.target, 3 * reps, ms) != hipSuccess) { rc = PF_E_HIP; break; }
-      if (ms < best[kind]) {
-        best[kind] = ms;
+    for (Cand &c : top) {
+      if (time_cand(kind, c.tile, c.target, 3 * reps, c.ms) != hipSuccess) { rc = PF_E_HIP; break; }
+      if (c.ms < best[kind]) {
+        best[kind] = c.ms;
         btile[kind] = c.tile;
         btgt[kind] = c.target;
       }
     }
+    std::sort(top.begin(), top.end(), [](const Cand &a, const Cand &b) { return a.ms < b.ms; });
+    if (top.size() > 2) top.resize(2);
+    top2[kind] = top;
   }
-  if (rc == PF_OK) {
-    S.split_b = best[2] + best[3] < best[1];
-    const int kb = S.split_b ? 2 : 1;
-    fused_set_kind(S, 0, btile[0], btgt[0]);
-    fused_set_kind(S, 1, btile[kb], btgt[kb]);
-    fused_set_kind(S, 2, btile[3], btgt[3]);
+  // Kernels timed one at a time do not see how they run one after another
+  // in a step (the previous kernel's tail, what it left in the caches):
+  // the finalists are compared as whole sequences -- sweep A, then either
+  // the split pair (phase + T, solute) or the fused sweep B, reading and
+  // writing the scratch fields as a step reads and writes its fields -- and
+  // the fastest sequence is kept.
+  auto geo_of = [&](int kind, int tile, int target) {
+    const int seg = pick_seg(nx, ny, kind ? nz - 2 : nz - 4, tile, target);
+    return kind ? make_geo(nx, ny, nz, 2, nz - 1, seg, tile) : make_geo(nx, ny, nz, 3, nz - 2, seg, tile);
+  };
+  auto time_seq = [&](const Cand &a, const Cand *pt, const Cand *sl, const Cand *fb, int nrep,
+                      float &ms) -> hipError_t {
+    const Geo ga = geo_of(0, a.tile, a.target);
+    const Geo gb = pt ? geo_of(2, pt->tile, pt->target) : geo_of(1, fb->tile, fb->target);
+    const Geo gs = sl ? geo_of(3, sl->tile, sl->target) : gb;
+    R *phiH = out, *cN = out + n, *tN = out + 2 * n, *phiN = out + 3 * n;
+    hipError_t e = hipSuccess;
+    for (int r = 0; r <= nrep && e == hipSuccess; ++r) {  // r = 0 is the warm-up
+      if (r == 1 && (e = hipEventRecord(e0, st)) != hipSuccess) break;
+      e = launch_march<R>(kSweepA, a.tile, ga, k, po, co, phiH, nullptr, Tc, nullptr, nullptr, 1,
+                          nullptr, st);
+      if (e != hipSuccess) break;
+      if (pt) {
+        e = launch_march<R>(kSweepPT, pt->tile, gb, k, phiH, co, phiN, nullptr, Tc, tN, nullptr, 1,
+                            nullptr, st);
+        if (e == hipSuccess) e = launch_solute_split<R>(sl->tile, gs, k, phiH, co, cN, nullptr, st);
+      } else {
+        e = launch_march<R>(kSweepB, fb->tile, gb, k, phiH, co, phiN, cN, Tc, tN, nullptr, 1,
+                            nullptr, st);
+      }
+    }
+    ms = 0.f;
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)nrep;
+    return e;
+  };
+  float best_seq = 1e30f;
+  for (const Cand &a : top2[0]) {
+    if (rc != PF_OK) break;
+    for (const Cand &pt : top2[2])
+      for (const Cand &sl : top2[3]) {
+        float ms = 0.f;
+        if (rc == PF_OK && time_seq(a, &pt, &sl, nullptr, 3 * reps, ms) != hipSuccess) rc = PF_E_HIP;
+        if (rc == PF_OK && ms < best_seq) {
+          best_seq = ms;
+          S.split_b = true;
+          fused_set_kind(S, 0, a.tile, a.target);
+          fused_set_kind(S, 1, pt.tile, pt.target);
+          fused_set_kind(S, 2, sl.tile, sl.target);
+        }
+      }
+    for (const Cand &fb : top2[1]) {
+      float ms = 0.f;
+      if (rc == PF_OK && time_seq(a, nullptr, nullptr, &fb, 3 * reps, ms) != hipSuccess) rc = PF_E_HIP;
+      if (rc == PF_OK && ms < best_seq) {
+        best_seq = ms;
+        S.split_b = false;
+        fused_set_kind(S, 0, a.tile, a.target);
+        fused_set_kind(S, 1, fb.tile, fb.target);
+      }
+    }
   }
   if (res) {
     res->n = cand < PF_AUTOTUNE_MAX ? cand : PF_AUTOTUNE_MAX;
