@@ -389,14 +389,22 @@ constexpr int march_waves() {
   return MODE != kSweepB && VAR == 0 ? 3 : 1;
 }
 
-template <typename R, int MODE, typename TL, int VAR = kVarAll>
-__device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
-                                           const R *__restrict__ po, const R *__restrict__ co,
-                                           R *__restrict__ pn, R *__restrict__ cn,
-                                           const R *__restrict__ Tcur, R *__restrict__ Tnext,
-                                           R *__restrict__ dpp, int write_T,
-                                           unsigned long long *__restrict__ macc_dst, int bx,
-                                           int by, int sg) {
+// Whether the tile column bx holds a column whose sweep writes x-face cells
+// (the phase update's i = 3 and nx-2, the solute's i = 2 and nx-1).  The
+// tiles without one run a loop body without the x-face stores (XF false).
+__device__ __forceinline__ bool tile_has_xface(int bx, int tx, int nx) {
+  const int ilo = 2 + bx * tx;
+  return ilo <= 3 || ilo + tx - 1 >= nx - 2;
+}
+
+template <typename R, int MODE, typename TL, int VAR, bool XF>
+__device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp, const SC<R> &s,
+                                                const R *__restrict__ po, const R *__restrict__ co,
+                                                R *__restrict__ pn, R *__restrict__ cn,
+                                                const R *__restrict__ Tcur, R *__restrict__ Tnext,
+                                                R *__restrict__ dpp, int write_T,
+                                                unsigned long long *__restrict__ macc_dst, int bx,
+                                                int by, int sg) {
   // FMAs only within one source expression, decided by the front end, so the
   // single-instance and batched kernels that inline this body round alike
 #pragma clang fp contract(on)
@@ -610,8 +618,10 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       if constexpr (sizeof(R) == 4) {
         lnr = ratio_ln_f32(h, den, cPP, cfP, s);
       } else {
-        const R arg = s.omCsCl * (den - cfP) * rcpn<R>(den - s.Ke * cfP);
-        lnr = dlog<R>(fabs(arg), lt);
+        // ln|(1-Cs_eq)(1-CLP)/((1-Cl_eq)(1-CSP))| = ln|A/B| with
+        // A = (1-Cs_eq)/(1-Cl_eq) (den - c), B = den - Ke c: a ratio within
+        // a few 1e-3 of 1 near equilibrium
+        lnr = ln_ratio_near1(s.omCsCl * (den - cfP), den - s.Ke * cfP, lt);
       }
       const R GG = hl * s.RTV * lnr - s.W0 * gl;
       R ruidos = (R)0;
@@ -632,7 +642,7 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;
       bstore(pn + pk, upd ? ooff : kNoStore, pnew);
-      {
+      if constexpr (XF) {
         // the x faces of an interior row (equacao_campo_fase.f90:151-167 in
         // closed form: the face cells of rows inside [3,n-2] in y and z):
         // cell 2 (nx-1) takes the swept value of cell 3 (nx-2), cell 1 (nx)
@@ -654,9 +664,11 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
         // x faces of the row (concentracao_soluto.f90:116-124: a face cell of
         // a row inside [2,n-1] in y and z copies its inner neighbour's new
         // value); the face kernel after the sweep leaves them out
-        constexpr uint32_t E = (uint32_t)sizeof(R);
-        bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
-               cnew);
+        if constexpr (XF) {
+          constexpr uint32_t E = (uint32_t)sizeof(R);
+          bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+                 cnew);
+        }
         if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
       }
       if constexpr (T_OUT) {
@@ -724,6 +736,25 @@ __device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, cons
   }
 }
 
+// One work-group's tile (bx, by) and z-segment sg of a sweep of kind MODE;
+// the x-face stores are compiled only into the body that tiles touching an
+// x face run.
+template <typename R, int MODE, typename TL, int VAR = kVarAll>
+__device__ __forceinline__ void march_tile(const Geo &g, const KParams &kp, const SC<R> &s,
+                                           const R *__restrict__ po, const R *__restrict__ co,
+                                           R *__restrict__ pn, R *__restrict__ cn,
+                                           const R *__restrict__ Tcur, R *__restrict__ Tnext,
+                                           R *__restrict__ dpp, int write_T,
+                                           unsigned long long *__restrict__ macc_dst, int bx,
+                                           int by, int sg) {
+  if (tile_has_xface(bx, TL::TX, g.nx))
+    march_tile_body<R, MODE, TL, VAR, true>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp, write_T,
+                                            macc_dst, bx, by, sg);
+  else
+    march_tile_body<R, MODE, TL, VAR, false>(g, kp, s, po, co, pn, cn, Tcur, Tnext, dpp,
+                                             write_T, macc_dst, bx, by, sg);
+}
+
 // LDS bytes of a march_tile work-group.
 template <typename R, typename TL>
 constexpr size_t march_lds_bytes(int mode) {
@@ -746,12 +777,12 @@ constexpr size_t solute_arrays_bytes() {
 // published before the barrier of iteration q and read after the barrier of
 // iteration q+1; its buffer is rewritten in iteration q+3, after a barrier
 // all its readers have passed), so one barrier per plane suffices.
-template <typename R, typename TL>
-__device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, const SC<R> &s,
-                                            const R *__restrict__ po, const R *__restrict__ co,
-                                            R *__restrict__ cn,
-                                            unsigned long long *__restrict__ macc_dst, int bx,
-                                            int by, int sg) {
+template <typename R, typename TL, bool XF>
+__device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp,
+                                                 const SC<R> &s, const R *__restrict__ po,
+                                                 const R *__restrict__ co, R *__restrict__ cn,
+                                                 unsigned long long *__restrict__ macc_dst,
+                                                 int bx, int by, int sg) {
 #pragma clang fp contract(on)
   constexpr int TX = TL::TX, NT = TL::NT, QX = TL::QX, QY = TL::QY, QPL = TL::QPL;
   constexpr int NRING = TL::NRING;
@@ -838,8 +869,11 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
       const R cnew = csum<R>(cPP, inc, hcol, k);
       bstore(cn + pk, own ? ooff : kNoStore, cnew);
       // x faces of the row, as in march_tile's sweep B
-      constexpr uint32_t E = (uint32_t)sizeof(R);
-      bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore, cnew);
+      if constexpr (XF) {
+        constexpr uint32_t E = (uint32_t)sizeof(R);
+        bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+               cnew);
+      }
       if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
@@ -848,6 +882,18 @@ __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, con
   for (int q = q_first; q < ks + 1; ++q) step(q, std::false_type{});
   for (int q = ks + 1; q <= q_last; ++q) step(q, std::true_type{});
   if (acc_on) mass_flush<NT>(macc, tid, macc_dst);
+}
+
+template <typename R, typename TL>
+__device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, const SC<R> &s,
+                                            const R *__restrict__ po, const R *__restrict__ co,
+                                            R *__restrict__ cn,
+                                            unsigned long long *__restrict__ macc_dst, int bx,
+                                            int by, int sg) {
+  if (tile_has_xface(bx, TL::TX, g.nx))
+    solute_tile_body<R, TL, true>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+  else
+    solute_tile_body<R, TL, false>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
 }
 
 // (its log table after the arrays)

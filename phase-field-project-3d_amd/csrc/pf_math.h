@@ -95,6 +95,38 @@ __host__ __device__ __forceinline__ double log_tab(double x, const LogEnt *tab) 
   return res;
 }
 
+// ln|A/B| for a ratio close to 1, FP64.  With t = (A - B)/(A + B) the ratio
+// is (1 + t)/(1 - t) and ln(A/B) = 2 atanh(t) = 2t (1 + t^2/3 + ... + t^12/13)
+// + O(t^15), truncated below 2^-56 relative for |t| <= 1/16 (A/B in
+// [15/17, 17/15]).  A - B is formed from the operands, so t keeps its full
+// relative precision however close the ratio is to 1, where the logarithm
+// of a rounded ratio carries an absolute error of about one ulp of 1.
+// Every other lane value (a ratio further from 1, zero, negative or not
+// finite, and t = NaN from A + B = 0) takes log_tab(|A/B|), the
+// reference's ln|ratio|; on the device only waves that hold such a value
+// evaluate it.  Which form a lane uses depends on its own t alone.
+__host__ __device__ __forceinline__ double ln_ratio_near1(double A, double B,
+                                                          const LogEnt *tab) {
+  const double t = (A - B) * rcp_pos(A + B);
+  const double t2 = t * t;
+  double p = fma(t2, 2.0 / 13.0, 2.0 / 11.0);
+  p = fma(t2, p, 2.0 / 9.0);
+  p = fma(t2, p, 2.0 / 7.0);
+  p = fma(t2, p, 2.0 / 5.0);
+  p = fma(t2, p, 2.0 / 3.0);
+  double r = t * fma(t2, p, 2.0);
+  const bool far = !(fabs(t) <= 0.0625);
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (__builtin_amdgcn_ballot_w64(far) != 0) {
+    const double lg = log_tab(fabs(A * rcp_fast(B)), tab);
+    r = far ? lg : r;
+  }
+#else
+  if (far) r = log_tab(fabs(A * rcp_fast(B)), tab);
+#endif
+  return r;
+}
+
 // ln|1 + x| in FP32 for an x formed without cancellation: a 6-term series
 // x (1 - x (1/2 - x (1/3 - x (1/4 - x (1/5 - x/6))))) below |x| = 1/16
 // (truncation < 1e-8 relative), the hardware logarithm of |1 + x| above --

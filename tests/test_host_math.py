@@ -42,3 +42,24 @@ def test_log_accuracy(tmp_path):
     assert int(n) >= 4000000
     assert float(max_ulp) <= 1.5, r.stdout
     assert float(mean_ulp) <= 0.35, r.stdout
+
+
+@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="no hipcc")
+def test_ratio_log_accuracy(tmp_path):
+    """pf::ln_ratio_near1, the free-energy logarithm ln|A/B| of the fused
+    sweeps (source/phase_body.inc:167-174): within 4 ulp of the result where
+    the 2 atanh series is used (|A-B|/(A+B) <= 1/16; the logarithm of a
+    rounded quotient would be off by up to an ulp of 1 there, many ulps of a
+    result near 0), within 5 ulp of 1 elsewhere, and log|A/B| of the special
+    operands (0, +-inf, NaN, A = -B)."""
+    exe = tmp_path / "ratio_log_accuracy"
+    subprocess.run([HIPCC, "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-I", CSRC,
+                    "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "ratio_log_accuracy.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe), "1000000"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    near_ulp, far_ulp1, n = r.stdout.split()[:3]
+    assert int(n) >= 1300000
+    assert float(near_ulp) <= 4.0, r.stdout
+    assert float(far_ulp1) <= 5.0, r.stdout

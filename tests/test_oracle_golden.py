@@ -239,3 +239,40 @@ def test_thermal_term_brute_force_superposition(oracle_lib):
     resp = np.zeros_like(dev)
     resp[inner] = acc[inner]
     np.testing.assert_allclose(got[inner] - TP, resp[inner], rtol=1e-9, atol=1e-9 * TP)
+
+
+def test_growth_deck_amplifies_one_ulp(oracle_lib):
+    """The reference scheme amplifies rounding on the growth deck (64x64x128,
+    one seed r=5 at the bottom centre, held at TP = 905 K, noise off): moving
+    phi by +-1 ulp (a seeded random sign per cell) after step 3 and running to
+    step 20 departs from the unperturbed run by >= 1e-12 in phi, four orders
+    of magnitude above the perturbation -- the fast-growing tail of the
+    front, where h'(phi) > 0 and the driving force make phi grow by a factor
+    of two per step, carries the difference.  A form that rounds differently
+    from the reference (the fused sweeps) therefore differs from the oracle
+    by the same spread whatever its accuracy; the GPU tests of that deck
+    bound it by a multiple of the spread (test_gpu_configs.py)."""
+    from pfamd.deck import Deck
+
+    O = oracle_lib
+    O.set_threads(max(1, min(8, os.cpu_count() or 1)))
+    deck = Deck(Nx=64, Ny=64, Nz=128, tipo_inicial=5, R_semente=5.0, h_quintico=0, ruido_tipo=0,
+                AAA=0.0, n_varreduras=2, stoptime=0.0, taxa_resfr=0.0, TP=905.0)
+    d = deck.resolved()
+
+    def run(seed):
+        p, L = params_from_deck(deck, O)
+        F = O.initial_fields(d.Nx, d.Ny, d.Nz, d.tipo_inicial, d.R_semente, d.T0, d.C0, d.Ke)
+        O.run_steps(p, L, 3, F)
+        if seed:
+            s = np.random.default_rng(seed).integers(-1, 2, size=np.asarray(F.phi_new).shape)
+            for arr in (F.phi_new, F.phi_old):
+                a = np.asarray(arr)
+                up, dn = np.nextafter(a, np.inf), np.nextafter(a, -np.inf)
+                a[:] = np.maximum(np.where(s > 0, up, np.where(s < 0, dn, a)), 0.0)
+        O.run_steps(p, L, 17, F)
+        return F.phi_new.copy()
+
+    ref = run(0)
+    spread = max(float(np.max(np.abs(run(seed) - ref))) for seed in (1, 2))
+    assert 1e-12 <= spread <= 1e-9, spread
