@@ -232,20 +232,44 @@ __device__ __forceinline__ R cfull(R stored, const SC<R> &s) {
 // the difference CL_X - CL_P first, which in the liquid (h = 0, den = 1)
 // is dc_X - dc_P: the increments of the near-uniform liquid keep the
 // precision of the stored offsets, far below an FP32 ulp of c.
-template <typename R>
+// GUARD selects the reciprocal: rcpn (a zero, infinite or NaN denominator
+// gives what a division gives) or rcpn_pos (NaN there, otherwise the same
+// bits).  The FP64 callers take the unguarded form and, in a wave where a
+// result is NaN, evaluate the guarded one (solute_node_w, solute_inc_w):
+// exact for every input at the cost of one compare in the usual case.
+template <typename R, bool GUARD>
+__device__ __forceinline__ R rcp_g(R x) {
+  if constexpr (GUARD) return rcpn<R>(x);
+  else return rcpn_pos<R>(x);
+}
+
+template <typename R, bool GUARD = true>
 __device__ __forceinline__ void solute_node(R p, R cst, int quint, const SC<R> &s,
                                             const LogEnt *lt, R &ig, R &q) {
   R h, hl;
   h_of(p, quint, h, hl);
   const R den = s.Ke * h + ((R)1 - h);
   const R cc = cfull<R>(cst, s);
-  const R rden = rcpn<R>(den);
+  const R rden = rcp_g<R, GUARD>(den);
   const R CL = cc * rden;
   const R CS = s.Ke * CL;
   const R g = ((R)1 - h) * CL * ((R)1 - CL) + h * CS * ((R)1 - CS);
-  ig = rcpn<R>(g);
-  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcpn<R>(den - cc), lt);
+  ig = rcp_g<R, GUARD>(g);
+  if constexpr (sizeof(R) == 8) q = dlog<R>(cc * rcp_g<R, GUARD>(den - cc), lt);
   else q = (cst + s.coff * ((R)1 - s.Ke) * h) * rden;
+}
+
+// solute_node with the FP64 reciprocals checked per wave (see rcp_g).
+template <typename R>
+__device__ __forceinline__ void solute_node_w(R p, R cst, int quint, const SC<R> &s,
+                                              const LogEnt *lt, R &ig, R &q) {
+  if constexpr (sizeof(R) == 8) {
+    solute_node<R, false>(p, cst, quint, s, lt, ig, q);
+    if (__builtin_amdgcn_ballot_w64(ig != ig || q != q) != 0)
+      solute_node<R, true>(p, cst, quint, s, lt, ig, q);
+  } else {
+    solute_node<R, true>(p, cst, quint, s, lt, ig, q);
+  }
 }
 
 template <typename R>
@@ -294,20 +318,41 @@ __device__ __forceinline__ float dmu<float>(float qX, float qP, float coff) {
 // negated (exact: dmu is odd and a + b = b + a), except for the first cell
 // of a z-segment (fresh): the carried flux is a product rounded on its own
 // (fzu, a statement of its own, is never fused into a sum).
-template <typename R>
+template <typename R, bool GUARD = true>
 __device__ __forceinline__ R solute_inc(const R *ig, const R *mu, int nc, int qx, R igc, R muc,
-                                        R igP, R muP, R igM, R muM, bool fresh, R &fzu_prev,
-                                        R D2dt, const SC<R> &s) {
+                                        R igP, R muP, R igM, R muM, bool fresh, R fzu_prev,
+                                        R &fzu, R D2dt, const SC<R> &s) {
   const R co = s.coff;
-  const R fx = dmu<R>(mu[nc + 1], muc, co) * rcpn<R>(ig[nc + 1] + igc) +
-               dmu<R>(mu[nc - 1], muc, co) * rcpn<R>(ig[nc - 1] + igc);
-  const R fy = dmu<R>(mu[nc + qx], muc, co) * rcpn<R>(ig[nc + qx] + igc) +
-               dmu<R>(mu[nc - qx], muc, co) * rcpn<R>(ig[nc - qx] + igc);
-  const R fzu = dmu<R>(muP, muc, co) * rcpn<R>(igP + igc);
-  const R fzd = fresh ? dmu<R>(muM, muc, co) * rcpn<R>(igM + igc) : -fzu_prev;
-  fzu_prev = fzu;
+  const R fx = dmu<R>(mu[nc + 1], muc, co) * rcp_g<R, GUARD>(ig[nc + 1] + igc) +
+               dmu<R>(mu[nc - 1], muc, co) * rcp_g<R, GUARD>(ig[nc - 1] + igc);
+  const R fy = dmu<R>(mu[nc + qx], muc, co) * rcp_g<R, GUARD>(ig[nc + qx] + igc) +
+               dmu<R>(mu[nc - qx], muc, co) * rcp_g<R, GUARD>(ig[nc - qx] + igc);
+  fzu = dmu<R>(muP, muc, co) * rcp_g<R, GUARD>(igP + igc);
+  const R fzd = fresh ? dmu<R>(muM, muc, co) * rcp_g<R, GUARD>(igM + igc) : -fzu_prev;
   const R fz = fzu + fzd;
   return D2dt * (fx * s.idx2 + fy * s.idy2 + fz * s.idz2);
+}
+
+// solute_inc with the FP64 reciprocals checked per wave (see rcp_g); the
+// flux through the cell's up face goes to fzu_prev for the next cell.
+template <typename R>
+__device__ __forceinline__ R solute_inc_w(const R *ig, const R *mu, int nc, int qx, R igc,
+                                          R muc, R igP, R muP, R igM, R muM, bool fresh,
+                                          R &fzu_prev, R D2dt, const SC<R> &s) {
+  R fzu;
+  R inc;
+  if constexpr (sizeof(R) == 8) {
+    inc = solute_inc<R, false>(ig, mu, nc, qx, igc, muc, igP, muP, igM, muM, fresh, fzu_prev,
+                               fzu, D2dt, s);
+    if (__builtin_amdgcn_ballot_w64(inc != inc || fzu != fzu) != 0)
+      inc = solute_inc<R, true>(ig, mu, nc, qx, igc, muc, igP, muP, igM, muM, fresh, fzu_prev,
+                                fzu, D2dt, s);
+  } else {
+    inc = solute_inc<R, true>(ig, mu, nc, qx, igc, muc, igP, muP, igM, muM, fresh, fzu_prev, fzu,
+                              D2dt, s);
+  }
+  fzu_prev = fzu;
+  return inc;
 }
 
 // The element at byte offset boff from base.  With a uniform base (a field
@@ -551,9 +596,9 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
     if constexpr (SWEEP_B) {
       if (cnode) {
         const R *P0 = slot(q - 1);
-        solute_node(a1, cnew_col, quint, s, lt, igP, muP);  // plane q - 1, own column
+        solute_node_w(a1, cnew_col, quint, s, lt, igP, muP);  // plane q - 1, own column
         if (tid < NRING)
-          solute_node(P0[(rly + 1) * PX + rlx + 1], cring, quint, s, lt, rig, rmu);
+          solute_node_w(P0[(rly + 1) * PX + rlx + 1], cring, quint, s, lt, rig, rmu);
       }
     }
     if constexpr (ST) {
@@ -657,7 +702,7 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
         // solute at (i,j,k), k in [2,nz-1]
         const R *ig = sig, *mu = smu;
         const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
-        const R inc = solute_inc<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
+        const R inc = solute_inc_w<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
         const R cnew = csum<R>(cPP, inc, hcol, k);
         bstore(cn + pk, own ? ooff : kNoStore, cnew);
@@ -777,7 +822,7 @@ constexpr size_t solute_arrays_bytes() {
 // published before the barrier of iteration q and read after the barrier of
 // iteration q+1; its buffer is rewritten in iteration q+3, after a barrier
 // all its readers have passed), so one barrier per plane suffices.
-template <typename R, typename TL, bool XF>
+template <typename R, typename TL, bool XF, int LEDGER>
 __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp,
                                                  const SC<R> &s, const R *__restrict__ po,
                                                  const R *__restrict__ co, R *__restrict__ cn,
@@ -820,7 +865,9 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
 
   R igM = 0, muM = 0, igZ = 0, muZ = 0, igP = 0, muP = 0;
   R pZ = 0, cZ = 0;  // phi and c of the own cell of plane k
-  const bool acc_on = macc_dst != nullptr;  // uniform
+  // LEDGER 0 / 1: a body without / with the ledger's increments; 2: decided
+  // per work-group (macc_dst null or not)
+  const bool acc_on = LEDGER == 2 ? macc_dst != nullptr : LEDGER == 1;  // uniform
   long long macc = 0;
   R fzu_prev = (R)0;
   // node plane q (all in [1, nz]); prefetched one plane ahead
@@ -846,12 +893,12 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
     fetch(min(q + 1, q_last), pq, cq, rp, rcv);
     R *gw = sig + (q % 3) * QPL;
     R *qw = smu + (q % 3) * QPL;
-    solute_node(pcur, ccur, quint, s, lt, igP, muP);
+    solute_node_w(pcur, ccur, quint, s, lt, igP, muP);
     gw[oc] = igP;
     qw[oc] = muP;
     if (tid < NRING) {
       R rig, rmu;
-      solute_node(rpc, rcc, quint, s, lt, rig, rmu);
+      solute_node_w(rpc, rcc, quint, s, lt, rig, rmu);
       gw[rc] = rig;
       qw[rc] = rmu;
     }
@@ -864,7 +911,7 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
       const R *ig = sig + (k % 3) * QPL, *mu = smu + (k % 3) * QPL;
       const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
       const R igc = igZ, muc = muZ;
-      const R inc = solute_inc<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
+      const R inc = solute_inc_w<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
       const R cnew = csum<R>(cPP, inc, hcol, k);
       bstore(cn + pk, own ? ooff : kNoStore, cnew);
@@ -884,16 +931,29 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
   if (acc_on) mass_flush<NT>(macc, tid, macc_dst);
 }
 
+// The loop bodies: with or without the x-face stores; FP64 also with or
+// without the ledger's increments (FP64 runs keep no ledger by default:
+// its per-cell conversion to ledger units is then not computed), FP32 with
+// the ledger decided per work-group (on by default).
 template <typename R, typename TL>
 __device__ __forceinline__ void solute_tile(const Geo &g, const KParams &kp, const SC<R> &s,
                                             const R *__restrict__ po, const R *__restrict__ co,
                                             R *__restrict__ cn,
                                             unsigned long long *__restrict__ macc_dst, int bx,
                                             int by, int sg) {
-  if (tile_has_xface(bx, TL::TX, g.nx))
-    solute_tile_body<R, TL, true>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
-  else
-    solute_tile_body<R, TL, false>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+  const bool xf = tile_has_xface(bx, TL::TX, g.nx);
+  if constexpr (sizeof(R) == 8) {
+    if (macc_dst) {
+      if (xf) solute_tile_body<R, TL, true, 1>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+      else solute_tile_body<R, TL, false, 1>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+    } else {
+      if (xf) solute_tile_body<R, TL, true, 0>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+      else solute_tile_body<R, TL, false, 0>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+    }
+  } else {
+    if (xf) solute_tile_body<R, TL, true, 2>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+    else solute_tile_body<R, TL, false, 2>(g, kp, s, po, co, cn, macc_dst, bx, by, sg);
+  }
 }
 
 // (its log table after the arrays)
