@@ -370,6 +370,12 @@ int pf_batch_mass_ledger(pf_batch *b, int32_t slot, double out[3]);
 int pf_stream_copy(size_t bytes, int32_t iters, double *gbs);
 /* on != 0 starts recording (and clears earlier records); 0 stops. */
 int pf_timing_enable(pf_ctx *ctx, int32_t on);
+/* Records only launches of the kinds whose bit (1u << PF_KT_*) is set in
+ * `kinds` (all kinds after pf_init).  Each record is a pair of HIP events
+ * on the context's stream, which costs the stream a few microseconds per
+ * launch: a run timed by the wall clock records only the kind whose
+ * duration it needs.  No counterpart in the reference (a measurement aid). */
+int pf_timing_kinds(pf_ctx *ctx, uint32_t kinds);
 /* Synchronous: sums of durations [ms] and launch counts per kind since the
  * last enable; clears the records. */
 int pf_timing_read(pf_ctx *ctx, double ms[PF_KT_COUNT], int64_t count[PF_KT_COUNT]);

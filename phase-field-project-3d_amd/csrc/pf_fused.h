@@ -50,24 +50,38 @@ struct FusedState {
 };
 
 // HIP-event pairs recorded around launches on one stream, by kernel kind.
+// Only the kinds whose bit is set in `mask` are recorded: each record costs
+// the stream a few microseconds, so a run that needs one kernel's duration
+// records only that kind.
 struct KTimer {
   bool on = false;
+  uint32_t mask = 0xffffffffu;
+  bool open = false;             // a record's first event is in, its second not
   std::vector<hipEvent_t> pool;  // 2 events per record
   std::vector<int> kind;         // kind of record r (events 2r, 2r+1)
   void begin(int k, hipStream_t s) {
-    if (!on) return;
+    open = false;
+    if (!on || !((mask >> k) & 1u)) return;
     const size_t r = kind.size();
     while (pool.size() < 2 * (r + 1)) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;
       pool.push_back(e);
     }
+    if (hipEventRecord(pool[2 * r], s) != hipSuccess) {
+      on = false;  // timing only: stop timing
+      return;
+    }
     kind.push_back(k);
-    if (hipEventRecord(pool[2 * r], s) != hipSuccess) on = false;  // timing only: stop timing
+    open = true;
   }
   void end(hipStream_t s) {
-    if (!on || kind.empty()) return;
-    if (hipEventRecord(pool[2 * (kind.size() - 1) + 1], s) != hipSuccess) on = false;
+    if (!on || !open) return;
+    open = false;
+    if (hipEventRecord(pool[2 * (kind.size() - 1) + 1], s) != hipSuccess) {
+      on = false;
+      kind.pop_back();
+    }
   }
   void release() {
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);

@@ -775,6 +775,13 @@ int pf_timing_enable(pf_ctx *ctx, int32_t on) {
   return PF_OK;
 }
 
+int pf_timing_kinds(pf_ctx *ctx, uint32_t kinds) {
+  int rc = check_ctx(ctx, nullptr);
+  if (rc != PF_OK) return rc;
+  ctx->timer.mask = kinds;
+  return PF_OK;
+}
+
 int pf_timing_read(pf_ctx *ctx, double ms[PF_KT_COUNT], int64_t count[PF_KT_COUNT]) {
   int rc = check_ctx(ctx, nullptr);
   if (rc == PF_OK && (!ms || !count)) rc = fail(PF_E_ARG, "null output");

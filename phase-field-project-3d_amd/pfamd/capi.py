@@ -71,7 +71,7 @@ EXPORTS = (
     "pf_initial_fields", "pf_init", "pf_finalize", "pf_temperatura", "pf_equacao_campo_fase",
     "pf_phase_new_to_old", "pf_concentracao_soluto", "pf_end_step", "pf_diagnosticos",
     "pf_download", "pf_download_all", "pf_run_steps", "pf_synchronize", "pf_device_bytes",
-    "pf_timing_enable", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
+    "pf_timing_enable", "pf_timing_kinds", "pf_timing_read", "pf_set_fused", "pf_stream_copy", "pf_set_tiling",
     "pf_get_tiling", "pf_autotune", "pf_set_mass_correction", "pf_mass_ledger",
     "pf_checkpoint_save", "pf_checkpoint_load", "pf_set_sweep_split",
     "pf_get_sweep_split", "pf_set_sweep_tiling", "pf_get_sweep_tiling",
@@ -156,6 +156,7 @@ def load(path: str = LIB_PATH):
         "pf_synchronize": ([vp], C.c_int),
         "pf_device_bytes": ([vp, C.POINTER(C.c_size_t)], C.c_int),
         "pf_timing_enable": ([vp, i32], C.c_int),
+        "pf_timing_kinds": ([vp, C.c_uint32], C.c_int),
         "pf_timing_read": ([vp, vp, vp], C.c_int),
         "pf_set_fused": ([vp, i32], C.c_int),
         "pf_stream_copy": ([C.c_size_t, i32, C.POINTER(f64)], C.c_int),
@@ -353,6 +354,12 @@ class Context:
 
     def timing(self, on: bool):
         check(self.lib.pf_timing_enable(self.h, 1 if on else 0), self.lib)
+
+    def timing_kinds(self, names=None):
+        """Record only launches of these kinds (names of PF_KT_NAMES; None:
+        every kind)."""
+        mask = 0xFFFFFFFF if names is None else sum(1 << PF_KT_NAMES.index(n) for n in names)
+        check(self.lib.pf_timing_kinds(self.h, mask), self.lib)
 
     def timing_read(self):
         """{kind: (total_ms, launches)} since timing(True)."""
