@@ -139,6 +139,10 @@ __device__ __forceinline__ float ln1p_abs_f32(float x) {
                      1.0f);
   const bool big = !(fabsf(x) < 0.0625f);  // NaN included
   if (__builtin_amdgcn_ballot_w64(big) != 0) {
+    // (an instruction with side effects, so that the compiler keeps the
+    // branch instead of evaluating the logarithm in every wave and
+    // selecting)
+    __builtin_amdgcn_sched_barrier(0);
     const float lg = __logf(fabsf(1.0f + x));
     r = big ? lg : r;
   }
