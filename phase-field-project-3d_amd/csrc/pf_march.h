@@ -369,6 +369,22 @@ __device__ __forceinline__ const T &at(const T *base, uint32_t boff) {
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + boff);
 }
 
+// base + a byte offset (a uniform plane offset: see plane_bytes)
+template <typename T>
+__device__ __forceinline__ T *boff(T *base, uint64_t b) {
+  return reinterpret_cast<T *>(reinterpret_cast<char *>(base) + b);
+}
+template <typename T>
+__device__ __forceinline__ const T *boff(const T *base, uint64_t b) {
+  return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + b);
+}
+// Byte offset of plane kk (1-based, kk >= 1) of a field with planes of szb
+// bytes: a 32 x 32 -> 64-bit product (a plane of a field is under 4 GiB,
+// checked at launch), two scalar instructions instead of a 64 x 64 one.
+__device__ __forceinline__ uint64_t plane_bytes(int kk, uint32_t szb) {
+  return (uint64_t)(uint32_t)(kk - 1) * (uint64_t)szb;
+}
+
 // Store of one element at a per-lane byte offset from a uniform plane base,
 // without a branch: a lane that must not store passes kNoStore, an offset
 // beyond the range of the buffer resource built here, and the hardware drops
@@ -432,6 +448,12 @@ constexpr int march_waves() {
     return VAR != 0 ? 4 : 6;
   }
   return MODE != kSweepB && VAR == 0 ? 3 : 1;
+}
+
+// q % NSLOT for the first plane a work-group's march loads, q = k_lo +
+// sg * seg - 2 (>= 0).
+__device__ __forceinline__ int q_first_slot(const Geo &g, int sg) {
+  return (g.k_lo + sg * g.seg - 2) % NSLOT;
 }
 
 // Whether the tile column bx holds a column whose sweep writes x-face cells
@@ -503,26 +525,35 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
   const uint32_t roff =
       (uint32_t)((clampi(i0 - 1 + rlx, 1, nx) - 1) + (clampi(j0 - 1 + rly, 1, ny) - 1) * nx) *
       (uint32_t)sizeof(R);
-  auto plane = [&](int q) -> size_t { return (size_t)(clampi(q, 1, nz) - 1) * sz; };
+  const uint32_t szb = (uint32_t)(sz * sizeof(R));  // bytes per plane of a field
+  auto plane = [&](int q) -> uint64_t { return plane_bytes(clampi(q, 1, nz), szb); };
 
   R pre[NLD];
   auto gload = [&](int q) {
-    const R *src = po + plane(q);
+    const R *src = boff(po, plane(q));
 #pragma unroll
     for (int u = 0; u < NLD; ++u)
       pre[u] = at(src, loff[u]);
   };
-  auto slot = [&](int q) -> R * { return sphi + (q % NSLOT) * PPL; };  // q >= 0
-  auto lstore = [&](int q) {
-    R *dst = slot(q);
+  // LDS slots of phi planes q-3 .. q+1 for the plane q being marched
+  // (slot of plane q = q % NSLOT), rotated from plane to plane: no index
+  // arithmetic per plane
+  R *sl[NSLOT];
+  {
+    const int s0 = q_first_slot(g, sg);
+#pragma unroll
+    for (int d = 0; d < NSLOT; ++d) sl[d] = sphi + ((s0 + d + NSLOT - 3) % NSLOT) * PPL;
+  }
+  auto slotr = [&](int d) -> R * { return sl[d + 3]; };  // d in [-3, 1]
+  auto lstore = [&](R *dst) {
 #pragma unroll
     for (int u = 0; u < NLD; ++u)
       if (tid + u * NT < PPL) dst[tid + u * NT] = pre[u];
   };
 
   // node at node-frame (lx,ly) of plane q-1 from phi planes q-2, q-1, q
-  auto node_at = [&](int q, int lx, int ly) -> Node<R> {
-    const R *Pm = slot(q - 2), *P0 = slot(q - 1), *Pp = slot(q);
+  auto node_at = [&](int lx, int ly) -> Node<R> {
+    const R *Pm = slotr(-2), *P0 = slotr(-1), *Pp = slotr(0);
     const int c = (ly + 1) * PX + (lx + 1);
     const R gx = (P0[c + 1] - P0[c - 1]) * s.hx;
     const R gy = (P0[c + PX] - P0[c - PX]) * s.hy;
@@ -573,29 +604,30 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
     const bool cnode = SWEEP_B && (ST || (q >= ks && kn >= 1 && kn <= nz));
     R cnew_col = 0, cring = 0, cPPnext = 0, cPPnow = 0;
     if (cnode) {
-      const size_t pk = (size_t)(kn - 1) * sz;
-      cnew_col = at(co + pk, ooff);
-      cring = at(co + pk, roff);  // every lane: no branch around the load
+      const uint64_t pk = plane_bytes(kn, szb);
+      cnew_col = at(boff(co, pk), ooff);
+      cring = at(boff(co, pk), roff);  // every lane: no branch around the load
     }
-    if constexpr (C_AHEAD) cPPnext = at(co + plane(q - 1), ooff);  // plane q - 1 (clamped)
-    if constexpr (ST && !SWEEP_B && !C_AHEAD) cPPnow = at(co + (size_t)(q - 3) * sz, ooff);
+    if constexpr (C_AHEAD) cPPnext = at(boff(co, plane(q - 1)), ooff);  // plane q - 1 (clamped)
+    if constexpr (ST && !SWEEP_B && !C_AHEAD)
+      cPPnow = at(boff(co, plane_bytes(q - 2, szb)), ooff);
     gload(min(q + 1, q_last));  // prefetch; lands while this plane computes
     __syncthreads();            // phi plane q (stored at the end of plane q - 1) in LDS
     const bool build = ST || (q >= ks && kn >= 2 && kn <= nz - 1);
-    const R aq = slot(q)[(oly + 1) * PX + (olx + 1)];  // phi of plane q, own column
+    const R aq = slotr(0)[(oly + 1) * PX + (olx + 1)];  // phi of plane q, own column
     Node<R> nr{};
     R rig = 0, rmu = 0;
     if (build) {
       // own node: phi of plane q and q-2 at the own column from the carry
-      const R *P0 = slot(q - 1);
+      const R *P0 = slotr(-1);
       const int c = (oly + 1) * PX + (olx + 1);
       nP = node_from((P0[c + 1] - P0[c - 1]) * s.hx, (P0[c + PX] - P0[c - PX]) * s.hy,
                      (aq - a2) * s.hz, s);
-      if (tid < NRING) nr = node_at(q, rlx, rly);
+      if (tid < NRING) nr = node_at(rlx, rly);
     }
     if constexpr (SWEEP_B) {
       if (cnode) {
-        const R *P0 = slot(q - 1);
+        const R *P0 = slotr(-1);
         solute_node_w(a1, cnew_col, quint, s, lt, igP, muP);  // plane q - 1, own column
         if (tid < NRING)
           solute_node_w(P0[(rly + 1) * PX + rlx + 1], cring, quint, s, lt, rig, rmu);
@@ -613,11 +645,11 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
         ran = (VAR & kVarSmooth) && kp.ruido_tipo == PF_RUIDO_SMOOTH
                   ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
                   : (float)hash_unit(hbase);
-      const R *Pm = slot(k - 1), *P0 = slot(k), *Pp = slot(k + 1);
+      const R *Pm = slotr(-3), *P0 = slotr(-2), *Pp = slotr(-1);  // planes k-1, k, k+1
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = a2;  // planes k, k+1, k-1 of the own column
 
-      const size_t pk = (size_t)(k - 1) * sz;  // plane k of a field
+      const uint64_t pk = plane_bytes(k, szb);  // plane k of a field
       const bool upd = ph_col && k >= 3 && k <= nz - 2;  // phase cells (phase_body.inc)
       R cPP;
       if constexpr (SWEEP_B) cPP = cZ;
@@ -686,7 +718,7 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
       const R cl = pstar > (R)1 ? (R)1 : pstar;
       const R pnew = upd ? (cl < (R)0 ? (R)0 : cl) : PP;
       const R dppv = upd ? pstar - PP : (R)0;
-      bstore(pn + pk, upd ? ooff : kNoStore, pnew);
+      bstore(boff(pn, pk), upd ? ooff : kNoStore, pnew);
       if constexpr (XF) {
         // the x faces of an interior row (equacao_campo_fase.f90:151-167 in
         // closed form: the face cells of rows inside [3,n-2] in y and z):
@@ -695,8 +727,8 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
         // the shell kernel after the sweep leaves these cells out
         constexpr uint32_t E = (uint32_t)sizeof(R);
         const bool xw = upd && i == 3, xe = upd && i == nx - 2;
-        bstore(pn + pk, xw ? ooff - E : xe ? ooff + E : kNoStore, pnew);
-        bstore(pn + pk, xw ? ooff - 2 * E : xe ? ooff + 2 * E : kNoStore, xw ? PW : PE);
+        bstore(boff(pn, pk), xw ? ooff - E : xe ? ooff + E : kNoStore, pnew);
+        bstore(boff(pn, pk), xw ? ooff - 2 * E : xe ? ooff + 2 * E : kNoStore, xw ? PW : PE);
       }
       if constexpr (SWEEP_B) {
         // solute at (i,j,k), k in [2,nz-1]
@@ -705,13 +737,13 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
         const R inc = solute_inc_w<R>(ig, mu, nc, QX, igZ, muZ, igP, muP, igM, muM, q == ks + 2,
                                     fzu_prev, D2dt, s);
         const R cnew = csum<R>(cPP, inc, hcol, k);
-        bstore(cn + pk, own ? ooff : kNoStore, cnew);
+        bstore(boff(cn, pk), own ? ooff : kNoStore, cnew);
         // x faces of the row (concentracao_soluto.f90:116-124: a face cell of
         // a row inside [2,n-1] in y and z copies its inner neighbour's new
         // value); the face kernel after the sweep leaves them out
         if constexpr (XF) {
           constexpr uint32_t E = (uint32_t)sizeof(R);
-          bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+          bstore(boff(cn, pk), own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
                  cnew);
         }
         if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
@@ -724,13 +756,13 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
           R lap = (R)0;
           if ((VAR & kVarTDiff) && s.Ddt != (R)0 && own) {
             const R TP2 = (R)2 * s.TP;
-            const R *tc = &at(Tcur + pk, ooff);
+            const R *tc = &at(boff(Tcur, pk), ooff);
             lap = (tc[1] - TP2 + tc[-1]) * s.idx2 + (tc[nx] - TP2 + tc[-nx]) * s.idy2 +
                   (tc[sz] - TP2 + tc[-(ptrdiff_t)sz]) * s.idz2;
           }
-          bstore(Tnext + pk, own ? ooff : kNoStore, s.Ddt * lap + s.fonte * hl2 * dppv + s.TP);
+          bstore(boff(Tnext, pk), own ? ooff : kNoStore, s.Ddt * lap + s.fonte * hl2 * dppv + s.TP);
         } else {
-          bstore(dpp + pk, upd ? ooff : kNoStore, dppv);
+          bstore(boff(dpp, pk), upd ? ooff : kNoStore, dppv);
         }
       } else {
         (void)Tcur; (void)Tnext; (void)dpp; (void)write_T; (void)dppv;
@@ -768,10 +800,14 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
     // no thread reads in this plane (nodes read slots q-2..q, the update
     // q-3..q-1) and every thread finished reading in plane q - 1; the next
     // plane's first barrier publishes it
-    lstore(q + 1);
+    lstore(slotr(1));
+    R *const t = sl[0];
+#pragma unroll
+    for (int d = 0; d < NSLOT - 1; ++d) sl[d] = sl[d + 1];
+    sl[NSLOT - 1] = t;
   };
   gload(q_first);
-  lstore(q_first);
+  lstore(slotr(0));
   for (int q = q_first; q < ks + 2; ++q) plane_step(q, std::false_type{});
   for (int q = ks + 2; q <= q_last; ++q) plane_step(q, std::true_type{});
   if constexpr (SWEEP_B) {
@@ -844,6 +880,7 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
   const int ks = g.k_lo + sg * g.seg;
   const int ke = min(ks + g.seg - 1, g.k_hi);
   const size_t sz = (size_t)nx * ny;
+  const uint32_t szb = (uint32_t)(sz * sizeof(R));  // bytes per plane of a field
   const int quint = kp.h_quintico;
   const LogEnt *lt = log_tab_fill<R>(smem_raw, solute_arrays_bytes<R, TL>(), tid, NT);
   if constexpr (sizeof(R) == 8) __syncthreads();  // the first node's log
@@ -876,13 +913,16 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
   // every lane loads (the ring values of non-ring lanes are clamped in-grid
   // duplicates, unused): no branch around a load
   auto fetch = [&](int q, R &p, R &c, R &p2, R &c2) {
-    const size_t pk = (size_t)(q - 1) * sz;
-    p = at(po + pk, ooff);
-    c = at(co + pk, ooff);
-    p2 = at(po + pk, roff);
-    c2 = at(co + pk, roff);
+    const uint64_t pk = plane_bytes(q, szb);
+    p = at(boff(po, pk), ooff);
+    c = at(boff(co, pk), ooff);
+    p2 = at(boff(po, pk), roff);
+    c2 = at(boff(co, pk), roff);
   };
   fetch(q_first, pq, cq, rp, rcv);
+  // node-plane buffers of planes q - 1 (bm) and q (bq), rotated over three
+  // from plane to plane (buffer of plane q = q % 3)
+  int bq = q_first % 3, bm = (q_first + 2) % 3;
   // One node plane.  The first two (ST false) only build and publish nodes;
   // from plane ks + 1 on each also updates cell plane q - 1, every lane
   // computing (lanes beyond the grid on clamped data) and only the stores
@@ -891,8 +931,8 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
     constexpr bool ST = decltype(steady)::value;
     const R pcur = pq, ccur = cq, rpc = rp, rcc = rcv;
     fetch(min(q + 1, q_last), pq, cq, rp, rcv);
-    R *gw = sig + (q % 3) * QPL;
-    R *qw = smu + (q % 3) * QPL;
+    R *gw = sig + bq * QPL;
+    R *qw = smu + bq * QPL;
     solute_node_w(pcur, ccur, quint, s, lt, igP, muP);
     gw[oc] = igP;
     qw[oc] = muP;
@@ -905,26 +945,28 @@ __device__ __forceinline__ void solute_tile_body(const Geo &g, const KParams &kp
     __syncthreads();
     if constexpr (ST) {
       const int k = q - 1;  // cell plane updated now
-      const size_t pk = (size_t)(k - 1) * sz;
+      const uint64_t pk = plane_bytes(k, szb);
       const R PP = pZ, cPP = cZ;
       const int nc = oc;
-      const R *ig = sig + (k % 3) * QPL, *mu = smu + (k % 3) * QPL;
+      const R *ig = sig + bm * QPL, *mu = smu + bm * QPL;  // plane k = q - 1
       const R D2dt = PP <= (R)0.95 ? s.Dliq2dt : s.Dsol2dt;
       const R igc = igZ, muc = muZ;
       const R inc = solute_inc_w<R>(ig, mu, nc, QX, igc, muc, igP, muP, igM, muM, q == ks + 1,
                                   fzu_prev, D2dt, s);
       const R cnew = csum<R>(cPP, inc, hcol, k);
-      bstore(cn + pk, own ? ooff : kNoStore, cnew);
+      bstore(boff(cn, pk), own ? ooff : kNoStore, cnew);
       // x faces of the row, as in march_tile's sweep B
       if constexpr (XF) {
         constexpr uint32_t E = (uint32_t)sizeof(R);
-        bstore(cn + pk, own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
+        bstore(boff(cn, pk), own && i == 2 ? ooff - E : own && i == nx - 1 ? ooff + E : kNoStore,
                cnew);
       }
       if (acc_on) macc += mass_units<R>(own ? inc : (R)0);
     }
     igM = igZ; muM = muZ; igZ = igP; muZ = muP;
     pZ = pcur; cZ = ccur;
+    bm = bq;
+    bq = bq == 2 ? 0 : bq + 1;
   };
   for (int q = q_first; q < ks + 1; ++q) step(q, std::false_type{});
   for (int q = ks + 1; q <= q_last; ++q) step(q, std::true_type{});
