@@ -132,13 +132,17 @@ def run_form(precision, steps, warmup, timing=True, autotune=True, tiling=None):
     # kernel that takes the most time -- each record costs the stream a few
     # microseconds, ~4% of an FP64 step and ~7% of an FP32 step when every
     # launch is recorded
+    # (the first warm-up step is not timed: its launches carry one-time costs,
+    # the first launch of each kernel several times its usual duration)
     warm = {}
     if warmup:
-        sim.ctx.timing(timing)
-        sim.run(warmup)
-        if timing:
-            warm = sim.ctx.timing_read()
-            sim.ctx.timing(False)
+        sim.run(1)
+        if warmup > 1:
+            sim.ctx.timing(timing)
+            sim.run(warmup - 1)
+            if timing:
+                warm = sim.ctx.timing_read()
+                sim.ctx.timing(False)
     _sync(sim)
     dom = None
     steps_kinds = [k for k in ("sweep_A", "sweep_B", "solute") if warm.get(k, (0, 0))[1] > 0]
@@ -609,9 +613,10 @@ def main(argv=None):
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
                                      "1.0, factors from a copy of known size)",
                      "traffic_kernel": traffic_key(r64, rl["kernel"], "fp64"),
-                     "limiter": "VALU issue and latency: 258 (sweep A) and 267 (phase + T) "
-                                "VALU instructions per cell at 60-63% issue, 3 waves/SIMD at "
-                                "155-162 VGPRs (FP64 register demand; a fourth wave spills)",
+                     "limiter": "VALU issue and latency: 226 (sweep A) and 239 (phase + T) "
+                                "VALU and 63-67 scalar instructions per cell, VALU issuing "
+                                "55-65% of the time, 3 waves/SIMD at 167-168 VGPRs (FP64 "
+                                "register demand; a fourth wave spills ~40 VGPRs)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
         "step_bytes_per_cell_alg": 64,

@@ -18,10 +18,13 @@ the GPU work) or against the single-instance form, bit for bit.
       sums, the ~1 ulp logarithms, one-Newton reciprocals -- lands at that
       spread; its bounds are 5x the spread.  Measured on the MI355X: drop-in
       (the reference's operation order; the device log is OCML's, not the
-      host libm's) phi <= 6.7e-16, c <= 4.2e-17 over the 2,000 steps.
+      host libm's) phi <= 6.7e-16, c <= 4.2e-17 over the 2,000 steps; fused,
+      growth deck, phi 2.9e-11 / 8.5e-11 / 1.5e-10 / 1.1e-10 and c 7.1e-13
+      ... 5.2e-12 at the four checkpoints (the spread), cooling deck phi
+      <= 7.8e-16.
   C2  256x256x528, 16 seeds r=4 on z=1, zero-mean noise 0.01 (seed 1), 3
       steps, drop-in and fused against the oracle, the same bounds
-      (measured phi 1.4e-17 drop-in, 5.6e-16 fused).
+      (measured phi 1.4e-17 drop-in, 3.3e-16 fused).
   C3  the FP32-mixed form on C1's grid against the FP64 form at 5,000 steps:
       phi max-abs <= 1e-3 and the interior solute mass within 1e-8 of the
       FP64 run, on both decks (measured on the MI355X: cooling 4.2e-11,
