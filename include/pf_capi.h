@@ -326,7 +326,12 @@ int pf_batch_diagnosticos(pf_batch *b, int32_t slot, double out[5]);
 int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget);
 /* Size classes: precision x plane-area bucket (<= 48^2, <= 96^2, <= 160^2,
  * larger); each class has a tile shape (0 = 64x8, 1 = 32x8, 2 = 32x16,
- * 3 = 16x16; default 16x16). */
+ * 3 = 16x16) per kernel kind: sweep A, sweep B (the split form's phase + T
+ * kernel) and the split form's solute kernel (default 16x16 for each).
+ * set_tiling takes one shape for every kind or PF_BATCH_TILES(a, b, s);
+ * get_tiling returns the one shape when the three agree, else the packed
+ * form. */
+#define PF_BATCH_TILES(a, b, s) (0x1000 | (a) | ((b) << 4) | ((s) << 8))
 int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny);
 int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape);
 int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape);
@@ -334,10 +339,10 @@ int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape);
  * (default 16384; segments are at least 16 planes): each instance gets its
  * share by cell count. */
 int pf_batch_set_group_target(pf_batch *b, int32_t target_groups);
-/* Times the whole batch's sweeps with every tile shape for size_class (-1:
- * each class present in turn) and keeps the fastest (no step is taken; the
- * T_new reported for the instances is not kept until their next step).
- * Synchronous. */
+/* Times every tile shape for size_class (-1: each class present in turn)
+ * and kernel kind, over the whole batch's launches of that kind, and keeps
+ * the fastest per kind (no step is taken; the T_new reported for the
+ * instances is not kept until their next step).  Synchronous. */
 int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps);
 /* Sweep B of the batched steps as one fused kernel (on = 0) or split into
  * a phase + T kernel and a solute kernel over the same tiles (on = 1, the

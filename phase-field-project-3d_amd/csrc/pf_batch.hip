@@ -5,8 +5,10 @@
 // the REAL(4) time, buffer roles, steps left) on the device.  A step is
 //   k_batch_begin          per instance: principal.f90:96-104 + the kernel
 //                          scalars of the sweeps (one thread per instance)
-//   (nv-1) x [sweep A, phase decoration]      one launch per tile shape
-//   sweep B, phase + solute decorations       (work-group -> tile table)
+//   (nv-1) x [sweep A, phase decoration]      one launch per precision and
+//   sweep B: phase + T, then the solute       tile shape (work-group -> tile
+//     with the phase decoration               table); a decoration launch
+//   solute faces, mass corrections due        covers both precisions
 //   k_batch_end            per instance: auxiliar M0 (principal.f90:157),
 //                          buffer roles, steps left
 // so K steps need no host work and are captured once in a hipGraph that is
@@ -45,7 +47,8 @@ constexpr int kShapeY[kNShapes] = {8, 8, 16, 16};
 // Device-resident state of one instance (arithmetic type R).
 template <typename R>
 struct InstDev {
-  Geo ga, gb;           // sweep geometry (k ranges, tiles, z segments)
+  Geo ga, gb, gs;       // geometry (k ranges, tiles, z segments) of sweep A,
+                        // sweep B and the split sweep B's solute kernel
   KParams kt, k;        // temperatura scalars (TP of the previous step), sweep scalars
   SC<R> s;              // derived sweep scalars of this step
   R *phi[2], *c[2], *T[2];
@@ -59,6 +62,11 @@ struct InstDev {
   MassLedger m;         // the instance's solute-mass ledger (pf_mass.h)
 };
 
+// the host writes the three geometries with one copy
+static_assert(offsetof(InstDev<double>, gs) == offsetof(InstDev<double>, ga) + 2 * sizeof(Geo) &&
+                  offsetof(InstDev<float>, gs) == offsetof(InstDev<float>, ga) + 2 * sizeof(Geo),
+              "ga, gb, gs adjacent");
+
 // The instance corrects its solute mass after the current step.
 template <typename R>
 __device__ __forceinline__ bool mass_due(const InstDev<R> &I) {
@@ -68,8 +76,19 @@ __device__ __forceinline__ bool mass_due(const InstDev<R> &I) {
 struct TileRef {
   int inst, bx, by, sg;
 };
+
+// A field pointer read from an instance record, typed as a pointer to
+// global memory: the compiler then accesses the field with global memory
+// instructions (a plain pointer loaded from memory is generic, and flat
+// loads share their wait counter with the LDS accesses of the z-march).
+template <typename T>
+__device__ __forceinline__ T *gfield(T *const &field) {
+  __attribute__((address_space(1))) T *g;
+  __builtin_memcpy(&g, &field, sizeof g);
+  return (T *)g;
+}
 struct Chunk {
-  int inst;
+  int inst, prec;
   long long first;
 };
 constexpr int kChunkCells = 1024;
@@ -80,13 +99,10 @@ constexpr int kChunkCells = 1024;
 // step -- a compact array the correction kernels read instead of every
 // instance's record
 template <typename R>
-__global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n, int *__restrict__ due) {
+__device__ __forceinline__ void batch_begin(InstDev<R> &I, int *due) {
 #pragma clang fp contract(off)
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  InstDev<R> &I = inst[q];
   if (!I.active) {
-    if (due) due[q] = 0;
+    if (due) *due = 0;
     return;
   }
   I.k.it += 1;
@@ -96,14 +112,11 @@ __global__ void k_batch_begin(InstDev<R> *__restrict__ inst, int n, int *__restr
   if (I.taxa > 0.0) I.k.TP = I.T0 - I.taxa * I.k.dt * (double)it;  // :104
   equilibrium(I.k.TP, I.Tm, I.Ke, I.Me, I.k.Cs_eq, I.k.Cl_eq);
   I.s = make_sc<R>(I.k);
-  if (due) due[q] = mass_due(I) ? 1 : 0;
+  if (due) *due = mass_due(I) ? 1 : 0;
 }
 
 template <typename R>
-__global__ void k_batch_end(InstDev<R> *__restrict__ inst, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  InstDev<R> &I = inst[q];
+__device__ __forceinline__ void batch_end(InstDev<R> &I) {
   if (!I.active) return;
   // auxiliar.f90 (principal.f90:157): M0 for the next step
   I.k.M0 = auxiliar_m0(I.k.TP, I.k.Cl_eq, I.k.Cs_eq, I.k.R, I.k.Vm, I.k.E0, I.k.W0, I.Sigma,
@@ -114,6 +127,36 @@ __global__ void k_batch_end(InstDev<R> *__restrict__ inst, int n) {
   I.steps_done += 1;
   I.steps_left -= 1;
   if (I.steps_left <= 0) I.active = 0;
+}
+
+// One thread per instance: the FP64 instances in the first b64 blocks of 64
+// threads, the FP32 ones in the blocks after (so that no wave runs both
+// precisions' code one after the other).
+__global__ void __launch_bounds__(64)
+    k_batch_begin(InstDev<double> *__restrict__ inst64, int n64, int *__restrict__ due64,
+                  InstDev<float> *__restrict__ inst32, int n32, int *__restrict__ due32,
+                  int b64) {
+  const int blk = blockIdx.x;
+  if (blk < b64) {
+    const int q = blk * 64 + threadIdx.x;
+    if (q < n64) batch_begin<double>(inst64[q], due64 ? due64 + q : nullptr);
+  } else {
+    const int q = (blk - b64) * 64 + threadIdx.x;
+    if (q < n32) batch_begin<float>(inst32[q], due32 ? due32 + q : nullptr);
+  }
+}
+
+__global__ void __launch_bounds__(64)
+    k_batch_end(InstDev<double> *__restrict__ inst64, int n64, InstDev<float> *__restrict__ inst32,
+                int n32, int b64) {
+  const int blk = blockIdx.x;
+  if (blk < b64) {
+    const int q = blk * 64 + threadIdx.x;
+    if (q < n64) batch_end<double>(inst64[q]);
+  } else {
+    const int q = (blk - b64) * 64 + threadIdx.x;
+    if (q < n32) batch_end<float>(inst32[q]);
+  }
 }
 
 // ---- batched sweeps and decorations ----------------------------------------
@@ -133,61 +176,100 @@ __global__ void __launch_bounds__(TL::NT)
   const InstDev<R> &I = inst[d.inst];
   if (!I.active) return;
   const int pin = I.po ^ ((sweep - 1) & 1), pout = pin ^ 1;
-  if constexpr (MODE == kSweepB)
-    march_tile<R, kSweepB, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
-                                       I.c[I.co ^ 1], I.T[I.to], I.T[I.to ^ 1], nullptr, 1,
+  // the instance's scalars as values (not references into the record,
+  // which the compiler would read again plane after plane)
+  const KParams kp = I.k;
+  const SC<R> sc = I.s;
+  const R *po = gfield(I.phi[pin]), *co = gfield(I.c[I.co]);
+  R *pn = gfield(I.phi[pout]);
+  if constexpr (MODE == kSweepB) {
+    const Geo g = I.gb;
+    march_tile<R, kSweepB, TL, VAR>(g, kp, sc, po, co, pn, gfield(I.c[I.co ^ 1]),
+                                       gfield(I.T[I.to]), gfield(I.T[I.to ^ 1]), nullptr, 1,
                                        ledger && I.mass_every > 0 ? &inst[d.inst].m.acc : nullptr,
                                        d.bx, d.by, d.sg);
-  else if constexpr (MODE == kSweepPT)
-    march_tile<R, kSweepPT, TL, VAR>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
-                                        nullptr, I.T[I.to], I.T[I.to ^ 1], nullptr, 1, nullptr,
-                                        d.bx, d.by, d.sg);
-  else
-    march_tile<R, kSweepA, TL, VAR>(I.ga, I.k, I.s, I.phi[pin], I.c[I.co], I.phi[pout],
-                                       nullptr, nullptr, nullptr, nullptr, 0, nullptr, d.bx,
-                                       d.by, d.sg);
+  } else if constexpr (MODE == kSweepPT) {
+    const Geo g = I.gb;
+    march_tile<R, kSweepPT, TL, VAR>(g, kp, sc, po, co, pn, nullptr, gfield(I.T[I.to]),
+                                        gfield(I.T[I.to ^ 1]), nullptr, 1, nullptr, d.bx, d.by,
+                                        d.sg);
+  } else {
+    const Geo g = I.ga;
+    march_tile<R, kSweepA, TL, VAR>(g, kp, sc, po, co, pn, nullptr, nullptr, nullptr, nullptr, 0,
+                                       nullptr, d.bx, d.by, d.sg);
+  }
 }
 
-// The solute update of the split sweep B over the same tile table: reads
-// the phi and c sweep B reads, writes c (solute_tile, the single-instance
-// split form's body).
+// Phase decoration (which = 0, shell of width 2, from the swept phi) or
+// solute faces (which = 1, width 1) of one chunk of an instance's shell; the
+// x-face cells of the interior rows are written by the sweeps.
+template <typename R>
+__device__ __forceinline__ void deco_chunk(const Chunk &ch, const InstDev<R> &I, int which,
+                                           int sweep, int tid, int nt) {
+  const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
+  const int w = which == 0 ? 2 : 1;
+  const long long cnt = shell_count(nx, ny, nz, w, false);
+  const long long end = ch.first + kChunkCells < cnt ? ch.first + kChunkCells : cnt;
+  const int pin = I.po ^ ((sweep - 1) & 1);
+  for (long long t = ch.first + tid; t < end; t += nt) {
+    int i, j, k;
+    shell_flat(t, nx, ny, nz, w, i, j, k, false);
+    if (which == 0)
+      phase_deco_cell(i, j, k, nx, ny, nz, (const R *)gfield(I.phi[pin]), gfield(I.phi[pin ^ 1]));
+    else
+      solute_deco_cell(i, j, k, nx, ny, nz, (const R *)gfield(I.c[I.co]), gfield(I.c[I.co ^ 1]));
+  }
+}
+
+// The solute update of the split sweep B over its own tile table: reads the
+// phi and c sweep B reads, writes c (solute_tile, the single-instance split
+// form's body).  The ndeco work-groups after the tiles' decorate the swept
+// phi of the instances of this precision (chunks, which = 0): the solute
+// reads only the pre-sweep phi and c, so the two are independent and the
+// decoration needs no launch of its own.
 template <typename R, typename TL>
 __global__ void __launch_bounds__(TL::NT)
     k_solute_batch(const TileRef *__restrict__ tiles, int ntiles, int per_xcd,
-                   InstDev<R> *__restrict__ inst, int sweep, int ledger) {
+                   InstDev<R> *__restrict__ inst, int sweep, int ledger,
+                   const Chunk *__restrict__ chunks, int ndeco) {
   const int b = blockIdx.x;
+  if (b >= 8 * per_xcd) {
+    const int q = b - 8 * per_xcd;
+    if (q >= ndeco) return;
+    const Chunk ch = chunks[q];
+    const InstDev<R> &I = inst[ch.inst];
+    if (I.active) deco_chunk<R>(ch, I, 0, sweep, threadIdx.y * TL::TX + threadIdx.x, TL::NT);
+    return;
+  }
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= ntiles) return;
   const TileRef d = tiles[t];
   const InstDev<R> &I = inst[d.inst];
   if (!I.active) return;
   const int pin = I.po ^ ((sweep - 1) & 1);
-  solute_tile<R, TL>(I.gb, I.k, I.s, I.phi[pin], I.c[I.co], I.c[I.co ^ 1],
+  const Geo g = I.gs;
+  const KParams kp = I.k;
+  const SC<R> sc = I.s;
+  solute_tile<R, TL>(g, kp, sc, gfield(I.phi[pin]), gfield(I.c[I.co]), gfield(I.c[I.co ^ 1]),
                      ledger && I.mass_every > 0 ? &inst[d.inst].m.acc : nullptr, d.bx, d.by,
                      d.sg);
 }
 
 // which = 0: phase decoration of sweep `sweep` (shell of width 2);
-// which = 1: solute faces of the step (width 1).
-template <typename R>
+// which = 1: solute faces of the step (width 1); the chunks of both
+// precisions in one launch (a chunk's precision picks the instance array).
 __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ chunks, int nchunk,
-                                                    const InstDev<R> *__restrict__ inst,
+                                                    const InstDev<double> *__restrict__ inst64,
+                                                    const InstDev<float> *__restrict__ inst32,
                                                     int which, int sweep) {
   if ((int)blockIdx.x >= nchunk) return;
   const Chunk ch = chunks[blockIdx.x];
-  const InstDev<R> &I = inst[ch.inst];
-  if (!I.active) return;
-  const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
-  const int w = which == 0 ? 2 : 1;
-  // the x-face cells of the interior rows are written by the sweeps
-  const long long cnt = shell_count(nx, ny, nz, w, false);
-  const long long end = ch.first + kChunkCells < cnt ? ch.first + kChunkCells : cnt;
-  const int pin = I.po ^ ((sweep - 1) & 1);
-  for (long long t = ch.first + threadIdx.x; t < end; t += 256) {
-    int i, j, k;
-    shell_flat(t, nx, ny, nz, w, i, j, k, false);
-    if (which == 0) phase_deco_cell(i, j, k, nx, ny, nz, (const R *)I.phi[pin], I.phi[pin ^ 1]);
-    else solute_deco_cell(i, j, k, nx, ny, nz, (const R *)I.c[I.co], I.c[I.co ^ 1]);
+  if (ch.prec == PF_FP64) {
+    const InstDev<double> &I = inst64[ch.inst];
+    if (I.active) deco_chunk<double>(ch, I, which, sweep, threadIdx.x, 256);
+  } else {
+    const InstDev<float> &I = inst32[ch.inst];
+    if (I.active) deco_chunk<float>(ch, I, which, sweep, threadIdx.x, 256);
   }
 }
 
@@ -196,8 +278,8 @@ __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ ch
 constexpr int kSweepSolute = 3;
 
 template <typename R, typename TL, int VAR>
-void launch_batch_march_v(int mode, const TileRef *tiles, int ntiles, InstDev<R> *inst,
-                          int sweep, int ledger, hipStream_t st) {
+void launch_batch_v(int mode, const TileRef *tiles, int ntiles, InstDev<R> *inst, int sweep,
+                    int ledger, const Chunk *chunks, int ndeco, hipStream_t st) {
   const int per_xcd = (ntiles + 7) / 8;
   const dim3 blk(TL::TX, TL::TY, 1), grd(8 * per_xcd, 1, 1);
   const size_t lds_b = march_lds_bytes<R, TL>(kSweepB), lds_pt = march_lds_bytes<R, TL>(kSweepPT);
@@ -209,53 +291,46 @@ void launch_batch_march_v(int mode, const TileRef *tiles, int ntiles, InstDev<R>
     hipLaunchKernelGGL((k_march_batch<R, kSweepPT, TL, VAR>), grd, blk, lds_pt, st, tiles, ntiles,
                        per_xcd, inst, sweep, 0);
   else if (mode == kSweepSolute)
-    hipLaunchKernelGGL((k_solute_batch<R, TL>), grd, blk, lds_s, st, tiles, ntiles, per_xcd, inst,
-                       sweep, ledger);
+    hipLaunchKernelGGL((k_solute_batch<R, TL>), dim3(8 * per_xcd + ndeco), blk, lds_s, st, tiles,
+                       ntiles, per_xcd, inst, sweep, ledger, chunks, ndeco);
   else
     hipLaunchKernelGGL((k_march_batch<R, kSweepA, TL, VAR & kVarSmooth>), grd, blk, lds_a, st,
                        tiles, ntiles, per_xcd, inst, sweep, 0);
 }
 
-// var: the optional paths (kVar*) some instance of the group needs (the
-// kernels without a path must not see an instance that needs it); ledger:
-// sweep B (or the split form's solute) adds its solute increments to the
-// instances' ledgers (off for the autotuner's timing launches)
-template <typename R, typename TL>
-hipError_t launch_batch_march(int mode, int var, const TileRef *tiles, int ntiles,
-                              InstDev<R> *inst, int sweep, int ledger, hipStream_t st) {
-  if (ntiles == 0) return hipSuccess;
-  switch (var) {
-    case 0: launch_batch_march_v<R, TL, 0>(mode, tiles, ntiles, inst, sweep, ledger, st); break;
-    case kVarSmooth:
-      launch_batch_march_v<R, TL, kVarSmooth>(mode, tiles, ntiles, inst, sweep, ledger, st);
-      break;
-    case kVarTDiff:
-      launch_batch_march_v<R, TL, kVarTDiff>(mode, tiles, ntiles, inst, sweep, ledger, st);
-      break;
-    default:
-      launch_batch_march_v<R, TL, kVarAll>(mode, tiles, ntiles, inst, sweep, ledger, st);
-  }
-  return hipGetLastError();
-}
-
-// The launches of one group: sweep A, or sweep B fused or split (the phase
-// + T kernel, then the solute kernel over the same tiles).
+// One launch of kernel kind `mode` (kSweepA, kSweepB, kSweepPT or
+// kSweepSolute) over a tile table of one tile shape.  var: the optional
+// paths (kVar*) some instance of the group needs (the kernels without a path
+// must not see an instance that needs it); ledger: sweep B (or the split
+// form's solute) adds its solute increments to the instances' ledgers (off
+// for the autotuner's timing launches); the solute kernel also decorates
+// `ndeco` chunks of swept phi.
 template <typename R>
-hipError_t launch_batch_shape(int shape, bool sweep_b, bool split, int var, const TileRef *tiles,
-                              int ntiles, InstDev<R> *inst, int sweep, int ledger,
-                              hipStream_t st) {
-  auto one = [&](int mode) -> hipError_t {
-    switch (shape) {
-      case 1: return launch_batch_march<R, Tile32x8>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
-      case 2: return launch_batch_march<R, Tile32x16>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
-      case 3: return launch_batch_march<R, Tile16x16>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
-      default: return launch_batch_march<R, Tile64x8>(mode, var, tiles, ntiles, inst, sweep, ledger, st);
+hipError_t launch_batch_group(int shape, int mode, int var, const TileRef *tiles, int ntiles,
+                              InstDev<R> *inst, int sweep, int ledger, const Chunk *chunks,
+                              int ndeco, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  auto go = [&](auto tl) {
+    using TL = decltype(tl);
+    switch (var) {
+      case 0: launch_batch_v<R, TL, 0>(mode, tiles, ntiles, inst, sweep, ledger, chunks, ndeco, st); break;
+      case kVarSmooth:
+        launch_batch_v<R, TL, kVarSmooth>(mode, tiles, ntiles, inst, sweep, ledger, chunks, ndeco, st);
+        break;
+      case kVarTDiff:
+        launch_batch_v<R, TL, kVarTDiff>(mode, tiles, ntiles, inst, sweep, ledger, chunks, ndeco, st);
+        break;
+      default:
+        launch_batch_v<R, TL, kVarAll>(mode, tiles, ntiles, inst, sweep, ledger, chunks, ndeco, st);
     }
   };
-  if (!sweep_b) return one(kSweepA);
-  if (!split) return one(kSweepB);
-  const hipError_t e = one(kSweepPT);
-  return e != hipSuccess ? e : one(kSweepSolute);
+  switch (shape) {
+    case 1: go(Tile32x8{}); break;
+    case 2: go(Tile32x16{}); break;
+    case 3: go(Tile16x16{}); break;
+    default: go(Tile64x8{});
+  }
+  return hipGetLastError();
 }
 
 // ---- solute-mass correction of the instances due this step (pf_mass.h) -----
@@ -273,7 +348,7 @@ __global__ void __launch_bounds__(256) k_bmass_interior(const InstDev<R> *__rest
   for (int q = 0; q < n; ++q) {
     if (!due[q]) continue;  // uniform over the block
     const InstDev<R> &I = inst[q];
-    const double t = block_sum256(mass_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz,
+    const double t = block_sum256(mass_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz,
                                             (int)blockIdx.x, I.k.c_off), sh);
     if (threadIdx.x == 0) part[(size_t)q * kMassBlocks + blockIdx.x] = t;
   }
@@ -301,7 +376,7 @@ __global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restric
   for (int q = 0; q < n; ++q) {
     if (!due[q]) continue;
     const InstDev<R> &I = inst[q];
-    mass_scale_rows(I.c[I.co ^ 1], I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, I.m.f,
+    mass_scale_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, I.m.f,
                     (int)blockIdx.x, I.k.c_off);
   }
 }
@@ -316,7 +391,7 @@ __global__ void __launch_bounds__(256) k_bmass_refresh(const InstDev<R> *__restr
     if (!due[q]) continue;
     const InstDev<R> &I = inst[q];
     const int nx = I.k.nx, ny = I.k.ny, nz = I.k.nz;
-    R *c = I.c[I.co ^ 1];
+    R *c = gfield(I.c[I.co ^ 1]);
     int i, j, k;
     for (long long t = blockIdx.x * 256LL + threadIdx.x; shell_flat(t, nx, ny, nz, 1, i, j, k);
          t += (long long)kRefreshBlocks * 256) {
@@ -352,6 +427,8 @@ int size_class(int prec, int nx, int ny) {
   return prec * 4 + bucket;
 }
 constexpr int kNClasses = 8;
+// kernel kinds with a tile shape of their own
+enum : int { kKindA = 0, kKindB = 1, kKindS = 2, kNKinds = 3 };
 
 }  // namespace
 }  // namespace pf
@@ -383,9 +460,11 @@ struct pf_batch {
   void *inst_dev[2] = {nullptr, nullptr};
   int inst_cap[2] = {0, 0};
   std::vector<int> slot_of[2];  // per precision: device index -> slot
-  // tile shape per size class; work-groups per sweep the z split aims for
-  // over the whole batch (each instance gets its share by cell count)
-  int cls_shape[pf::kNClasses];
+  // tile shape per size class and kernel kind (kKindA: sweep A, kKindB:
+  // sweep B or the split form's phase + T kernel, kKindS: the split form's
+  // solute kernel); work-groups per sweep the z split aims for over the
+  // whole batch (each instance gets its share by cell count)
+  int cls_shape[pf::kNClasses][pf::kNKinds];
   int group_target = 16384;
   // sweep B split into the phase + T kernel and the solute kernel (as the
   // single-instance split form; same bits)
@@ -398,19 +477,18 @@ struct pf_batch {
   pf::Chunk *chunks_dev = nullptr;
   size_t chunks_cap = 0;
   struct Group {
-    int prec, shape;
-    bool sweep_b;
+    int prec, shape, kind;
     int var;  // kernel variant: the optional paths an instance of the group needs
     size_t off;
     int n;
+    int ndeco;  // solute groups: phase-decoration chunks its launch also runs
+    size_t deco_off;
   };
   std::vector<Group> groups;
-  struct Deco {
-    int prec, which;
-    size_t off;
-    int n;
-  };
-  std::vector<Deco> decos;
+  // chunk tables of the shell work: [which] = 0 phase decoration, 1 solute
+  // faces; the FP64 instances' chunks first, then the FP32 ones
+  size_t deco_off[2] = {0, 0};
+  int deco_n[2] = {0, 0}, deco_n64[2] = {0, 0};
   // captured graph of graph_steps steps
   int graph_steps = 16;
   hipGraphExec_t graph = nullptr;
@@ -493,63 +571,82 @@ pf::InstDev<R> make_inst(const pf_batch_slot &s) {
 template <typename R>
 void set_geo(pf::InstDev<R> &I, const pf_batch &b, const pf_batch_slot &s) {
   const int cls = pf::size_class(s.prec, s.nx, s.ny);
-  const int shape = b.cls_shape[cls];
   // z segments: the instance's share (by cells) of group_target work-groups
   // over the whole batch; a batch of many instances marches whole columns,
   // which saves the 4 warm-up planes per segment
-  const int plane_tiles = ((s.nx - 2 + pf::kShapeX[shape] - 1) / pf::kShapeX[shape]) *
-                          ((s.ny - 2 + pf::kShapeY[shape] - 1) / pf::kShapeY[shape]);
   const double share = b.total_cells ? (double)s.n / (double)b.total_cells : 1.0;
   const int want = std::max(1, (int)(share * b.group_target + 0.5));
-  const int nseg = std::max(1, (want + plane_tiles - 1) / plane_tiles);
-  const int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
-  I.ga = pf::batch_geo(s.nx, s.ny, s.nz, false, shape, seg);
-  I.gb = pf::batch_geo(s.nx, s.ny, s.nz, true, shape, seg);
+  auto geo = [&](int kind, bool sweep_b) {
+    const int shape = b.cls_shape[cls][kind];
+    const int plane_tiles = ((s.nx - 2 + pf::kShapeX[shape] - 1) / pf::kShapeX[shape]) *
+                            ((s.ny - 2 + pf::kShapeY[shape] - 1) / pf::kShapeY[shape]);
+    const int nseg = std::max(1, (want + plane_tiles - 1) / plane_tiles);
+    const int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
+    return pf::batch_geo(s.nx, s.ny, s.nz, sweep_b, shape, seg);
+  };
+  I.ga = geo(pf::kKindA, false);
+  I.gb = geo(pf::kKindB, true);
+  I.gs = geo(pf::kKindS, true);
 }
 
 int rebuild(pf_batch *b) {
   b->total_cells = 0;
   for (const auto &s : b->slots)
     if (s.used) b->total_cells += s.n;
-  // tile tables: one group per (precision, shape, sweep)
+  // tile tables: one group per (precision, kernel kind, shape); the
+  // solute kind only when sweep B is split
   std::vector<pf::TileRef> tiles;
   std::vector<pf::Chunk> chunks;
   b->groups.clear();
-  b->decos.clear();
+  const int nkinds = b->split_b ? pf::kNKinds : pf::kKindS;
   for (int prec = 0; prec < 2; ++prec) {
     const auto &ids = b->slot_of[prec];
     if (ids.empty()) continue;
     // read geometry from the slots (host mirror) in device-index order
-    for (int sweep_b = 0; sweep_b < 2; ++sweep_b)
+    for (int kind = 0; kind < nkinds; ++kind)
       for (int shape = 0; shape < pf::kNShapes; ++shape) {
         const size_t off = tiles.size();
         int var = 0;
         for (size_t q = 0; q < ids.size(); ++q) {
           const pf_batch_slot &s = b->slots[ids[q]];
           const int cls = pf::size_class(s.prec, s.nx, s.ny);
-          if (b->cls_shape[cls] != shape) continue;
+          if (b->cls_shape[cls][kind] != shape) continue;
           if (s.p.AAA != 0.0 && s.p.ruido_tipo == PF_RUIDO_SMOOTH) var |= pf::kVarSmooth;
           if (s.p.D != 0.0) var |= pf::kVarTDiff;
           pf::InstDev<double> tmp;  // geometry only
           set_geo(tmp, *b, s);
-          const pf::Geo &g = sweep_b ? tmp.gb : tmp.ga;
+          const pf::Geo &g = kind == pf::kKindA ? tmp.ga : kind == pf::kKindB ? tmp.gb : tmp.gs;
           for (int sg = 0; sg < g.nseg; ++sg)
             for (int by = 0; by < g.nby; ++by)
               for (int bx = 0; bx < g.nbx; ++bx) tiles.push_back({(int)q, bx, by, sg});
         }
         const int n = (int)(tiles.size() - off);
-        if (n) b->groups.push_back({prec, shape, sweep_b != 0, var, off, n});
+        if (n) b->groups.push_back({prec, shape, kind, var, off, n, 0, 0});
       }
-    for (int which = 0; which < 2; ++which) {
+  }
+  for (int which = 0; which < 2; ++which) {
+    b->deco_off[which] = chunks.size();
+    for (int prec = 0; prec < 2; ++prec) {
+      const auto &ids = b->slot_of[prec];
       const size_t off = chunks.size();
       for (size_t q = 0; q < ids.size(); ++q) {
         const pf_batch_slot &s = b->slots[ids[q]];
         const long long cnt = pf::shell_count(s.nx, s.ny, s.nz, which == 0 ? 2 : 1, false);
-        for (long long f = 0; f < cnt; f += pf::kChunkCells) chunks.push_back({(int)q, f});
+        for (long long f = 0; f < cnt; f += pf::kChunkCells) chunks.push_back({(int)q, prec, f});
       }
-      b->decos.push_back({prec, which, off, (int)(chunks.size() - off)});
+      if (prec == PF_FP64) b->deco_n64[which] = (int)(chunks.size() - off);
     }
+    b->deco_n[which] = (int)(chunks.size() - b->deco_off[which]);
   }
+  // the split form's last phase decoration runs in the first solute launch
+  // of each precision
+  for (int prec = 0; prec < 2 && b->split_b; ++prec)
+    for (auto &g : b->groups)
+      if (g.prec == prec && g.kind == pf::kKindS) {
+        g.ndeco = prec == PF_FP64 ? b->deco_n64[0] : b->deco_n[0] - b->deco_n64[0];
+        g.deco_off = b->deco_off[0] + (prec == PF_FP64 ? 0 : b->deco_n64[0]);
+        break;
+      }
   if (tiles.size() > b->tiles_cap) {
     if (b->tiles_dev) HIP_OK(hipFree(b->tiles_dev));
     b->tiles_dev = nullptr;
@@ -596,13 +693,13 @@ int rebuild(pf_batch *b) {
         pf::InstDev<double> tmp;
         set_geo(tmp, *b, s);
         char *base = (char *)b->inst_dev[prec] + q * sizeof(pf::InstDev<double>);
-        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<double>, ga), &tmp.ga, 2 * sizeof(pf::Geo),
+        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<double>, ga), &tmp.ga, 3 * sizeof(pf::Geo),
                               hipMemcpyHostToDevice, b->stream));
       } else {
         pf::InstDev<float> tmp;
         set_geo(tmp, *b, s);
         char *base = (char *)b->inst_dev[prec] + q * sizeof(pf::InstDev<float>);
-        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<float>, ga), &tmp.ga, 2 * sizeof(pf::Geo),
+        HIP_OK(hipMemcpyAsync(base + offsetof(pf::InstDev<float>, ga), &tmp.ga, 3 * sizeof(pf::Geo),
                               hipMemcpyHostToDevice, b->stream));
       }
     }
@@ -616,60 +713,69 @@ int rebuild(pf_batch *b) {
   return PF_OK;
 }
 
+// One group's launch (sweep `sweep` of the step); `deco`: the solute
+// launch also runs its phase-decoration chunks.
+hipError_t launch_group(pf_batch *b, const pf_batch::Group &g, int sweep, int ledger, bool deco) {
+  const int mode = g.kind == pf::kKindA   ? pf::kSweepA
+                   : g.kind == pf::kKindS ? pf::kSweepSolute
+                   : b->split_b           ? pf::kSweepPT
+                                          : pf::kSweepB;
+  const pf::Chunk *ch = b->chunks_dev + g.deco_off;
+  const int nd = deco ? g.ndeco : 0;
+  if (g.prec == PF_FP64)
+    return pf::launch_batch_group<double>(g.shape, mode, g.var, b->tiles_dev + g.off, g.n,
+                                          (pf::InstDev<double> *)b->inst_dev[0], sweep, ledger,
+                                          ch, nd, b->stream);
+  return pf::launch_batch_group<float>(g.shape, mode, g.var, b->tiles_dev + g.off, g.n,
+                                       (pf::InstDev<float> *)b->inst_dev[1], sweep, ledger, ch,
+                                       nd, b->stream);
+}
+
 // Enqueues one step of every active instance.
 int enqueue_step(pf_batch *b) {
   hipStream_t st = b->stream;
-  for (int prec = 0; prec < 2; ++prec) {
-    const int n = (int)b->slot_of[prec].size();
-    if (!n) continue;
-    const unsigned nb = (unsigned)((n + 63) / 64);
-    if (prec == PF_FP64)
-      hipLaunchKernelGGL(pf::k_batch_begin<double>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<double> *)b->inst_dev[prec], n, b->mass_due[prec]);
-    else
-      hipLaunchKernelGGL(pf::k_batch_begin<float>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<float> *)b->inst_dev[prec], n, b->mass_due[prec]);
-    HIP_OK(hipGetLastError());
-  }
-  auto sweeps = [&](bool sweep_b, int sweep) -> int {
+  auto *i64 = (pf::InstDev<double> *)b->inst_dev[0];
+  auto *i32 = (pf::InstDev<float> *)b->inst_dev[1];
+  const int n64 = (int)b->slot_of[0].size(), n32 = (int)b->slot_of[1].size();
+  const int b64 = (n64 + 63) / 64;
+  const unsigned nbi = (unsigned)(b64 + (n32 + 63) / 64);
+  hipLaunchKernelGGL(pf::k_batch_begin, dim3(nbi), dim3(64), 0, st, i64, n64, b->mass_due[0], i32,
+                     n32, b->mass_due[1], b64);
+  HIP_OK(hipGetLastError());
+  auto sweeps = [&](int kind, int sweep, int prec = -1) -> int {
     for (const auto &g : b->groups) {
-      if (g.sweep_b != sweep_b) continue;
-      hipError_t e;
-      if (g.prec == PF_FP64)
-        e = pf::launch_batch_shape<double>(g.shape, sweep_b, b->split_b, g.var,
-                                           b->tiles_dev + g.off, g.n,
-                                           (pf::InstDev<double> *)b->inst_dev[0], sweep, 1, st);
-      else
-        e = pf::launch_batch_shape<float>(g.shape, sweep_b, b->split_b, g.var,
-                                          b->tiles_dev + g.off, g.n,
-                                          (pf::InstDev<float> *)b->inst_dev[1], sweep, 1, st);
+      if (g.kind != kind || (prec >= 0 && g.prec != prec)) continue;
+      const hipError_t e = launch_group(b, g, sweep, 1, true);
       if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
     }
     return PF_OK;
   };
   auto decos = [&](int which, int sweep) -> int {
-    for (const auto &d : b->decos) {
-      if (d.which != which || d.n == 0) continue;
-      if (d.prec == PF_FP64)
-        hipLaunchKernelGGL(pf::k_deco_batch<double>, dim3(d.n), dim3(256), 0, st,
-                           b->chunks_dev + d.off, d.n,
-                           (const pf::InstDev<double> *)b->inst_dev[0], which, sweep);
-      else
-        hipLaunchKernelGGL(pf::k_deco_batch<float>, dim3(d.n), dim3(256), 0, st,
-                           b->chunks_dev + d.off, d.n,
-                           (const pf::InstDev<float> *)b->inst_dev[1], which, sweep);
-      HIP_OK(hipGetLastError());
-    }
+    if (b->deco_n[which] == 0) return PF_OK;
+    hipLaunchKernelGGL(pf::k_deco_batch, dim3(b->deco_n[which]), dim3(256), 0, st,
+                       b->chunks_dev + b->deco_off[which], b->deco_n[which], i64, i32, which,
+                       sweep);
+    HIP_OK(hipGetLastError());
     return PF_OK;
   };
   int rc;
   const int nv = b->nv < 1 ? 1 : b->nv;
   for (int sw = 1; sw < nv; ++sw) {
-    if ((rc = sweeps(false, sw)) != PF_OK) return rc;
+    if ((rc = sweeps(pf::kKindA, sw)) != PF_OK) return rc;
     if ((rc = decos(0, sw)) != PF_OK) return rc;
   }
-  if ((rc = sweeps(true, nv)) != PF_OK) return rc;
-  if ((rc = decos(0, nv)) != PF_OK) return rc;
+  // sweep B, per precision: the phase + T kernels before the solute
+  // kernels (which read the same phi and c, partly still in the caches, and
+  // whose launches also decorate the swept phi)
+  if (b->split_b) {
+    for (int prec = 0; prec < 2; ++prec) {
+      if ((rc = sweeps(pf::kKindB, nv, prec)) != PF_OK) return rc;
+      if ((rc = sweeps(pf::kKindS, nv, prec)) != PF_OK) return rc;
+    }
+  } else {
+    if ((rc = sweeps(pf::kKindB, nv)) != PF_OK) return rc;
+    if ((rc = decos(0, nv)) != PF_OK) return rc;
+  }
   if ((rc = decos(1, nv)) != PF_OK) return rc;
   // solute-mass corrections of the instances due this step (after the
   // step's face copy, as in the single-instance loop)
@@ -702,18 +808,8 @@ int enqueue_step(pf_batch *b) {
     }
     HIP_OK(hipGetLastError());
   }
-  for (int prec = 0; prec < 2; ++prec) {
-    const int n = (int)b->slot_of[prec].size();
-    if (!n) continue;
-    const unsigned nb = (unsigned)((n + 63) / 64);
-    if (prec == PF_FP64)
-      hipLaunchKernelGGL(pf::k_batch_end<double>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<double> *)b->inst_dev[prec], n);
-    else
-      hipLaunchKernelGGL(pf::k_batch_end<float>, dim3(nb), dim3(64), 0, st,
-                         (pf::InstDev<float> *)b->inst_dev[prec], n);
-    HIP_OK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(pf::k_batch_end, dim3(nbi), dim3(64), 0, st, i64, n64, i32, n32, b64);
+  HIP_OK(hipGetLastError());
   return PF_OK;
 }
 
@@ -809,7 +905,8 @@ int pf_batch_create(size_t hbm_budget, pf_batch **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail_msg(PF_E_NODEV, "no HIP device visible");
   pf_batch *b = new pf_batch();
-  for (int c = 0; c < pf::kNClasses; ++c) b->cls_shape[c] = 3;  // 16x16 until autotuned
+  for (int c = 0; c < pf::kNClasses; ++c)  // 16x16 until autotuned
+    for (int k = 0; k < pf::kNKinds; ++k) b->cls_shape[c][k] = 3;
   if (hipGetDevice(&b->device) != hipSuccess) {
     delete b;
     return fail_msg(PF_E_HIP, "hipGetDevice");
@@ -1142,11 +1239,21 @@ int pf_batch_bytes(const pf_batch *b, size_t *used, size_t *budget) {
 int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
-  if (size_class < 0 || size_class >= pf::kNClasses || shape < 0 || shape >= pf::kNShapes)
-    return fail_msg(PF_E_ARG, "size class in [0,%d), shape in [0,%d)", pf::kNClasses,
-                    pf::kNShapes);
+  int sh[pf::kNKinds];
+  if (shape >= 0 && shape < pf::kNShapes) {
+    for (int k = 0; k < pf::kNKinds; ++k) sh[k] = shape;
+  } else if ((shape & ~0xfff) == 0x1000) {  // PF_BATCH_TILES(a, b, s)
+    for (int k = 0; k < pf::kNKinds; ++k) sh[k] = (shape >> (4 * k)) & 0xf;
+  } else {
+    sh[0] = -1;
+  }
+  bool ok = size_class >= 0 && size_class < pf::kNClasses;
+  for (int k = 0; k < pf::kNKinds; ++k) ok = ok && sh[k] >= 0 && sh[k] < pf::kNShapes;
+  if (!ok)
+    return fail_msg(PF_E_ARG, "size class in [0,%d), shape in [0,%d) or PF_BATCH_TILES(a, b, s)",
+                    pf::kNClasses, pf::kNShapes);
   HIP_OK(hipStreamSynchronize(b->stream));
-  b->cls_shape[size_class] = shape;
+  for (int k = 0; k < pf::kNKinds; ++k) b->cls_shape[size_class][k] = sh[k];
   b->dirty = true;
   return PF_OK;
 }
@@ -1155,7 +1262,9 @@ int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
   if (size_class < 0 || size_class >= pf::kNClasses) return fail_msg(PF_E_ARG, "bad size class");
-  if (shape) *shape = b->cls_shape[size_class];
+  const int *sh = b->cls_shape[size_class];
+  if (shape)
+    *shape = sh[0] == sh[1] && sh[1] == sh[2] ? sh[0] : PF_BATCH_TILES(sh[0], sh[1], sh[2]);
   return PF_OK;
 }
 
@@ -1173,12 +1282,15 @@ int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny) {
   return pf::size_class(precision == PF_FP32_MIXED ? 1 : 0, nx, ny);
 }
 
-// Times each tile shape for the size class (-1: every class present) as
-// the sweeps of the whole batch take with that choice (the other classes
-// keep theirs: how a class's work-groups interleave with the rest matters
-// more than its time alone) and keeps the fastest.  Both sweeps are timed
-// with the roles of the first phase sweep (read the current phi, c, T;
-// write the step's dead buffers), so no step is taken.
+// Times each tile shape for the size class (-1: every class present) and
+// each kernel kind (sweep A, sweep B or the split form's phase + T kernel,
+// the split form's solute kernel) as the sweeps of the whole batch take with
+// that choice, launched in the order of a step (the other classes and kinds
+// keep theirs: how a class's work-groups interleave with the rest, and what
+// the kernel before leaves in the caches, matter more than its time alone),
+// and keeps the fastest.  Every kind is timed with the roles of the first
+// phase sweep (read the current phi, c, T; write the step's dead buffers),
+// so no step is taken.
 int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
   int rc = check_batch(b);
   if (rc != PF_OK) return rc;
@@ -1187,49 +1299,47 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_OK(hipEventCreate(&e0));
   HIP_OK(hipEventCreate(&e1));
+  const int nkinds = b->split_b ? pf::kNKinds : pf::kKindS;
   for (int cls = 0; cls < pf::kNClasses && rc == PF_OK; ++cls) {
     if (size_class >= 0 && cls != size_class) continue;
     bool present = false;
     for (const auto &s : b->slots)
       present = present || (s.used && pf::size_class(s.prec, s.nx, s.ny) == cls);
     if (!present) continue;
-    float best = 1e30f;
-    int bs = b->cls_shape[cls];
-    for (int shape = 0; shape < pf::kNShapes && rc == PF_OK; ++shape) {
-      b->cls_shape[cls] = shape;
-      if ((rc = rebuild(b)) != PF_OK) break;
-      hipError_t e = hipSuccess;
-      for (int r = 0; r <= reps && e == hipSuccess; ++r) {
-        if (r == 1) e = hipEventRecord(e0, b->stream);
-        for (const auto &g : b->groups) {
-          if (e != hipSuccess) break;
+    for (int kind = 0; kind < nkinds && rc == PF_OK; ++kind) {
+      float best = 1e30f;
+      int bs = b->cls_shape[cls][kind];
+      for (int shape = 0; shape < pf::kNShapes && rc == PF_OK; ++shape) {
+        b->cls_shape[cls][kind] = shape;
+        if ((rc = rebuild(b)) != PF_OK) break;
+        hipError_t e = hipSuccess;
+        for (int r = 0; r <= reps && e == hipSuccess; ++r) {
+          if (r == 1) e = hipEventRecord(e0, b->stream);
           // timing launches: the ledgers are left alone
-          if (g.prec == PF_FP64)
-            e = pf::launch_batch_shape<double>(g.shape, g.sweep_b, b->split_b, g.var,
-                                               b->tiles_dev + g.off, g.n,
-                                               (pf::InstDev<double> *)b->inst_dev[0], 1, 0,
-                                               b->stream);
-          else
-            e = pf::launch_batch_shape<float>(g.shape, g.sweep_b, b->split_b, g.var,
-                                              b->tiles_dev + g.off, g.n,
-                                              (pf::InstDev<float> *)b->inst_dev[1], 1, 0,
-                                              b->stream);
+          for (int k = 0; k < nkinds && e == hipSuccess; ++k) {
+            if (k == pf::kKindS) continue;
+            for (int prec = 0; prec < 2; ++prec)
+              for (int kk = k; kk <= (k == pf::kKindB ? nkinds - 1 : k); ++kk)
+                for (const auto &g : b->groups)
+                  if (e == hipSuccess && g.kind == kk && g.prec == prec)
+                    e = launch_group(b, g, 1, 0, false);
+          }
+        }
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(e));
+          break;
+        }
+        if (ms < best) {
+          best = ms;
+          bs = shape;
         }
       }
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) {
-        rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(e));
-        break;
-      }
-      if (ms < best) {
-        best = ms;
-        bs = shape;
-      }
+      b->cls_shape[cls][kind] = bs;
     }
-    b->cls_shape[cls] = bs;
   }
   if (hipStreamSynchronize(b->stream) != hipSuccess && rc == PF_OK)
     rc = fail_msg(PF_E_HIP, "autotune: %s", hipGetErrorString(hipGetLastError()));

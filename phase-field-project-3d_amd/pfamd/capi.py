@@ -19,6 +19,12 @@ PF_OK, PF_E_ARG, PF_E_HIP, PF_E_NOMEM, PF_E_STATE, PF_E_NODEV = 0, -1, -2, -3, -
 PF_FP64, PF_FP32_MIXED = 0, 1
 
 
+def batch_tiles(a: int, b: int, s: int) -> int:
+    """PF_BATCH_TILES: a batch size class's tile shapes for sweep A, sweep B
+    (the split form's phase + T kernel) and the split form's solute kernel."""
+    return 0x1000 | a | (b << 4) | (s << 8)
+
+
 class PfParams(C.Structure):
     """struct pf_params (include/pf_capi.h)."""
 
@@ -449,9 +455,16 @@ class Batch:
         check(self.lib.pf_batch_set_tiling(self.h, int(size_class), int(shape)), self.lib)
 
     def tiling(self, size_class: int) -> int:
+        """One shape when every kernel kind of the class has it, else the
+        batch_tiles(a, b, s) value."""
         sh = C.c_int32(0)
         check(self.lib.pf_batch_get_tiling(self.h, int(size_class), C.byref(sh)), self.lib)
         return sh.value
+
+    def tiling_kinds(self, size_class: int):
+        """(sweep A, sweep B, solute) tile shapes of the class."""
+        v = self.tiling(size_class)
+        return (v, v, v) if v < 0x1000 else (v & 0xf, (v >> 4) & 0xf, (v >> 8) & 0xf)
 
     def set_group_target(self, target_groups: int):
         check(self.lib.pf_batch_set_group_target(self.h, int(target_groups)), self.lib)

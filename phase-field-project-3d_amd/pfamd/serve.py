@@ -30,7 +30,7 @@ sleeps when it has nothing to run and the next arrival is in the future).
 from __future__ import annotations
 
 import time
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
@@ -69,6 +69,8 @@ class ServeReport:
     peak_instances: int = 0
     gpu_wait_s: float = 0.0  # host time spent waiting for ticks to finish
     overtakes: int = 0
+    # one (start, end, steps, runs in flight, cells in flight) per tick
+    tick_log: List[tuple] = field(default_factory=list)
 
     def latencies(self):
         return np.array([c.latency for c in self.completions])
@@ -174,6 +176,7 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
             # inputs of arrived requests while the GPU works
             left = [b.status(sl)[1] for sl in active]
             k = min(tick_steps, min(left) if end_at_completion else max(left))
+            ts = now()
             b.run(k)
             for r in queue:
                 if r.arrival > now():
@@ -184,6 +187,8 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
             b.synchronize()
             rep.gpu_wait_s += time.perf_counter() - tw
             rep.busy_ticks += 1
+            rep.tick_log.append((ts, now(), k, len(active),
+                                 sum(x.problem.cells for x, _ in active.values())))
             # 3. retire the completed runs
             for sl in [sl for sl in active if b.status(sl)[1] == 0]:
                 r, adm = active.pop(sl)

@@ -108,15 +108,18 @@ def test_batch_tiling_and_budget():
     with capi.Batch(hbm_budget=2 * 48 * n + 100) as b:
         cls = b.size_class(prec, deck.Nx, deck.Ny)
         outs = []
-        for shape, target in ((0, 64), (3, 4096)):
+        for shape, target in ((0, 64), (3, 4096), (capi.batch_tiles(1, 2, 3), 512)):
             b.set_tiling(cls, shape)
             b.set_group_target(target)
             assert b.tiling(cls) == shape
+            if shape >= 0x1000:
+                assert b.tiling_kinds(cls) == (1, 2, 3)
             sl = _add(b, "fp64_seed_noise", 10)
             b.run(10)
             outs.append(_from_batch(b, sl, "fp64_seed_noise"))
             b.remove(sl)
         _same(outs[0], outs[1], "tilings")
+        _same(outs[0], outs[2], "tilings per kernel kind")
         _add(b, "fp64_seed_noise", 1)
         _add(b, "fp64_seed_noise", 1)
         with pytest.raises(capi.PfError):
