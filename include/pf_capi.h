@@ -336,7 +336,7 @@ int32_t pf_batch_size_class(int32_t precision, int32_t nx, int32_t ny);
 int pf_batch_set_tiling(pf_batch *b, int32_t size_class, int32_t shape);
 int pf_batch_get_tiling(const pf_batch *b, int32_t size_class, int32_t *shape);
 /* Work-groups per sweep the z-march split aims for over the whole batch
- * (default 16384; segments are at least 16 planes): each instance gets its
+ * (default 7168; segments of 8 to 16 planes): each instance gets its
  * share by cell count. */
 int pf_batch_set_group_target(pf_batch *b, int32_t target_groups);
 /* Times every tile shape for size_class (-1: each class present in turn)

@@ -3,14 +3,15 @@
 //
 // Every instance keeps its fields and its loop scalars (it, TP, Cs/Cl, M0,
 // the REAL(4) time, buffer roles, steps left) on the device.  A step is
-//   k_batch_begin          per instance: principal.f90:96-104 + the kernel
+//   begin (k_batch_turn)   per instance: principal.f90:96-104 + the kernel
 //                          scalars of the sweeps (one thread per instance)
 //   (nv-1) x [sweep A, phase decoration]      one launch per precision and
 //   sweep B: phase + T, then the solute       tile shape (work-group -> tile
 //     with the phase decoration               table); a decoration launch
 //   solute faces, mass corrections due        covers both precisions
-//   k_batch_end            per instance: auxiliar M0 (principal.f90:157),
+//   end (k_batch_turn)     per instance: auxiliar M0 (principal.f90:157),
 //                          buffer roles, steps left
+// with the end of a step and the beginning of the next in one launch
 // so K steps need no host work and are captured once in a hipGraph that is
 // replayed.  An instance that has run its steps goes inactive on the device
 // (its work-groups exit at once) until the host removes it.
@@ -131,31 +132,24 @@ __device__ __forceinline__ void batch_end(InstDev<R> &I) {
 
 // One thread per instance: the FP64 instances in the first b64 blocks of 64
 // threads, the FP32 ones in the blocks after (so that no wave runs both
-// precisions' code one after the other).
+// precisions' code one after the other).  The end of one step and the
+// beginning of the next are one launch: end (when do_end), then begin (when
+// do_begin).
 __global__ void __launch_bounds__(64)
-    k_batch_begin(InstDev<double> *__restrict__ inst64, int n64, int *__restrict__ due64,
-                  InstDev<float> *__restrict__ inst32, int n32, int *__restrict__ due32,
-                  int b64) {
+    k_batch_turn(InstDev<double> *__restrict__ inst64, int n64, int *__restrict__ due64,
+                 InstDev<float> *__restrict__ inst32, int n32, int *__restrict__ due32, int b64,
+                 int do_end, int do_begin) {
   const int blk = blockIdx.x;
   if (blk < b64) {
     const int q = blk * 64 + threadIdx.x;
-    if (q < n64) batch_begin<double>(inst64[q], due64 ? due64 + q : nullptr);
+    if (q >= n64) return;
+    if (do_end) batch_end<double>(inst64[q]);
+    if (do_begin) batch_begin<double>(inst64[q], due64 ? due64 + q : nullptr);
   } else {
     const int q = (blk - b64) * 64 + threadIdx.x;
-    if (q < n32) batch_begin<float>(inst32[q], due32 ? due32 + q : nullptr);
-  }
-}
-
-__global__ void __launch_bounds__(64)
-    k_batch_end(InstDev<double> *__restrict__ inst64, int n64, InstDev<float> *__restrict__ inst32,
-                int n32, int b64) {
-  const int blk = blockIdx.x;
-  if (blk < b64) {
-    const int q = blk * 64 + threadIdx.x;
-    if (q < n64) batch_end<double>(inst64[q]);
-  } else {
-    const int q = (blk - b64) * 64 + threadIdx.x;
-    if (q < n32) batch_end<float>(inst32[q]);
+    if (q >= n32) return;
+    if (do_end) batch_end<float>(inst32[q]);
+    if (do_begin) batch_begin<float>(inst32[q], due32 ? due32 + q : nullptr);
   }
 }
 
@@ -427,6 +421,14 @@ int size_class(int prec, int nx, int ny) {
   return prec * 4 + bucket;
 }
 constexpr int kNClasses = 8;
+// z segments of the batched sweeps: between kSegMin and kSegMax planes
+// (each segment reads 4 planes before its first update).  With the default
+// group target a batch of ~30M cells marches 16-plane segments and one of
+// < 10M cells 8-plane ones: measured on MI355X, the fastest uniform segment
+// length is 16 for a mix of 18 instances / 30M cells (8: +6%, 24: +3%,
+// 32: +6% step time) and 8 for mixes of 4-8 instances / 6-9M cells (16:
+// +4%, 32: +18-38%).
+constexpr int kSegMin = 8, kSegMax = 16;
 // kernel kinds with a tile shape of their own
 enum : int { kKindA = 0, kKindB = 1, kKindS = 2, kNKinds = 3 };
 
@@ -453,6 +455,9 @@ struct pf_batch_slot {
 struct pf_batch {
   int device = 0;
   hipStream_t stream = nullptr;
+  // the FP32 instances' chain of a step (forked from stream, joined back)
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   size_t budget = 0, used = 0;
   int nv = -1;  // shared n_varreduras of the instances
   std::vector<pf_batch_slot> slots;
@@ -463,9 +468,10 @@ struct pf_batch {
   // tile shape per size class and kernel kind (kKindA: sweep A, kKindB:
   // sweep B or the split form's phase + T kernel, kKindS: the split form's
   // solute kernel); work-groups per sweep the z split aims for over the
-  // whole batch (each instance gets its share by cell count)
+  // whole batch (each instance gets its share by cell count), with segments
+  // of at least kSegMin planes
   int cls_shape[pf::kNClasses][pf::kNKinds];
-  int group_target = 16384;
+  int group_target = 7168;
   // sweep B split into the phase + T kernel and the solute kernel (as the
   // single-instance split form; same bits)
   bool split_b = true;
@@ -496,11 +502,11 @@ struct pf_batch {
   double *diag_part = nullptr;
   int diag_cap = 0;
   double *pinned = nullptr;
-  // partial interior sums of the solute-mass correction (kMassBlocks per
-  // instance of the larger per-precision array) and, per precision, the
-  // instances due for a correction after the current step
-  double *mass_part = nullptr;
-  size_t mass_part_cap = 0;
+  // per precision: partial interior sums of the solute-mass correction
+  // (kMassBlocks per instance) and the instances due for a correction after
+  // the current step
+  double *mass_part[2] = {nullptr, nullptr};
+  size_t mass_part_cap[2] = {0, 0};
   int *mass_due[2] = {nullptr, nullptr};
   int mass_due_cap[2] = {0, 0};
 };
@@ -572,16 +578,17 @@ template <typename R>
 void set_geo(pf::InstDev<R> &I, const pf_batch &b, const pf_batch_slot &s) {
   const int cls = pf::size_class(s.prec, s.nx, s.ny);
   // z segments: the instance's share (by cells) of group_target work-groups
-  // over the whole batch; a batch of many instances marches whole columns,
-  // which saves the 4 warm-up planes per segment
-  const double share = b.total_cells ? (double)s.n / (double)b.total_cells : 1.0;
+  // over the whole batch; a batch with many cells marches longer segments,
+  // which saves warm-up planes, a small batch shorter ones, which shortens
+  // its launches
+  const double share = b.total_cells > s.n ? (double)s.n / (double)b.total_cells : 1.0;
   const int want = std::max(1, (int)(share * b.group_target + 0.5));
   auto geo = [&](int kind, bool sweep_b) {
     const int shape = b.cls_shape[cls][kind];
     const int plane_tiles = ((s.nx - 2 + pf::kShapeX[shape] - 1) / pf::kShapeX[shape]) *
                             ((s.ny - 2 + pf::kShapeY[shape] - 1) / pf::kShapeY[shape]);
     const int nseg = std::max(1, (want + plane_tiles - 1) / plane_tiles);
-    const int seg = std::max(16, (s.nz - 2 + nseg - 1) / nseg);
+    const int seg = std::min(pf::kSegMax, std::max(pf::kSegMin, (s.nz - 2 + nseg - 1) / nseg));
     return pf::batch_geo(s.nx, s.ny, s.nz, sweep_b, shape, seg);
   };
   I.ga = geo(pf::kKindA, false);
@@ -671,13 +678,15 @@ int rebuild(pf_batch *b) {
       b->mass_due_cap[prec] = need;
     }
   }
-  const size_t nmp = (size_t)std::max(b->slot_of[0].size(), b->slot_of[1].size()) * pf::kMassBlocks;
-  if (nmp > b->mass_part_cap) {
-    if (b->mass_part) HIP_OK(hipFree(b->mass_part));
-    b->mass_part = nullptr;
-    b->mass_part_cap = 0;
-    HIP_OK(hipMalloc((void **)&b->mass_part, nmp * sizeof(double)));
-    b->mass_part_cap = nmp;
+  for (int prec = 0; prec < 2; ++prec) {
+    const size_t nmp = b->slot_of[prec].size() * pf::kMassBlocks;
+    if (nmp > b->mass_part_cap[prec]) {
+      if (b->mass_part[prec]) HIP_OK(hipFree(b->mass_part[prec]));
+      b->mass_part[prec] = nullptr;
+      b->mass_part_cap[prec] = 0;
+      HIP_OK(hipMalloc((void **)&b->mass_part[prec], nmp * sizeof(double)));
+      b->mass_part_cap[prec] = nmp;
+    }
   }
   if (!tiles.empty())
     HIP_OK(hipMemcpyAsync(b->tiles_dev, tiles.data(), tiles.size() * sizeof(pf::TileRef),
@@ -715,7 +724,8 @@ int rebuild(pf_batch *b) {
 
 // One group's launch (sweep `sweep` of the step); `deco`: the solute
 // launch also runs its phase-decoration chunks.
-hipError_t launch_group(pf_batch *b, const pf_batch::Group &g, int sweep, int ledger, bool deco) {
+hipError_t launch_group(pf_batch *b, const pf_batch::Group &g, int sweep, int ledger, bool deco,
+                        hipStream_t st) {
   const int mode = g.kind == pf::kKindA   ? pf::kSweepA
                    : g.kind == pf::kKindS ? pf::kSweepSolute
                    : b->split_b           ? pf::kSweepPT
@@ -725,36 +735,32 @@ hipError_t launch_group(pf_batch *b, const pf_batch::Group &g, int sweep, int le
   if (g.prec == PF_FP64)
     return pf::launch_batch_group<double>(g.shape, mode, g.var, b->tiles_dev + g.off, g.n,
                                           (pf::InstDev<double> *)b->inst_dev[0], sweep, ledger,
-                                          ch, nd, b->stream);
+                                          ch, nd, st);
   return pf::launch_batch_group<float>(g.shape, mode, g.var, b->tiles_dev + g.off, g.n,
                                        (pf::InstDev<float> *)b->inst_dev[1], sweep, ledger, ch,
-                                       nd, b->stream);
+                                       nd, st);
 }
 
-// Enqueues one step of every active instance.
-int enqueue_step(pf_batch *b) {
-  hipStream_t st = b->stream;
+// Enqueues the sweeps, decorations and solute-mass corrections of one
+// precision's instances for one step (between its beginning and its end) on
+// stream st.
+int enqueue_chain(pf_batch *b, int prec, hipStream_t st) {
   auto *i64 = (pf::InstDev<double> *)b->inst_dev[0];
   auto *i32 = (pf::InstDev<float> *)b->inst_dev[1];
-  const int n64 = (int)b->slot_of[0].size(), n32 = (int)b->slot_of[1].size();
-  const int b64 = (n64 + 63) / 64;
-  const unsigned nbi = (unsigned)(b64 + (n32 + 63) / 64);
-  hipLaunchKernelGGL(pf::k_batch_begin, dim3(nbi), dim3(64), 0, st, i64, n64, b->mass_due[0], i32,
-                     n32, b->mass_due[1], b64);
-  HIP_OK(hipGetLastError());
-  auto sweeps = [&](int kind, int sweep, int prec = -1) -> int {
+  auto sweeps = [&](int kind, int sweep) -> int {
     for (const auto &g : b->groups) {
-      if (g.kind != kind || (prec >= 0 && g.prec != prec)) continue;
-      const hipError_t e = launch_group(b, g, sweep, 1, true);
+      if (g.kind != kind || g.prec != prec) continue;
+      const hipError_t e = launch_group(b, g, sweep, 1, true, st);
       if (e != hipSuccess) return fail_msg(PF_E_HIP, "batched sweep: %s", hipGetErrorString(e));
     }
     return PF_OK;
   };
   auto decos = [&](int which, int sweep) -> int {
-    if (b->deco_n[which] == 0) return PF_OK;
-    hipLaunchKernelGGL(pf::k_deco_batch, dim3(b->deco_n[which]), dim3(256), 0, st,
-                       b->chunks_dev + b->deco_off[which], b->deco_n[which], i64, i32, which,
-                       sweep);
+    const int n = prec == PF_FP64 ? b->deco_n64[which] : b->deco_n[which] - b->deco_n64[which];
+    if (n == 0) return PF_OK;
+    const size_t off = b->deco_off[which] + (prec == PF_FP64 ? 0 : b->deco_n64[which]);
+    hipLaunchKernelGGL(pf::k_deco_batch, dim3(n), dim3(256), 0, st, b->chunks_dev + off, n, i64,
+                       i32, which, sweep);
     HIP_OK(hipGetLastError());
     return PF_OK;
   };
@@ -764,52 +770,84 @@ int enqueue_step(pf_batch *b) {
     if ((rc = sweeps(pf::kKindA, sw)) != PF_OK) return rc;
     if ((rc = decos(0, sw)) != PF_OK) return rc;
   }
-  // sweep B, per precision: the phase + T kernels before the solute
-  // kernels (which read the same phi and c, partly still in the caches, and
-  // whose launches also decorate the swept phi)
+  // sweep B: the phase + T kernels before the solute kernels (which read
+  // the same phi and c, partly still in the caches, and whose launches also
+  // decorate the swept phi)
+  if ((rc = sweeps(pf::kKindB, nv)) != PF_OK) return rc;
   if (b->split_b) {
-    for (int prec = 0; prec < 2; ++prec) {
-      if ((rc = sweeps(pf::kKindB, nv, prec)) != PF_OK) return rc;
-      if ((rc = sweeps(pf::kKindS, nv, prec)) != PF_OK) return rc;
-    }
-  } else {
-    if ((rc = sweeps(pf::kKindB, nv)) != PF_OK) return rc;
-    if ((rc = decos(0, nv)) != PF_OK) return rc;
+    if ((rc = sweeps(pf::kKindS, nv)) != PF_OK) return rc;
+  } else if ((rc = decos(0, nv)) != PF_OK) {
+    return rc;
   }
   if ((rc = decos(1, nv)) != PF_OK) return rc;
   // solute-mass corrections of the instances due this step (after the
   // step's face copy, as in the single-instance loop)
-  for (int prec = 0; prec < 2; ++prec) {
-    const int n = (int)b->slot_of[prec].size();
-    bool any = false;
-    for (int id : b->slot_of[prec]) any = any || b->slots[id].mass_every > 0;
-    if (!n || !any) continue;
-    const int *due = b->mass_due[prec];
-    if (prec == PF_FP64) {
-      auto *I = (pf::InstDev<double> *)b->inst_dev[prec];
-      hipLaunchKernelGGL(pf::k_bmass_interior<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
-                         n, due, b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, I, n, due,
-                         b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n,
-                         due);
-      hipLaunchKernelGGL(pf::k_bmass_refresh<double>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                         I, n, due);
-    } else {
-      auto *I = (pf::InstDev<float> *)b->inst_dev[prec];
-      hipLaunchKernelGGL(pf::k_bmass_interior<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I,
-                         n, due, b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, I, n, due,
-                         b->mass_part);
-      hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, I, n,
-                         due);
-      hipLaunchKernelGGL(pf::k_bmass_refresh<float>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                         I, n, due);
-    }
-    HIP_OK(hipGetLastError());
+  const int n = (int)b->slot_of[prec].size();
+  bool any = false;
+  for (int id : b->slot_of[prec]) any = any || b->slots[id].mass_every > 0;
+  if (!n || !any) return PF_OK;
+  const int *due = b->mass_due[prec];
+  double *part = b->mass_part[prec];
+  if (prec == PF_FP64) {
+    hipLaunchKernelGGL(pf::k_bmass_interior<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, i64,
+                       n, due, part);
+    hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, i64, n, due, part);
+    hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, i64, n,
+                       due);
+    hipLaunchKernelGGL(pf::k_bmass_refresh<double>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
+                       i64, n, due);
+  } else {
+    hipLaunchKernelGGL(pf::k_bmass_interior<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, i32,
+                       n, due, part);
+    hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, i32, n, due, part);
+    hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, i32, n,
+                       due);
+    hipLaunchKernelGGL(pf::k_bmass_refresh<float>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
+                       i32, n, due);
   }
-  hipLaunchKernelGGL(pf::k_batch_end, dim3(nbi), dim3(64), 0, st, i64, n64, i32, n32, b64);
   HIP_OK(hipGetLastError());
+  return PF_OK;
+}
+
+// One step's sweeps, decorations and corrections: with instances of both
+// precisions, the FP32 chain runs on a second stream beside the FP64 chain
+// (forked from and joined back to the batch's stream; a captured graph
+// keeps the two as parallel branches), so that each chain's launches fill
+// the GPU the other leaves idle.
+int enqueue_step_body(pf_batch *b) {
+  const bool both = !b->slot_of[0].empty() && !b->slot_of[1].empty();
+  int rc;
+  if (!both) return enqueue_chain(b, b->slot_of[0].empty() ? PF_FP32_MIXED : PF_FP64, b->stream);
+  HIP_OK(hipEventRecord(b->ev_fork, b->stream));
+  HIP_OK(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+  if ((rc = enqueue_chain(b, PF_FP64, b->stream)) != PF_OK) return rc;
+  if ((rc = enqueue_chain(b, PF_FP32_MIXED, b->stream2)) != PF_OK) return rc;
+  HIP_OK(hipEventRecord(b->ev_join, b->stream2));
+  HIP_OK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+  return PF_OK;
+}
+
+// Enqueues k steps of every active instance (k >= 1); one launch ends a
+// step and begins the next.
+int enqueue_steps(pf_batch *b, int k) {
+  hipStream_t st = b->stream;
+  auto *i64 = (pf::InstDev<double> *)b->inst_dev[0];
+  auto *i32 = (pf::InstDev<float> *)b->inst_dev[1];
+  const int n64 = (int)b->slot_of[0].size(), n32 = (int)b->slot_of[1].size();
+  const int b64 = (n64 + 63) / 64;
+  const unsigned nbi = (unsigned)(b64 + (n32 + 63) / 64);
+  auto turn = [&](int do_end, int do_begin) -> int {
+    hipLaunchKernelGGL(pf::k_batch_turn, dim3(nbi), dim3(64), 0, st, i64, n64, b->mass_due[0],
+                       i32, n32, b->mass_due[1], b64, do_end, do_begin);
+    HIP_OK(hipGetLastError());
+    return PF_OK;
+  };
+  int rc;
+  if ((rc = turn(0, 1)) != PF_OK) return rc;
+  for (int q = 0; q < k; ++q) {
+    if ((rc = enqueue_step_body(b)) != PF_OK) return rc;
+    if ((rc = turn(1, q + 1 < k)) != PF_OK) return rc;
+  }
   return PF_OK;
 }
 
@@ -822,7 +860,7 @@ int capture_graph(pf_batch *b) {
   hipGraph_t g = nullptr;
   HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = PF_OK;
-  for (int q = 0; q < b->graph_steps && rc == PF_OK; ++q) rc = enqueue_step(b);
+  rc = enqueue_steps(b, b->graph_steps);
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (rc != PF_OK) {
     if (g) (void)hipGraphDestroy(g);
@@ -915,6 +953,12 @@ int pf_batch_create(size_t hbm_budget, pf_batch **out) {
     delete b;
     return fail_msg(PF_E_HIP, "hipStreamCreate");
   }
+  if (hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+    pf_batch_destroy(b);
+    return fail_msg(PF_E_HIP, "hipStreamCreate / hipEventCreate");
+  }
   size_t fr = 0, tot = 0;
   if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;  // then only an explicit budget admits
   b->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
@@ -937,10 +981,15 @@ int pf_batch_destroy(pf_batch *b) {
   if (b->tiles_dev) (void)hipFree(b->tiles_dev);
   if (b->chunks_dev) (void)hipFree(b->chunks_dev);
   if (b->diag_part) (void)hipFree(b->diag_part);
-  if (b->mass_part) (void)hipFree(b->mass_part);
+  for (int prec = 0; prec < 2; ++prec)
+    if (b->mass_part[prec]) (void)hipFree(b->mass_part[prec]);
   for (int prec = 0; prec < 2; ++prec)
     if (b->mass_due[prec]) (void)hipFree(b->mass_due[prec]);
   if (b->pinned) (void)hipHostFree(b->pinned);
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   if (e != hipSuccess) return fail_msg(PF_E_HIP, "pending work failed: %s", hipGetErrorString(e));
@@ -1092,8 +1141,7 @@ int pf_batch_run(pf_batch *b, int32_t nsteps) {
       left -= b->graph_steps;
     }
   }
-  for (; left > 0; --left)
-    if ((rc = enqueue_step(b)) != PF_OK) return rc;
+  if (left > 0 && (rc = enqueue_steps(b, left)) != PF_OK) return rc;
   for (auto &s : b->slots)
     if (s.used) s.steps_left = std::max(0, s.steps_left - nsteps);
   return PF_OK;
@@ -1322,7 +1370,7 @@ int pf_batch_autotune(pf_batch *b, int32_t size_class, int32_t reps) {
               for (int kk = k; kk <= (k == pf::kKindB ? nkinds - 1 : k); ++kk)
                 for (const auto &g : b->groups)
                   if (e == hipSuccess && g.kind == kk && g.prec == prec)
-                    e = launch_group(b, g, 1, 0, false);
+                    e = launch_group(b, g, 1, 0, false, b->stream);
           }
         }
         float ms = 0.f;

@@ -101,24 +101,43 @@ PF_HD void equilibrium(double TP, double Tm, float Ke, double Me, double &Cs_eq,
   Cl_eq = (Tm - TP) / Me;
 }
 
+// Terms of auxiliar's integral (below): phi_i runs from 0.11 in rounded
+// steps of 0.01 while phi_i <= 0.8999 -- the same additions, hence the same
+// count, every time.
+constexpr int m0_terms() {
+  double phi_i = 0.11;
+  int n = 0;
+  do {
+    ++n;
+    phi_i = phi_i + 0.01;
+  } while (phi_i <= 0.8999);
+  return n;
+}
+constexpr int kM0Terms = m0_terms();
+static_assert(kM0Terms == 79, "auxiliar's integral has 79 terms");
+
 // source/auxiliar.f90:14-42: trapezoid rule over phi_i = 0.11..0.89 with the
 // hard-coded end values, then 1/M0 from the thin-interface relation, in the
 // reference's operation order with its REAL(4) literals and (1-Ke).  Shared
 // by the host loop and the batched form's per-instance device update, so
 // both give the same bits.
+
 PF_HD double auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm, double E0,
                          double W0, float Sigma, float Beta, float Ke, double Me, double D_liq,
                          float ajuste) {
 #pragma clang fp contract(off)
   double acc = 0.0, phi_i = 0.11;
-  do {
+  // with a fixed trip count the compiler overlaps the terms' divisions (the
+  // terms are independent; the sum keeps its order)
+#pragma unroll 8
+  for (int n = 0; n < kM0Terms; ++n) {
     const double h = phi_i * phi_i * (3.0 - 2.0 * phi_i);
     const double f1 = phi_i * (1.0 - phi_i);
     const double num = h * (1.0 - h);
     const double den = ((1.0 - h) * Cl_eq * (1.0 - Cl_eq) + h * Cs_eq * (1.0 - Cs_eq)) * f1;
     acc = acc + num / den;
     phi_i = phi_i + 0.01;
-  } while (phi_i <= 0.8999);
+  }
   const double integral = ((double)0.01f / 2.0) * ((double)54.39180136f + 2.0 * acc + 234.073053);
   const double qsi = R * TP * (1.0 / Vm) * (Cl_eq - Cs_eq) * (Cl_eq - Cs_eq) * integral;
   const float omk = 1.0f - Ke;

@@ -69,7 +69,15 @@ class ServeReport:
     peak_instances: int = 0
     gpu_wait_s: float = 0.0  # host time spent waiting for ticks to finish
     overtakes: int = 0
-    # one (start, end, steps, runs in flight, cells in flight) per tick
+    # host time in admissions (pf_batch_add and tuning), in enqueueing ticks
+    # (pf_batch_run, which rebuilds the tables and recaptures the graph after
+    # a change of the instance set) and in retiring (download and remove)
+    admit_s: float = 0.0
+    enqueue_s: float = 0.0
+    retire_s: float = 0.0
+    prepare_s: float = 0.0  # building initial fields while a tick runs
+    # one (start, end, steps, runs in flight, cells in flight, enqueue time)
+    # per tick
     tick_log: List[tuple] = field(default_factory=list)
 
     def latencies(self):
@@ -88,6 +96,10 @@ class ServeReport:
                 "peak_bytes": self.peak_bytes,
                 "ticks": self.busy_ticks,
                 "gpu_wait_s": self.gpu_wait_s,
+                "admit_s": self.admit_s,
+                "enqueue_s": self.enqueue_s,
+                "retire_s": self.retire_s,
+                "prepare_s": self.prepare_s,
                 "overtakes": self.overtakes}
 
 
@@ -149,6 +161,7 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
                         break  # the head has been overtaken enough: wait for it
                     q += 1
                     continue
+                ta = time.perf_counter()
                 s, f = prepared.pop(r.rid, None) or r.problem.setup(lib)
                 slot = b.add(s.p, s.L, r.problem.steps, f[0], f[2], f[4])
                 del f
@@ -164,6 +177,7 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
                     if cls not in tuned:  # first instance of a size class: tune it
                         b.autotune(reps=1, size_class=cls)
                         tuned.add(cls)
+                rep.admit_s += time.perf_counter() - ta
             used, _ = b.bytes()
             rep.peak_bytes = max(rep.peak_bytes, used)
             rep.peak_instances = max(rep.peak_instances, len(active))
@@ -178,18 +192,23 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
             k = min(tick_steps, min(left) if end_at_completion else max(left))
             ts = now()
             b.run(k)
+            te = now() - ts
+            rep.enqueue_s += te
+            tp = time.perf_counter()
             for r in queue:
                 if r.arrival > now():
                     break
                 if r.rid not in prepared:
                     prepared[r.rid] = r.problem.setup(lib)
+            rep.prepare_s += time.perf_counter() - tp
             tw = time.perf_counter()
             b.synchronize()
             rep.gpu_wait_s += time.perf_counter() - tw
             rep.busy_ticks += 1
             rep.tick_log.append((ts, now(), k, len(active),
-                                 sum(x.problem.cells for x, _ in active.values())))
+                                 sum(x.problem.cells for x, _ in active.values()), te))
             # 3. retire the completed runs
+            tr = time.perf_counter()
             for sl in [sl for sl in active if b.status(sl)[1] == 0]:
                 r, adm = active.pop(sl)
                 c = Completion(r.rid, r.arrival, adm, now())
@@ -201,6 +220,7 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
                 if log:
                     log(f"[serve] request {r.rid} done: latency {c.latency:.2f} s, "
                         f"{len(active)} running, {len(queue)} waiting")
+            rep.retire_s += time.perf_counter() - tr
         rep.makespan = now()
         return rep
     finally:
