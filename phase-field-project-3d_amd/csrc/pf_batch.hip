@@ -250,13 +250,37 @@ __global__ void __launch_bounds__(TL::NT)
 }
 
 // which = 0: phase decoration of sweep `sweep` (shell of width 2);
-// which = 1: solute faces of the step (width 1); the chunks of both
-// precisions in one launch (a chunk's precision picks the instance array).
+// which = 1: solute faces of the step (width 1); a chunk's precision picks
+// the instance array.  With nmass > 0 the kMassBlocks work-groups after the
+// chunks' compute partial b of the interior sum of the new c of every
+// instance of precision mass_prec due for a solute-mass correction (into
+// part; the face copy writes only faces, the sum reads only the interior).
 __global__ void __launch_bounds__(256) k_deco_batch(const Chunk *__restrict__ chunks, int nchunk,
                                                     const InstDev<double> *__restrict__ inst64,
                                                     const InstDev<float> *__restrict__ inst32,
-                                                    int which, int sweep) {
-  if ((int)blockIdx.x >= nchunk) return;
+                                                    int which, int sweep, int nmass, int mass_prec,
+                                                    const int *__restrict__ due,
+                                                    double *__restrict__ part) {
+  if ((int)blockIdx.x >= nchunk) {
+    const int mb = (int)blockIdx.x - nchunk;
+    if (mb >= kMassBlocks) return;
+    __shared__ double sh[256];
+    for (int q = 0; q < nmass; ++q) {
+      if (!due[q]) continue;  // uniform over the block
+      double t;
+      if (mass_prec == PF_FP64) {
+        const InstDev<double> &I = inst64[q];
+        t = block_sum256(mass_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz, mb, I.k.c_off),
+                         sh);
+      } else {
+        const InstDev<float> &I = inst32[q];
+        t = block_sum256(mass_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz, mb, I.k.c_off),
+                         sh);
+      }
+      if (threadIdx.x == 0) part[(size_t)q * kMassBlocks + mb] = t;
+    }
+    return;
+  }
   const Chunk ch = chunks[blockIdx.x];
   if (ch.prec == PF_FP64) {
     const InstDev<double> &I = inst64[ch.inst];
@@ -333,44 +357,25 @@ hipError_t launch_batch_group(int shape, int mode, int var, const TileRef *tiles
 // batched and alone.  Each kernel covers every instance and skips those not
 // due, so a captured step needs no host decision.
 
-// block b: partial b of every due instance's interior sum of its new c
+// Rescales block b's rows of every due instance's new c by f = ledger / S,
+// with S the fold of the instance's kMassBlocks interior partials: every
+// work-group folds them itself, in the same order, so all hold the same S
+// and f (work-group 0 records both in the ledger).
 template <typename R>
-__global__ void __launch_bounds__(256) k_bmass_interior(const InstDev<R> *__restrict__ inst, int n,
-                                                        const int *__restrict__ due,
-                                                        double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_bmass_scale(InstDev<R> *__restrict__ inst, int n,
+                                                     const int *__restrict__ due,
+                                                     const double *__restrict__ part) {
   __shared__ double sh[256];
   for (int q = 0; q < n; ++q) {
     if (!due[q]) continue;  // uniform over the block
-    const InstDev<R> &I = inst[q];
-    const double t = block_sum256(mass_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz,
-                                            (int)blockIdx.x, I.k.c_off), sh);
-    if (threadIdx.x == 0) part[(size_t)q * kMassBlocks + blockIdx.x] = t;
-  }
-}
-
-// block q: instance q's interior sum and factor
-template <typename R>
-__global__ void __launch_bounds__(256) k_bmass_update(InstDev<R> *__restrict__ inst, int n,
-                                                      const int *__restrict__ due,
-                                                      const double *__restrict__ part) {
-  __shared__ double sh[256];
-  const int q = blockIdx.x;
-  if (q >= n || !due[q]) return;
-  const double S = mass_fold(part + (size_t)q * kMassBlocks, sh);
-  if (threadIdx.x == 0) {
-    MassLedger &m = inst[q].m;
-    m.S = S;
-    m.f = ledger_value(m) / S;
-  }
-}
-
-template <typename R>
-__global__ void __launch_bounds__(256) k_bmass_scale(const InstDev<R> *__restrict__ inst, int n,
-                                                     const int *__restrict__ due) {
-  for (int q = 0; q < n; ++q) {
-    if (!due[q]) continue;
-    const InstDev<R> &I = inst[q];
-    mass_scale_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, I.m.f,
+    InstDev<R> &I = inst[q];
+    const double S = mass_fold(part + (size_t)q * kMassBlocks, sh);
+    const double f = ledger_value(I.m) / S;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      I.m.S = S;
+      I.m.f = f;
+    }
+    mass_scale_rows(gfield(I.c[I.co ^ 1]), I.k.nx, I.k.ny, I.k.nz, I.k.it, I.k.seed, f,
                     (int)blockIdx.x, I.k.c_off);
   }
 }
@@ -755,12 +760,21 @@ int enqueue_chain(pf_batch *b, int prec, hipStream_t st) {
     }
     return PF_OK;
   };
+  // instances of this precision keeping a solute-mass ledger: the
+  // correction's interior sums ride on the solute-face launch
+  const int ninst = (int)b->slot_of[prec].size();
+  bool mass = false;
+  for (int id : b->slot_of[prec]) mass = mass || b->slots[id].mass_every > 0;
+  const int *due = b->mass_due[prec];
+  double *part = b->mass_part[prec];
   auto decos = [&](int which, int sweep) -> int {
     const int n = prec == PF_FP64 ? b->deco_n64[which] : b->deco_n[which] - b->deco_n64[which];
-    if (n == 0) return PF_OK;
+    const int nm = which == 1 && mass ? ninst : 0;
+    const unsigned grid = (unsigned)n + (nm ? pf::kMassBlocks : 0);
+    if (grid == 0) return PF_OK;
     const size_t off = b->deco_off[which] + (prec == PF_FP64 ? 0 : b->deco_n64[which]);
-    hipLaunchKernelGGL(pf::k_deco_batch, dim3(n), dim3(256), 0, st, b->chunks_dev + off, n, i64,
-                       i32, which, sweep);
+    hipLaunchKernelGGL(pf::k_deco_batch, dim3(grid), dim3(256), 0, st, b->chunks_dev + off, n,
+                       i64, i32, which, sweep, nm, prec, due, part);
     HIP_OK(hipGetLastError());
     return PF_OK;
   };
@@ -781,29 +795,19 @@ int enqueue_chain(pf_batch *b, int prec, hipStream_t st) {
   }
   if ((rc = decos(1, nv)) != PF_OK) return rc;
   // solute-mass corrections of the instances due this step (after the
-  // step's face copy, as in the single-instance loop)
-  const int n = (int)b->slot_of[prec].size();
-  bool any = false;
-  for (int id : b->slot_of[prec]) any = any || b->slots[id].mass_every > 0;
-  if (!n || !any) return PF_OK;
-  const int *due = b->mass_due[prec];
-  double *part = b->mass_part[prec];
+  // step's face copy, as in the single-instance loop; the interior sums ran
+  // with the face copy)
+  if (!ninst || !mass) return PF_OK;
   if (prec == PF_FP64) {
-    hipLaunchKernelGGL(pf::k_bmass_interior<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, i64,
-                       n, due, part);
-    hipLaunchKernelGGL(pf::k_bmass_update<double>, dim3(n), dim3(256), 0, st, i64, n, due, part);
-    hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, i64, n,
-                       due);
+    hipLaunchKernelGGL(pf::k_bmass_scale<double>, dim3(pf::kMassBlocks), dim3(256), 0, st, i64,
+                       ninst, due, (const double *)part);
     hipLaunchKernelGGL(pf::k_bmass_refresh<double>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                       i64, n, due);
+                       i64, ninst, due);
   } else {
-    hipLaunchKernelGGL(pf::k_bmass_interior<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, i32,
-                       n, due, part);
-    hipLaunchKernelGGL(pf::k_bmass_update<float>, dim3(n), dim3(256), 0, st, i32, n, due, part);
-    hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, i32, n,
-                       due);
+    hipLaunchKernelGGL(pf::k_bmass_scale<float>, dim3(pf::kMassBlocks), dim3(256), 0, st, i32,
+                       ninst, due, (const double *)part);
     hipLaunchKernelGGL(pf::k_bmass_refresh<float>, dim3(pf::kRefreshBlocks), dim3(256), 0, st,
-                       i32, n, due);
+                       i32, ninst, due);
   }
   HIP_OK(hipGetLastError());
   return PF_OK;
