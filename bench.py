@@ -307,20 +307,23 @@ def precision_study(checkpoints=(5000, 10000, 20000)):
             "checkpoints": out}
 
 
-def batched_study(steps=5000):
+def batched_study(steps=5000, graph_steps=16):
     """The 64-instance ensemble batched vs one after another (the same
-    problems, single-instance fused loop)."""
+    problems, single-instance fused loop).  graph_steps 0 launches the
+    batched steps one by one instead of replaying a captured graph (for a
+    kernel trace: under rocprofv3 the batched form's graph capture ended in
+    a segmentation fault inside the HIP runtime)."""
     from pfamd import ensemble
 
     probs = ensemble.ensemble_problems(64, 7, steps)
-    rb = ensemble.run_batched(probs)
+    rb = ensemble.run_batched(probs, graph_steps=graph_steps)
     log(f"[bench] batched ensemble: {rb.wall_s:.2f} s, {rb.mlups:.0f} MLUPS")
     rs = ensemble.run_sequential(probs)
     log(f"[bench] sequential ensemble: {rs.wall_s:.2f} s, {rs.mlups:.0f} MLUPS")
     return {"config": "64 instances, edges log-uniform 32-160 (z >= x,y), cooling U[5,50] K/s, "
                       "1-8 bottom seeds r=4, effective noise U[0,0.02], FP32-mixed, "
                       f"{steps} steps each, seed 7; one hipGraph-replayed launch per sweep",
-            "cells_total": sum(p.cells for p in probs),
+            "cells_total": sum(p.cells for p in probs), "graph_steps": graph_steps,
             "solves_per_s": round(len(probs) / rb.wall_s, 3),
             "sequential_solves_per_s": round(len(probs) / rs.wall_s, 3),
             "speedup_vs_sequential": round(rs.wall_s / rb.wall_s, 3),
@@ -328,7 +331,7 @@ def batched_study(steps=5000):
             "wall_s": round(rb.wall_s, 3), "sequential_wall_s": round(rs.wall_s, 3)}
 
 
-def serving_study(n=24, rate=4.0, peak_mlups=None):
+def serving_study(n=24, rate=4.0, peak_mlups=None, graph_steps=16):
     """Poisson request stream served with continuous batching and served
     one request at a time (FIFO), same stream.  Beside the ratio, the bound
     that the stream itself sets: its cell-steps run at the headline grid's
@@ -338,7 +341,7 @@ def serving_study(n=24, rate=4.0, peak_mlups=None):
     from pfamd import capi, serve
 
     reqs = serve.request_stream(n=n, rate=rate, seed=11)
-    rb = serve.serve_batched(reqs)
+    rb = serve.serve_batched(reqs, graph_steps=graph_steps)
     sb = {k: round(v, 3) for k, v in rb.summary().items()}
     log(f"[bench] serving (batched): {sb}")
     rs = serve.serve_sequential(reqs)
@@ -574,6 +577,8 @@ def main(argv=None):
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-karma", action="store_true")
+    ap.add_argument("--graph-steps", type=int, default=16,
+                    help="steps per captured graph of the batched and serving legs (0: no graph)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tiling")
     ap.add_argument("--tiling", type=parse_tiling, default=None,
@@ -663,13 +668,14 @@ def main(argv=None):
         out["fp32_mixed"]["precision_study"] = precision_study()
     if not a.no_batched:
         log("[bench] batched ensemble (64 instances)")
-        out["batched"] = batched_study()
+        out["batched"] = batched_study(graph_steps=a.graph_steps)
     if not a.no_serving:
         log("[bench] serving (Poisson stream)")
         peaks = {capi.PF_FP64: mlups}
         if "fp32_mixed" in out:
             peaks[capi.PF_FP32_MIXED] = out["fp32_mixed"]["value"]
-        out["serving"] = serving_study(peak_mlups=peaks if len(peaks) == 2 else None)
+        out["serving"] = serving_study(peak_mlups=peaks if len(peaks) == 2 else None,
+                                       graph_steps=a.graph_steps)
     if not a.no_karma:
         log("[bench] Karma model")
         out["karma"] = karma_study(cpu=not a.no_cpu_baseline)
