@@ -280,9 +280,11 @@ int pf_synchronize(pf_ctx *ctx);
 /* ---- batched form ----------------------------------------------------------
  * Many independent instances (each a pf_init-style problem with its own grid,
  * precision, deck scalars and step count) advanced together: one launch per
- * sweep covers every instance, the loop scalars of principal.f90:96-157 (it,
- * REAL(4) time, TP, Cs/Cl, auxiliar M0) advance on the device per instance,
- * and K steps are captured once in a hipGraph and replayed.  No counterpart
+ * sweep, precision and tile shape covers its instances (the FP64 and the
+ * FP32-mixed instances' launches on two streams of the batch, side by side),
+ * the loop scalars of principal.f90:96-157 (it, REAL(4) time, TP, Cs/Cl,
+ * auxiliar M0) advance on the device per instance, and K steps are captured
+ * once in a hipGraph and replayed.  No counterpart
  * in the reference, which runs one problem per process.  An instance's
  * result does not depend on the other instances or on the tiling: it equals
  * pf_run_steps on the same instance (tested), except that DPP is not kept --
