@@ -161,6 +161,8 @@ def serve_batched(requests: List[Request], hbm_budget: int = 0, tick_steps: int 
                         break  # the head has been overtaken enough: wait for it
                     q += 1
                     continue
+                if q > 0 and head_skips >= overtake_limit:
+                    break  # (also within one admission pass)
                 ta = time.perf_counter()
                 s, f = prepared.pop(r.rid, None) or r.problem.setup(lib)
                 slot = b.add(s.p, s.L, r.problem.steps, f[0], f[2], f[4])
