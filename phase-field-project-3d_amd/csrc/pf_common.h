@@ -150,49 +150,65 @@ PF_HD double auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double
 PF_HD size_t cidx(int i, int j, int k, int nx, int ny) {
   return (size_t)(i - 1) + (size_t)nx * ((size_t)(j - 1) + (size_t)ny * (size_t)(k - 1));
 }
+__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w, bool xf = true) {
+  return 2LL * w * nx * ny + (2LL * w * nx + (xf ? 2LL * w * (ny - 2 * w) : 0LL)) * (nz - 2 * w);
+}
+
+// shell_flat's index arithmetic in the integer type I (the same values in
+// either type when they fit it).
+template <typename I>
+__host__ __device__ __forceinline__ void shell_flat_in(I t, I nx, I ny, I nz, I w, int &i, int &j,
+                                                      int &k, bool xf) {
+  const I plane = nx * ny;
+  const I full = w * plane;  // the w planes at one z face
+  const I per = 2 * w * nx + (xf ? 2 * w * (ny - 2 * w) : 0);  // one middle plane
+  const I mid = per * (nz - 2 * w);
+  if (t < full || t >= full + mid) {
+    const I u = t < full ? t : t - full - mid;
+    const I kk = u / plane;
+    k = (int)(t < full ? kk + 1 : nz - w + 1 + kk);
+    const I r = u - kk * plane;
+    const I jj = r / nx;
+    i = (int)(r - jj * nx) + 1;
+    j = (int)jj + 1;
+    return;
+  }
+  const I u = t - full;
+  const I km = u / per;
+  k = (int)(w + 1 + km);
+  const I r = u - km * per;
+  const I rows = 2 * w * nx;
+  if (r < rows) {
+    const I q = r / nx;
+    i = (int)(r - q * nx) + 1;
+    j = (int)(q < w ? q + 1 : ny - (2 * w - 1 - q));
+    return;
+  }
+  const I q = r - rows;
+  const I qq = q / (2 * w);
+  const I col = q - qq * (2 * w);
+  j = (int)(qq + w + 1);
+  i = (int)(col < w ? col + 1 : nx - (2 * w - 1 - col));
+}
+
 // Cell t (0-based) of the boundary shell of thickness w (2 for the phase
 // decoration, 1 for the solute faces), enumerated plane by plane: the w
 // planes at each z face are whole; every other plane contributes its w rows
 // at each y face (x-contiguous) and, if xf, the w cells at each x face of its
 // other rows (without them: the cells the fused sweeps write themselves).
-// Returns false when t is past the shell.
+// Returns false when t is past the shell.  In 32-bit unsigned arithmetic
+// whenever the shell has fewer than 2^31 cells (a 64-bit division is a
+// long software sequence on the GPU; the indices are the same either way).
 __host__ __device__ __forceinline__ bool shell_flat(long long t, int nx, int ny, int nz, int w, int &i,
                                            int &j, int &k, bool xf = true) {
-  const long long plane = (long long)nx * ny;
-  const long long full = (long long)w * plane;           // the w planes at one z face
-  const long long per = 2LL * w * nx + (xf ? 2LL * w * (ny - 2 * w) : 0LL);  // one middle plane
-  const long long mid = per * (nz - 2 * w);
-  if (t < 0 || t >= 2 * full + mid) return false;
-  long long r;
-  if (t < full || t >= full + mid) {
-    const long long u = t < full ? t : t - full - mid;
-    const int kk = (int)(u / plane);
-    k = t < full ? kk + 1 : nz - w + 1 + kk;
-    r = u - (long long)kk * plane;
-    i = (int)(r % nx) + 1;
-    j = (int)(r / nx) + 1;
-    return true;
-  }
-  const long long u = t - full;
-  const int km = (int)(u / per);
-  k = w + 1 + km;
-  r = u - (long long)km * per;
-  const long long rows = 2LL * w * nx;
-  if (r < rows) {
-    const int q = (int)(r / nx);
-    i = (int)(r % nx) + 1;
-    j = q < w ? q + 1 : ny - (2 * w - 1 - q);
-    return true;
-  }
-  const int q = (int)(r - rows);
-  const int col = q % (2 * w);
-  j = q / (2 * w) + w + 1;
-  i = col < w ? col + 1 : nx - (2 * w - 1 - col);
+  const long long total = shell_count(nx, ny, nz, w, xf);
+  if (t < 0 || t >= total) return false;
+  if (total < (1LL << 31))
+    shell_flat_in<uint32_t>((uint32_t)t, (uint32_t)nx, (uint32_t)ny, (uint32_t)nz, (uint32_t)w, i,
+                            j, k, xf);
+  else
+    shell_flat_in<long long>(t, nx, ny, nz, w, i, j, k, xf);
   return true;
-}
-
-__host__ __device__ inline long long shell_count(int nx, int ny, int nz, int w, bool xf = true) {
-  return 2LL * w * nx * ny + (2LL * w * nx + (xf ? 2LL * w * (ny - 2 * w) : 0LL)) * (nz - 2 * w);
 }
 
 // One cell of the phase decoration (equacao_campo_fase.f90:151-167 in

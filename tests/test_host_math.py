@@ -63,3 +63,20 @@ def test_ratio_log_accuracy(tmp_path):
     assert int(n) >= 1300000
     assert float(near_ulp) <= 4.0, r.stdout
     assert float(far_ulp1) <= 5.0, r.stdout
+
+
+@pytest.mark.skipif(not (os.path.exists(HIPCC) or shutil.which("hipcc")), reason="no hipcc")
+def test_shell_enumeration(tmp_path):
+    """pf::shell_flat, the boundary-shell enumeration of the decoration
+    kernels: 32-bit and 64-bit index arithmetic agree, and the cells are
+    exactly the shell, once each, on 6^3 ragged grids x 2 widths x 2
+    x-face options."""
+    exe = tmp_path / "shell_enum"
+    subprocess.run([HIPCC, "-x", "hip", "--cuda-host-only", "-O2", "-std=c++17", "-I", CSRC,
+                    "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "shell_enum.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    grids, cells = (int(v) for v in r.stdout.split()[:2])
+    assert grids == 6 ** 3 * 4 and cells > 0
