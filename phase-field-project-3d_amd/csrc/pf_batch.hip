@@ -1,18 +1,23 @@
 // pf_batch.hip -- the batched form: many independent instances of uneven
-// grid sizes advanced together, one launch per sweep for all of them.
+// grid sizes advanced together, one launch per sweep, precision and tile
+// shape for all of them.
 //
 // Every instance keeps its fields and its loop scalars (it, TP, Cs/Cl, M0,
 // the REAL(4) time, buffer roles, steps left) on the device.  A step is
 //   begin (k_batch_turn)   per instance: principal.f90:96-104 + the kernel
 //                          scalars of the sweeps (one thread per instance)
-//   (nv-1) x [sweep A, phase decoration]      one launch per precision and
-//   sweep B: phase + T, then the solute       tile shape (work-group -> tile
-//     with the phase decoration               table); a decoration launch
-//   solute faces, mass corrections due        covers both precisions
+// then, per precision (the FP32 chain on a second stream, forked and
+// joined within the step),
+//   (nv-1) x [sweep A, phase decoration]      one launch per tile shape
+//   sweep B: phase + T, then the solute       (work-group -> tile table)
+//     with the phase decoration
+//   solute faces (with a due mass correction's interior sums), and for the
+//   instances due, the correction's rescaling and face refresh
+// and
 //   end (k_batch_turn)     per instance: auxiliar M0 (principal.f90:157),
 //                          buffer roles, steps left
-// with the end of a step and the beginning of the next in one launch
-// so K steps need no host work and are captured once in a hipGraph that is
+// with the end of a step and the beginning of the next in one launch, so
+// K steps need no host work and are captured once in a hipGraph that is
 // replayed.  An instance that has run its steps goes inactive on the device
 // (its work-groups exit at once) until the host removes it.
 //
