@@ -829,11 +829,12 @@ int enqueue_step_body(pf_batch *b) {
   if (!both) return enqueue_chain(b, b->slot_of[0].empty() ? PF_FP32_MIXED : PF_FP64, b->stream);
   HIP_OK(hipEventRecord(b->ev_fork, b->stream));
   HIP_OK(hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
-  if ((rc = enqueue_chain(b, PF_FP64, b->stream)) != PF_OK) return rc;
-  if ((rc = enqueue_chain(b, PF_FP32_MIXED, b->stream2)) != PF_OK) return rc;
+  rc = enqueue_chain(b, PF_FP64, b->stream);
+  if (rc == PF_OK) rc = enqueue_chain(b, PF_FP32_MIXED, b->stream2);
+  // joined even after a failed launch, so that a graph capture can end
   HIP_OK(hipEventRecord(b->ev_join, b->stream2));
   HIP_OK(hipStreamWaitEvent(b->stream, b->ev_join, 0));
-  return PF_OK;
+  return rc;
 }
 
 // Enqueues k steps of every active instance (k >= 1); one launch ends a
