@@ -121,12 +121,10 @@ __device__ __forceinline__ void batch_begin(InstDev<R> &I, int *due) {
   if (due) *due = mass_due(I) ? 1 : 0;
 }
 
+// The end of a step after M0: buffer roles, steps left.
 template <typename R>
-__device__ __forceinline__ void batch_end(InstDev<R> &I) {
+__device__ __forceinline__ void batch_end_roles(InstDev<R> &I) {
   if (!I.active) return;
-  // auxiliar.f90 (principal.f90:157): M0 for the next step
-  I.k.M0 = auxiliar_m0(I.k.TP, I.k.Cl_eq, I.k.Cs_eq, I.k.R, I.k.Vm, I.k.E0, I.k.W0, I.Sigma,
-                       I.Beta, I.Ke, I.Me, I.k.D_liq, I.ajuste);
   I.po ^= (I.nv & 1);  // principal.f90:127-150, 200-226 as buffer roles
   I.co ^= 1;
   I.to ^= 1;
@@ -135,27 +133,46 @@ __device__ __forceinline__ void batch_end(InstDev<R> &I) {
   if (I.steps_left <= 0) I.active = 0;
 }
 
-// One thread per instance: the FP64 instances in the first b64 blocks of 64
-// threads, the FP32 ones in the blocks after (so that no wave runs both
-// precisions' code one after the other).  The end of one step and the
-// beginning of the next are one launch: end (when do_end), then begin (when
-// do_begin).
+// auxiliar's abscissae in constant memory (the lanes index them)
+__constant__ const M0Abscissae kM0Phi = m0_abscissae();
+
+// One 64-lane work-group per instance.  The end of one step (do_end) and
+// the beginning of the next (do_begin) are one launch.  The end's M0
+// (auxiliar.f90, principal.f90:157) evaluates the integral's terms one per
+// lane and sums them on lane 0 in auxiliar_m0's order (the same terms and
+// order, so the same bits): its chain of 79 divisions is the longest latency
+// of the step's end.
+template <typename R>
+__device__ __forceinline__ void batch_turn(InstDev<R> &I, int *due, int do_end, int do_begin,
+                                           double *terms) {
+  const int lane = threadIdx.x;
+  const bool m0 = do_end && I.active;  // uniform; read before lane 0 writes the record
+  if (m0)
+    for (int n = lane; n < kM0Terms; n += 64) terms[n] = m0_term(kM0Phi.v[n], I.k.Cl_eq, I.k.Cs_eq);
+  __syncthreads();
+  if (lane != 0) return;
+  if (m0) {
+    double acc = 0.0;
+    for (int n = 0; n < kM0Terms; ++n) acc = acc + terms[n];
+    I.k.M0 = m0_from_sum(acc, I.k.TP, I.k.Cl_eq, I.k.Cs_eq, I.k.R, I.k.Vm, I.k.E0, I.k.W0,
+                         I.Sigma, I.Beta, I.Ke, I.Me, I.k.D_liq, I.ajuste);
+  }
+  if (do_end) batch_end_roles<R>(I);
+  if (do_begin) batch_begin<R>(I, due);
+}
+
+// Work-group b < n64: FP64 instance b; the rest: the FP32 instances.
 __global__ void __launch_bounds__(64)
     k_batch_turn(InstDev<double> *__restrict__ inst64, int n64, int *__restrict__ due64,
-                 InstDev<float> *__restrict__ inst32, int n32, int *__restrict__ due32, int b64,
+                 InstDev<float> *__restrict__ inst32, int n32, int *__restrict__ due32,
                  int do_end, int do_begin) {
-  const int blk = blockIdx.x;
-  if (blk < b64) {
-    const int q = blk * 64 + threadIdx.x;
-    if (q >= n64) return;
-    if (do_end) batch_end<double>(inst64[q]);
-    if (do_begin) batch_begin<double>(inst64[q], due64 ? due64 + q : nullptr);
-  } else {
-    const int q = (blk - b64) * 64 + threadIdx.x;
-    if (q >= n32) return;
-    if (do_end) batch_end<float>(inst32[q]);
-    if (do_begin) batch_begin<float>(inst32[q], due32 ? due32 + q : nullptr);
-  }
+  __shared__ double terms[kM0Terms];
+  const int q = blockIdx.x;
+  if (q < n64)
+    batch_turn<double>(inst64[q], due64 ? due64 + q : nullptr, do_end, do_begin, terms);
+  else if (q - n64 < n32)
+    batch_turn<float>(inst32[q - n64], due32 ? due32 + (q - n64) : nullptr, do_end, do_begin,
+                      terms);
 }
 
 // ---- batched sweeps and decorations ----------------------------------------
@@ -844,11 +861,9 @@ int enqueue_steps(pf_batch *b, int k) {
   auto *i64 = (pf::InstDev<double> *)b->inst_dev[0];
   auto *i32 = (pf::InstDev<float> *)b->inst_dev[1];
   const int n64 = (int)b->slot_of[0].size(), n32 = (int)b->slot_of[1].size();
-  const int b64 = (n64 + 63) / 64;
-  const unsigned nbi = (unsigned)(b64 + (n32 + 63) / 64);
   auto turn = [&](int do_end, int do_begin) -> int {
-    hipLaunchKernelGGL(pf::k_batch_turn, dim3(nbi), dim3(64), 0, st, i64, n64, b->mass_due[0],
-                       i32, n32, b->mass_due[1], b64, do_end, do_begin);
+    hipLaunchKernelGGL(pf::k_batch_turn, dim3((unsigned)(n64 + n32)), dim3(64), 0, st, i64, n64,
+                       b->mass_due[0], i32, n32, b->mass_due[1], do_end, do_begin);
     HIP_OK(hipGetLastError());
     return PF_OK;
   };

@@ -116,12 +116,53 @@ constexpr int m0_terms() {
 constexpr int kM0Terms = m0_terms();
 static_assert(kM0Terms == 79, "auxiliar's integral has 79 terms");
 
+// The integral's abscissae: phi_i from 0.11 in rounded steps of 0.01, the
+// same sequential additions as auxiliar.f90's loop, evaluated at compile
+// time (IEEE double, round to nearest, as at run time).
+struct M0Abscissae {
+  double v[kM0Terms];
+};
+constexpr M0Abscissae m0_abscissae() {
+  M0Abscissae t{};
+  double phi_i = 0.11;
+  for (int n = 0; n < kM0Terms; ++n) {
+    t.v[n] = phi_i;
+    phi_i = phi_i + 0.01;
+  }
+  return t;
+}
+
+// source/auxiliar.f90:14-29: one term of the integral's sum at phi_i.
+PF_HD double m0_term(double phi_i, double Cl_eq, double Cs_eq) {
+#pragma clang fp contract(off)
+  const double h = phi_i * phi_i * (3.0 - 2.0 * phi_i);
+  const double f1 = phi_i * (1.0 - phi_i);
+  const double num = h * (1.0 - h);
+  const double den = ((1.0 - h) * Cl_eq * (1.0 - Cl_eq) + h * Cs_eq * (1.0 - Cs_eq)) * f1;
+  return num / den;
+}
+
+// source/auxiliar.f90:31-42: M0 from the sum of the terms (acc, summed in
+// the loop's order).
+PF_HD double m0_from_sum(double acc, double TP, double Cl_eq, double Cs_eq, double R, double Vm,
+                         double E0, double W0, float Sigma, float Beta, float Ke, double Me,
+                         double D_liq, float ajuste) {
+#pragma clang fp contract(off)
+  const double integral = ((double)0.01f / 2.0) * ((double)54.39180136f + 2.0 * acc + 234.073053);
+  const double qsi = R * TP * (1.0 / Vm) * (Cl_eq - Cs_eq) * (Cl_eq - Cs_eq) * integral;
+  const float omk = 1.0f - Ke;
+  const double inv = (E0 * E0 / (double)Sigma) *
+                     (((double)Beta * R * TP * (double)omk / (Vm * Me)) +
+                      (E0 * qsi / (D_liq * sqrt(2.0 * W0))));
+  return (double)ajuste * 1.0 / inv;
+}
+
 // source/auxiliar.f90:14-42: trapezoid rule over phi_i = 0.11..0.89 with the
 // hard-coded end values, then 1/M0 from the thin-interface relation, in the
 // reference's operation order with its REAL(4) literals and (1-Ke).  Shared
-// by the host loop and the batched form's per-instance device update, so
-// both give the same bits.
-
+// by the host loop and the batched form's per-instance device update (which
+// evaluates the terms in parallel, one lane each, and sums them in this
+// order), so both give the same bits.
 PF_HD double auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double Vm, double E0,
                          double W0, float Sigma, float Beta, float Ke, double Me, double D_liq,
                          float ajuste) {
@@ -131,20 +172,10 @@ PF_HD double auxiliar_m0(double TP, double Cl_eq, double Cs_eq, double R, double
   // terms are independent; the sum keeps its order)
 #pragma unroll 8
   for (int n = 0; n < kM0Terms; ++n) {
-    const double h = phi_i * phi_i * (3.0 - 2.0 * phi_i);
-    const double f1 = phi_i * (1.0 - phi_i);
-    const double num = h * (1.0 - h);
-    const double den = ((1.0 - h) * Cl_eq * (1.0 - Cl_eq) + h * Cs_eq * (1.0 - Cs_eq)) * f1;
-    acc = acc + num / den;
+    acc = acc + m0_term(phi_i, Cl_eq, Cs_eq);
     phi_i = phi_i + 0.01;
   }
-  const double integral = ((double)0.01f / 2.0) * ((double)54.39180136f + 2.0 * acc + 234.073053);
-  const double qsi = R * TP * (1.0 / Vm) * (Cl_eq - Cs_eq) * (Cl_eq - Cs_eq) * integral;
-  const float omk = 1.0f - Ke;
-  const double inv = (E0 * E0 / (double)Sigma) *
-                     (((double)Beta * R * TP * (double)omk / (Vm * Me)) +
-                      (E0 * qsi / (D_liq * sqrt(2.0 * W0))));
-  return (double)ajuste * 1.0 / inv;
+  return m0_from_sum(acc, TP, Cl_eq, Cs_eq, R, Vm, E0, W0, Sigma, Beta, Ke, Me, D_liq, ajuste);
 }
 
 PF_HD size_t cidx(int i, int j, int k, int nx, int ny) {
