@@ -640,11 +640,19 @@ __device__ __forceinline__ void march_tile_body(const Geo &g, const KParams &kp,
       // registers beside them (phase_body.inc:181-189; the draw depends on
       // the step, not the sweep -- passing it from the first sweep to the
       // last through HBM measured slower than hashing again)
+      // The draw enters only as amp 16 phi^2 (1-phi)^2: where phi^2 or
+      // (1-phi)^2 is zero (the bulk liquid and solid, phi exactly 0 or 1)
+      // the term is +-0 whatever the draw, and adding it changes no sum (at
+      // most the sign of a zero phi).  A wave with no other cell skips the
+      // hash.
       float ran = 0.f;
-      if (noise)
+      if (noise && __builtin_amdgcn_ballot_w64(a2 * a2 != (R)0 &&
+                                               ((R)1 - a2) * ((R)1 - a2) != (R)0) != 0) {
+        __builtin_amdgcn_sched_barrier(0);  // keeps the branch (not select)
         ran = (VAR & kVarSmooth) && kp.ruido_tipo == PF_RUIDO_SMOOTH
                   ? (float)ruido_suave(i, j, k, kp.it, kp.seed)
                   : (float)hash_unit(hbase);
+      }
       const R *Pm = slotr(-3), *P0 = slotr(-2), *Pp = slotr(-1);  // planes k-1, k, k+1
       const int c = (ty + 2) * PX + (tx + 2);
       const R PP = a2;  // planes k, k+1, k-1 of the own column
