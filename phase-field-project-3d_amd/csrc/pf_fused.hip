@@ -506,30 +506,39 @@ int autotune_impl(FusedState &S, const KParams &k, void *phi_old, void *c_old, v
     ms /= (float)nrep;
     return e;
   };
-  float best_seq = 1e30f;
+  // Each sequence is timed in two rounds over all of them and keeps its
+  // faster time, so that a drift of the clocks during the comparison does
+  // not decide between sequences within a percent of each other.
+  struct Seq {
+    const Cand *a, *pt, *sl, *fb;
+    float ms;
+  };
+  std::vector<Seq> seqs;
   for (const Cand &a : top2[0]) {
-    if (rc != PF_OK) break;
     for (const Cand &pt : top2[2])
-      for (const Cand &sl : top2[3]) {
-        float ms = 0.f;
-        if (rc == PF_OK && time_seq(a, &pt, &sl, nullptr, 3 * reps, ms) != hipSuccess) rc = PF_E_HIP;
-        if (rc == PF_OK && ms < best_seq) {
-          best_seq = ms;
-          S.split_b = true;
-          fused_set_kind(S, 0, a.tile, a.target);
-          fused_set_kind(S, 1, pt.tile, pt.target);
-          fused_set_kind(S, 2, sl.tile, sl.target);
-        }
-      }
-    for (const Cand &fb : top2[1]) {
+      for (const Cand &sl : top2[3]) seqs.push_back({&a, &pt, &sl, nullptr, 1e30f});
+    for (const Cand &fb : top2[1]) seqs.push_back({&a, nullptr, nullptr, &fb, 1e30f});
+  }
+  for (int round = 0; round < 2 && rc == PF_OK; ++round)
+    for (Seq &q : seqs) {
       float ms = 0.f;
-      if (rc == PF_OK && time_seq(a, nullptr, nullptr, &fb, 3 * reps, ms) != hipSuccess) rc = PF_E_HIP;
-      if (rc == PF_OK && ms < best_seq) {
-        best_seq = ms;
-        S.split_b = false;
-        fused_set_kind(S, 0, a.tile, a.target);
-        fused_set_kind(S, 1, fb.tile, fb.target);
+      if (time_seq(*q.a, q.pt, q.sl, q.fb, 3 * reps, ms) != hipSuccess) {
+        rc = PF_E_HIP;
+        break;
       }
+      q.ms = std::min(q.ms, ms);
+    }
+  float best_seq = 1e30f;
+  for (const Seq &q : seqs) {
+    if (rc != PF_OK || !(q.ms < best_seq)) continue;
+    best_seq = q.ms;
+    S.split_b = q.pt != nullptr;
+    fused_set_kind(S, 0, q.a->tile, q.a->target);
+    if (q.pt) {
+      fused_set_kind(S, 1, q.pt->tile, q.pt->target);
+      fused_set_kind(S, 2, q.sl->tile, q.sl->target);
+    } else {
+      fused_set_kind(S, 1, q.fb->tile, q.fb->target);
     }
   }
   if (res) {
