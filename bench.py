@@ -618,9 +618,10 @@ def main(argv=None):
                      "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x "
                                      "1.0, factors from a copy of known size)",
                      "traffic_kernel": traffic_key(r64, rl["kernel"], "fp64"),
-                     "limiter": "VALU issue and latency: 226 (sweep A) and 239 (phase + T) "
-                                "VALU and 63-67 scalar instructions per cell, VALU issuing "
-                                "55-65% of the time, 3 waves/SIMD at 167-168 VGPRs (FP64 "
+                     "limiter": "VALU issue and latency: 208-226 (sweep A) and 219-239 "
+                                "(phase + T) VALU (waves without / with the noise hash) and "
+                                "63-66 scalar instructions per cell, VALU issuing 56-61% of "
+                                "the time, 3 waves/SIMD at 167-168 VGPRs (FP64 "
                                 "register demand; a fourth wave spills ~40 VGPRs)",
                      "kernel": rl["kernel"], "avg_ms": round(rl["avg_ms"], 4),
                      "alg_bytes_per_launch": rl["alg_bytes"]},
